@@ -1,0 +1,953 @@
+/*
+ * blaref.c -- CPU restatement (FP64) of the Simson `bla` time step for the
+ * temporal plane channel.  TEST INFRASTRUCTURE ONLY (see blaref.h): it is
+ * the oracle the HIP step is checked against and is never on the product
+ * path.
+ *
+ * Written from the mathematics of the reference routines; each function
+ * names the reference lines whose behaviour it restates.  Transforms are
+ * plain DFTs (mixed-radix recursion) and direct Chebyshev sums, not FFTPACK,
+ * so agreement with the reference is to rounding, not bitwise.
+ */
+#include "blaref.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef double complex cplx;
+
+#define PI 3.14159265358979323846
+
+/* ------------------------------------------------------------------ */
+/* DFT: y_k = sum_j x_j exp(sign*2*pi*i*j*k/n), n = 2^a 3^b 5^c ...   */
+/* ------------------------------------------------------------------ */
+
+static int smallest_factor(int n)
+{
+    for (int p = 2; p * p <= n; p++)
+        if (n % p == 0)
+            return p;
+    return n;
+}
+
+static cplx *roots_table(int n)
+{
+    cplx *w = malloc(sizeof(cplx) * (size_t)n);
+    for (int t = 0; t < n; t++) {
+        /* reduce the angle to keep the table accurate near pi/2, pi ... */
+        long num = 4L * t, q = num / n, r = num % n;  /* 2 pi t/n = q*pi/2 + 2 pi r/(4n) */
+        double a = 2.0 * PI * (double)r / (4.0 * (double)n);
+        double c = cos(a), s = sin(a);
+        switch (q & 3) {
+        case 0: w[t] = c + I * s; break;
+        case 1: w[t] = -s + I * c; break;
+        case 2: w[t] = -c - I * s; break;
+        default: w[t] = s - I * c; break;
+        }
+    }
+    return w;
+}
+
+/* recursive mixed-radix: x strided by s, y contiguous (length n) */
+static void dft_rec(int n, const cplx *x, int s, cplx *y, int sign, int N,
+                    const cplx *W)
+{
+    if (n == 1) {
+        y[0] = x[0];
+        return;
+    }
+    int p = smallest_factor(n), m = n / p, step = N / n;
+    for (int r = 0; r < p; r++)
+        dft_rec(m, x + (size_t)r * s, s * p, y + (size_t)r * m, sign, N, W);
+    cplx in[64], out[64];
+    if (p > 64)
+        abort();
+    for (int k = 0; k < m; k++) {
+        for (int r = 0; r < p; r++)
+            in[r] = y[(size_t)r * m + k];
+        for (int q = 0; q < p; q++) {
+            cplx acc = 0;
+            for (int r = 0; r < p; r++) {
+                long e = ((long)(k + q * m) * r) % n;
+                cplx w = W[(e * step) % N];
+                acc += in[r] * (sign > 0 ? w : conj(w));
+            }
+            out[q] = acc;
+        }
+        for (int q = 0; q < p; q++)
+            y[k + (size_t)q * m] = out[q];
+    }
+}
+
+typedef struct {
+    int n;
+    cplx *w;
+    cplx *buf;
+} dftplan;
+
+static void plan_init(dftplan *pl, int n)
+{
+    pl->n = n;
+    pl->w = roots_table(n);
+    pl->buf = NULL;
+}
+
+static void plan_free(dftplan *pl)
+{
+    free(pl->w);
+}
+
+/* in place on contiguous x with scratch y (length n) */
+static void dft_inplace(const dftplan *pl, cplx *x, cplx *y, int sign)
+{
+    dft_rec(pl->n, x, 1, y, sign, pl->n, pl->w);
+    memcpy(x, y, sizeof(cplx) * (size_t)pl->n);
+}
+
+void orc_cdft(int n, double *re, double *im, int sign)
+{
+    dftplan pl;
+    plan_init(&pl, n);
+    cplx *x = malloc(sizeof(cplx) * (size_t)n), *y = malloc(sizeof(cplx) * (size_t)n);
+    for (int j = 0; j < n; j++)
+        x[j] = re[j] + I * im[j];
+    dft_inplace(&pl, x, y, sign);
+    for (int j = 0; j < n; j++) {
+        re[j] = creal(x[j]);
+        im[j] = cimag(x[j]);
+    }
+    free(x);
+    free(y);
+    plan_free(&pl);
+}
+
+/* vrfftb (vecfft.txt:214-280): Hermitian half spectrum -> real signal.
+ * Only the real parts of X_0 and X_{n/2} enter. */
+static void rbwd(const dftplan *pl, const cplx *X, double *x, cplx *t1, cplx *t2)
+{
+    int n = pl->n, h = n / 2;
+    t1[0] = creal(X[0]);
+    t1[h] = creal(X[h]);
+    for (int k = 1; k < h; k++) {
+        t1[k] = X[k];
+        t1[n - k] = conj(X[k]);
+    }
+    dft_inplace(pl, t1, t2, +1);
+    for (int m = 0; m < n; m++)
+        x[m] = creal(t1[m]);
+}
+
+/* vrfftf (vecfft.txt:134-170): real signal -> X_0..X_{n/2} */
+static void rfwd(const dftplan *pl, const double *x, cplx *X, cplx *t1, cplx *t2)
+{
+    int n = pl->n, h = n / 2;
+    for (int m = 0; m < n; m++)
+        t1[m] = x[m];
+    dft_inplace(pl, t1, t2, -1);
+    for (int k = 0; k <= h; k++)
+        X[k] = t1[k];
+    X[0] = creal(X[0]);
+    X[h] = creal(X[h]);
+}
+
+void orc_rdft_bwd(int n, const double *xr, const double *xi, double *x)
+{
+    dftplan pl;
+    plan_init(&pl, n);
+    cplx *X = malloc(sizeof(cplx) * (size_t)(n / 2 + 1));
+    cplx *t1 = malloc(sizeof(cplx) * (size_t)n), *t2 = malloc(sizeof(cplx) * (size_t)n);
+    for (int k = 0; k <= n / 2; k++)
+        X[k] = xr[k] + I * xi[k];
+    rbwd(&pl, X, x, t1, t2);
+    free(X);
+    free(t1);
+    free(t2);
+    plan_free(&pl);
+}
+
+void orc_rdft_fwd(int n, const double *x, double *xr, double *xi)
+{
+    dftplan pl;
+    plan_init(&pl, n);
+    cplx *X = malloc(sizeof(cplx) * (size_t)(n / 2 + 1));
+    cplx *t1 = malloc(sizeof(cplx) * (size_t)n), *t2 = malloc(sizeof(cplx) * (size_t)n);
+    rfwd(&pl, x, X, t1, t2);
+    for (int k = 0; k <= n / 2; k++) {
+        xr[k] = creal(X[k]);
+        xi[k] = cimag(X[k]);
+    }
+    free(X);
+    free(t1);
+    free(t2);
+    plan_free(&pl);
+}
+
+/* ------------------------------------------------------------------ */
+/* Chebyshev operations                                                */
+/* ------------------------------------------------------------------ */
+
+/* cos(pi*m/(n-1)) for m in [0, 2(n-1)) */
+static double *cheb_cos_table(int n)
+{
+    int M = 2 * (n - 1);
+    double *c = malloc(sizeof(double) * (size_t)M);
+    cplx *w = roots_table(M);      /* exp(2 pi i m / M) = exp(i pi m/(n-1)) */
+    for (int m = 0; m < M; m++)
+        c[m] = creal(w[m]);
+    free(w);
+    return c;
+}
+
+/* vchbf (vecfft.txt:554-561): X_i = sum_k c_k c_i x_k cos(k i pi/(n-1)),
+ * c_0 = c_{n-1} = 1/2; multiplied by `scale`. */
+static void chbf_tab(int n, const double *ct, const double *x, double *out, double scale)
+{
+    int M = 2 * (n - 1);
+    for (int i = 0; i < n; i++) {
+        double acc = 0.5 * (x[0] + ((i & 1) ? -x[n - 1] : x[n - 1]));
+        for (int k = 1; k < n - 1; k++)
+            acc += x[k] * ct[(long)k * i % M];
+        if (i == 0 || i == n - 1)
+            acc *= 0.5;
+        out[i] = acc * scale;
+    }
+}
+
+/* vchbb (vecfft.txt:626-631): x_i = sum_k X_k cos(k i pi/(n-1)) */
+static void chbb_tab(int n, const double *ct, const double *X, double *out)
+{
+    int M = 2 * (n - 1);
+    for (int i = 0; i < n; i++) {
+        double acc = 0.0;
+        for (int k = 0; k < n; k++)
+            acc += X[k] * ct[(long)k * i % M];
+        out[i] = acc;
+    }
+}
+
+void orc_chbf(int n, const double *x, double *out)
+{
+    double *ct = cheb_cos_table(n);
+    chbf_tab(n, ct, x, out, 1.0);
+    free(ct);
+}
+
+void orc_chbb(int n, const double *x, double *out)
+{
+    double *ct = cheb_cos_table(n);
+    chbb_tab(n, ct, x, out);
+    free(ct);
+}
+
+/* dcheb.f:24-36: coefficients of d/dy.  b_{n-1} = 0, b_{n-2} = 2(n-1)a_{n-1},
+ * b_k = b_{k+2} + 2(k+1) a_{k+1}, then b_0 halved (c_0 = 2). */
+static void dcheb(int n, const double *a, double *b)
+{
+    double bk2 = 0.0, bk1 = 0.0;      /* b_{k+2}, b_{k+1} */
+    b[n - 1] = 0.0;
+    for (int k = n - 2; k >= 0; k--) {
+        double bk = bk2 + 2.0 * (double)(k + 1) * a[k + 1];
+        b[k] = bk;
+        bk2 = bk1;
+        bk1 = bk;
+    }
+    b[0] *= 0.5;
+}
+
+void orc_dcheb(int n, const double *a, double *b)
+{
+    dcheb(n, a, b);
+}
+
+/* icheb.f:24-36: integral with b_0 = intcon. */
+void orc_icheb(int n, const double *a, double *b, double intcon)
+{
+    for (int i = 2; i < n - 1; i++)
+        b[i] = (a[i - 1] - a[i + 1]) * (0.5 / (double)i);
+    b[n - 1] = a[n - 2] * (0.5 / (double)(n - 1));
+    b[1] = a[0] - 0.5 * a[2];
+    b[0] = intcon;
+}
+
+/* Chebyshev tau system for u'' - lam u = f (setmacrhs.f:23-74 builds it,
+ * trid.f:120-199 solves it).  Unknowns a_0..a_{n-1}.  Rows:
+ *   sum_{k even} a_k = bc_e,  sum_{k odd} a_k = bc_o,
+ *   for k = 2..n-1 (g = f + lam*a, the u'' coefficients, truncated to
+ *   degree n-3):
+ *     a_k = c_{k-2} g_{k-2}/(4k(k-1)) - [k<=n-3] g_k/(2(k^2-1))
+ *           + [k<=n-5] g_{k+2}/(4k(k+1)),   c_0 = 2, c_j = 1.
+ * Each parity class is a tridiagonal chain plus one full row; it is
+ * reduced bottom-up to a_k = rho_k - gam_k a_{k-2}, closed with the BC row
+ * and substituted forward. */
+static void tau_solve(int n, double lam, const double *f, double bce, double bco,
+                      double *a, double *gam, double *rho)
+{
+    for (int par = 0; par < 2; par++) {
+        int k0 = par;
+        int klast = (n - 1 - par) % 2 == 0 ? n - 1 : n - 2;
+        /* bottom-up elimination over k = klast, klast-2, ..., k0+2 */
+        double g_up = 0.0, r_up = 0.0;    /* gam, rho of row k+2 */
+        for (int k = klast; k >= k0 + 2; k -= 2) {
+            double kk = (double)k;
+            double ckm2 = (k - 2 == 0) ? 2.0 : 1.0;
+            double L = -ckm2 * lam / (4.0 * kk * (kk - 1.0));
+            double D = 1.0 + (k <= n - 3 ? lam / (2.0 * (kk * kk - 1.0)) : 0.0);
+            double U = (k <= n - 5) ? -lam / (4.0 * kk * (kk + 1.0)) : 0.0;
+            double R = ckm2 * f[k - 2] / (4.0 * kk * (kk - 1.0));
+            if (k <= n - 3)
+                R -= f[k] / (2.0 * (kk * kk - 1.0));
+            if (k <= n - 5)
+                R += f[k + 2] / (4.0 * kk * (kk + 1.0));
+            double piv = D - U * g_up;
+            double gk = L / piv;
+            double rk = (R - U * r_up) / piv;
+            gam[k] = gk;
+            rho[k] = rk;
+            g_up = gk;
+            r_up = rk;
+        }
+        /* a_k = P_k + Q_k a_{k0}; accumulate the BC row */
+        double P = 0.0, Q = 1.0, sP = 0.0, sQ = 1.0;
+        for (int k = k0 + 2; k <= klast; k += 2) {
+            P = rho[k] - gam[k] * P;
+            Q = -gam[k] * Q;
+            sP += P;
+            sQ += Q;
+        }
+        double bc = par == 0 ? bce : bco;
+        double a0 = (bc - sP) / sQ;
+        a[k0] = a0;
+        for (int k = k0 + 2; k <= klast; k += 2)
+            a[k] = rho[k] - gam[k] * a[k - 2];
+    }
+}
+
+void orc_tau_solve(int n, double lam, const double *f, double bc_e, double bc_o,
+                   double *u)
+{
+    double *g = malloc(sizeof(double) * (size_t)n), *r = malloc(sizeof(double) * (size_t)n);
+    tau_solve(n, lam, f, bc_e, bc_o, u, g, r);
+    free(g);
+    free(r);
+}
+
+/* ------------------------------------------------------------------ */
+/* State                                                               */
+/* ------------------------------------------------------------------ */
+
+struct orc {
+    int nx, ny, nz, nxh, nxp, nzp;
+    double re, xl, zl, rot;
+    int cflux;
+    double mflux, px;
+    double u0low, u0upp, w0low, w0upp;
+    double *alfa, *beta, *eta, *deta, *ct;
+    dftplan px_plan, pz_plan;
+};
+
+orc_t *orc_create(int nx, int ny, int nz, double re, double xl, double zl,
+                  double rot, int cflux, double retau)
+{
+    orc_t *o = calloc(1, sizeof(orc_t));
+    o->nx = nx;
+    o->ny = ny;
+    o->nz = nz;
+    o->nxh = nx / 2;
+    o->nxp = nx + nx / 2;           /* par.f:86, nfxd = 1 */
+    o->nzp = nz + nz / 2;           /* nfzd = 1 */
+    o->re = re;
+    o->xl = xl;
+    o->zl = zl;
+    o->rot = rot;
+    o->cflux = cflux;
+    if (cflux) {                    /* bla.f:1629-1630 */
+        o->mflux = 4. / 3.;
+        o->px = -2. / re;
+    } else {                        /* bla.f:1645 */
+        o->mflux = 0.;
+        o->px = -(retau / re) * (retau / re);
+    }
+    /* preprbl.f:34-55 */
+    o->alfa = malloc(sizeof(double) * (size_t)o->nxh);
+    for (int x = 0; x < o->nxh; x++)
+        o->alfa[x] = 2. * PI / xl * (double)x;
+    o->beta = malloc(sizeof(double) * (size_t)nz);
+    o->beta[0] = 0.;
+    for (int z = 1; z <= nz / 2; z++) {
+        o->beta[z] = 2. * PI / zl * (double)z;
+        o->beta[nz - z] = -2. * PI / zl * (double)z;
+    }
+    o->eta = malloc(sizeof(double) * (size_t)ny);
+    o->ct = cheb_cos_table(ny);
+    for (int y = 0; y < ny; y++)
+        o->eta[y] = o->ct[y];
+    o->deta = malloc(sizeof(double) * (size_t)ny);
+    o->deta[0] = 1. - o->eta[1];
+    o->deta[ny - 1] = o->deta[0];
+    for (int y = 1; y < ny - 1; y++)
+        o->deta[y] = (o->eta[y - 1] - o->eta[y + 1]) * .5;
+    plan_init(&o->px_plan, o->nxp);
+    plan_init(&o->pz_plan, o->nzp);
+    return o;
+}
+
+void orc_destroy(orc_t *o)
+{
+    if (!o)
+        return;
+    free(o->alfa);
+    free(o->beta);
+    free(o->eta);
+    free(o->deta);
+    free(o->ct);
+    plan_free(&o->px_plan);
+    plan_free(&o->pz_plan);
+    free(o);
+}
+
+double orc_get_px(const orc_t *o) { return o->px; }
+void orc_set_px(orc_t *o, double px) { o->px = px; }
+
+#define IDX(o, s, z, y, x) ((((size_t)(s) * (o)->nz + (z)) * (o)->ny + (y)) * (o)->nxh + (x))
+
+/* ------------------------------------------------------------------ */
+/* Step 2: nonlinear terms per y plane (nonlinbl.f)                    */
+/* ------------------------------------------------------------------ */
+
+/* storage z index -> row of the padded nzp grid (getxz.f:41-61) */
+static int zrow(const orc_t *o, int z)
+{
+    return z < o->nz / 2 ? z : z + o->nzp - o->nz;
+}
+
+static double nonlin_plane(orc_t *o, double *ur, double *ui, int y, int do_cfl)
+{
+    int nxh = o->nxh, nz = o->nz, nxp = o->nxp, nzp = o->nzp, hx = nxp / 2 + 1;
+    size_t box = (size_t)nzp * hx;
+    cplx *B = calloc(6 * box, sizeof(cplx));          /* [f][zp][kx] */
+    double *R = malloc(sizeof(double) * 6 * (size_t)nzp * nxp);   /* [f][zp][xp] */
+    int nmax = nxp > nzp ? nxp : nzp;
+    cplx *t1 = malloc(sizeof(cplx) * (size_t)nmax), *t2 = malloc(sizeof(cplx) * (size_t)nmax);
+    cplx *col = malloc(sizeof(cplx) * (size_t)nzp);
+    /* fields in padded box: 0 u, 1 v, 2 w, 3 om_x, 4 om_y, 5 om_z
+     * (getxz with ipad=1, nonlinbl.f:229-247); the storage oddball
+     * z = nz/2 lands in the zeroed band and is dropped. */
+    static const int slot_of[6] = {0, 1, 2, 3, -1, 4};
+    for (int f = 0; f < 6; f++) {
+        if (slot_of[f] < 0)
+            continue;
+        for (int z = 0; z < nz; z++) {
+            if (z == nz / 2)
+                continue;
+            int zp = zrow(o, z);
+            for (int x = 0; x < nxh; x++) {
+                size_t s = IDX(o, slot_of[f], z, y, x);
+                B[f * box + (size_t)zp * hx + x] = ur[s] + I * ui[s];
+            }
+        }
+    }
+    /* omega_y = i beta u - i alpha w  (nonlinbl.f:308-355) */
+    for (int z = 0; z < nz; z++) {
+        if (z == nz / 2)
+            continue;
+        int zp = zrow(o, z);
+        for (int x = 0; x < nxh; x++) {
+            cplx uu = B[0 * box + (size_t)zp * hx + x], ww = B[2 * box + (size_t)zp * hx + x];
+            B[4 * box + (size_t)zp * hx + x] = I * o->beta[z] * uu - I * o->alfa[x] * ww;
+        }
+    }
+    /* Fourier synthesis (fft2db.f): z complex over the nx/2 columns, then
+     * x half-complex -> real for every row. */
+    for (int f = 0; f < 6; f++) {
+        for (int x = 0; x < nxh; x++) {
+            for (int zp = 0; zp < nzp; zp++)
+                col[zp] = B[f * box + (size_t)zp * hx + x];
+            dft_inplace(&o->pz_plan, col, t1, +1);
+            for (int zp = 0; zp < nzp; zp++)
+                B[f * box + (size_t)zp * hx + x] = col[zp];
+        }
+        for (int zp = 0; zp < nzp; zp++)
+            rbwd(&o->px_plan, B + f * box + (size_t)zp * hx,
+                 R + ((size_t)f * nzp + zp) * nxp, t1, t2);
+    }
+    /* CFL (boxcfl.f:29-76) */
+    double cflp = 0.;
+    if (do_cfl) {
+        double dyinv = 1. / o->deta[y];      /* (ny-1)/(nyp-1) = 1 */
+        if (o->ny >= 17) {
+            double cap = (double)(o->ny - 1) * (double)(o->ny - 1) / 10. / 3.14;
+            if (cap < dyinv)
+                dyinv = cap;
+        }
+        double dxi = (double)(o->nx - 2) / o->xl;
+        double dzi = (double)(o->nz - 2) / o->zl;
+        for (size_t p = 0; p < (size_t)nzp * nxp; p++) {
+            double c = fabs(R[p]) * dxi + fabs(R[(size_t)nzp * nxp + p]) * dyinv +
+                       fabs(R[2 * (size_t)nzp * nxp + p]) * dzi;
+            if (c > cflp)
+                cflp = c;
+        }
+    }
+    /* H = u x (omega + 2 Omega e_z) in rotational form (advection.f:52-72) */
+    size_t np = (size_t)nzp * nxp;
+    for (size_t p = 0; p < np; p++) {
+        double u = R[p], v = R[np + p], w = R[2 * np + p];
+        double ox = R[3 * np + p], oy = R[4 * np + p], oz = R[5 * np + p] + 2. * o->rot;
+        double h1 = v * oz - w * oy;
+        double h2 = w * ox - u * oz;
+        double h3 = u * oy - v * ox;
+        R[p] = h1;
+        R[np + p] = h2;
+        R[2 * np + p] = h3;
+    }
+    /* Fourier analysis (fft2df.f) and truncation into slots 0-2 (putxz.f,
+     * including the z oddball row, which lands in storage z = nz/2). */
+    for (int f = 0; f < 3; f++) {
+        for (int zp = 0; zp < nzp; zp++)
+            rfwd(&o->px_plan, R + ((size_t)f * nzp + zp) * nxp,
+                 B + f * box + (size_t)zp * hx, t1, t2);
+        for (int x = 0; x < nxh; x++) {
+            for (int zp = 0; zp < nzp; zp++)
+                col[zp] = B[f * box + (size_t)zp * hx + x];
+            dft_inplace(&o->pz_plan, col, t1, -1);
+            for (int z = 0; z < nz; z++) {
+                int zp = zrow(o, z);
+                size_t s = IDX(o, f, z, y, x);
+                ur[s] = creal(col[zp]);
+                ui[s] = cimag(col[zp]);
+            }
+        }
+    }
+    free(B);
+    free(R);
+    free(t1);
+    free(t2);
+    free(col);
+    return cflp;
+}
+
+double orc_nonlin(orc_t *o, double *ur, double *ui, int do_cfl)
+{
+    double cfl = 0.;
+#pragma omp parallel for schedule(dynamic) reduction(max : cfl)
+    for (int y = 0; y < o->ny; y++) {
+        double c = nonlin_plane(o, ur, ui, y, do_cfl);
+        if (c > cfl)
+            cfl = c;
+    }
+    return cfl;
+}
+
+/* ------------------------------------------------------------------ */
+/* Step 3: linear solve per z plane (linearbl.f, tau method, ibc = 0)  */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    double *v[48];
+    double *gam, *rho;
+} work_t;
+
+static void work_alloc(work_t *w, int n)
+{
+    for (int i = 0; i < 48; i++)
+        w->v[i] = calloc((size_t)n, sizeof(double));
+    w->gam = calloc((size_t)n, sizeof(double));
+    w->rho = calloc((size_t)n, sizeof(double));
+}
+
+static void work_free(work_t *w)
+{
+    for (int i = 0; i < 48; i++)
+        free(w->v[i]);
+    free(w->gam);
+    free(w->rho);
+}
+
+static double mass_flux(int n, const double *u)
+{
+    /* linearbl.f:1367-1370: integral over [-1,1] of sum u_k T_k */
+    double m = 0.;
+    for (int k = 0; k < n; k += 2)
+        m -= u[k] * 2. / (double)(k * k - 1);
+    return m;
+}
+
+static void linear_plane(orc_t *o, double *ur, double *ui, double *puw, int z,
+                         double an, double bn, double anp1, double bnp1, work_t *W)
+{
+    int n = o->ny, nxh = o->nxh;
+    double re = o->re;
+    double cn = 2. / ((double)o->nxp * (double)(n - 1) * (double)o->nzp);
+    double beta = o->beta[z];
+    double c1 = 2. * re / (an + bn) * an;           /* linearbl.f:304 */
+    double c1n = 2. * re / (anp1 + bnp1);           /* linearbl.f:1729 (tau) */
+    double c2 = 2. * re * bnp1 / (anp1 + bnp1);     /* linearbl.f:1721 */
+    double *tmp = W->v[47];
+    for (int x = 0; x < nxh; x++) {
+        int k0 = (z == 0 && x == 0);
+        double alfa = o->alfa[x];
+        double k2 = alfa * alfa + beta * beta;
+        double lam2 = k2 + 2. * re / (an + bn);
+        double k2i = k0 ? 0. : 1. / k2;
+        double *h1r = W->v[0], *h1i = W->v[1], *h2r = W->v[2], *h2i = W->v[3];
+        double *h3r = W->v[4], *h3i = W->v[5];
+        double *pvr = W->v[6], *pvi = W->v[7], *pwr = W->v[8], *pwi = W->v[9];
+        double *hvr = W->v[10], *hvi = W->v[11], *hwr = W->v[12], *hwi = W->v[13];
+        double *fvr = W->v[14], *fvi = W->v[15], *fwr = W->v[16], *fwi = W->v[17];
+        double *phr = W->v[18], *phi = W->v[19], *phh = W->v[20];
+        double *omr = W->v[21], *omi = W->v[22];
+        double *vr = W->v[23], *vi = W->v[24], *vh = W->v[25];
+        double *dvr = W->v[26], *dvi = W->v[27], *dvh = W->v[28];
+        double *domr = W->v[29], *domi = W->v[30];
+        double *ur_ = W->v[31], *ui_ = W->v[32], *wr_ = W->v[33], *wi_ = W->v[34];
+        double *oxr = W->v[35], *oxi = W->v[36], *ozr = W->v[37], *ozi = W->v[38];
+        double *zero = W->v[39];
+        double *gr = W->v[40], *gi = W->v[41];
+        /* gather and transform H to Chebyshev space (linearbl.f:262-273) */
+        double *hh[6] = {h1r, h1i, h2r, h2i, h3r, h3i};
+        for (int c = 0; c < 3; c++) {
+            for (int y = 0; y < n; y++) {
+                tmp[y] = ur[IDX(o, c, z, y, x)];
+                W->v[42][y] = ui[IDX(o, c, z, y, x)];
+            }
+            chbf_tab(n, o->ct, tmp, hh[2 * c], cn);
+            chbf_tab(n, o->ct, W->v[42], hh[2 * c + 1], cn);
+        }
+        for (int y = 0; y < n; y++) {
+            pvr[y] = ur[IDX(o, 5, z, y, x)];
+            pvi[y] = ui[IDX(o, 5, z, y, x)];
+            pwr[y] = ur[IDX(o, 6, z, y, x)];
+            pwi[y] = ui[IDX(o, 6, z, y, x)];
+            zero[y] = 0.;
+        }
+        /* h_omega = i beta H1 - i alpha H3, f_omega = p_omega - c1 h_omega,
+         * h_v = -(d/dy(i alpha H1 + i beta H3) + k^2 H2), f_v = p_v - c1 h_v
+         * (linearbl.f:325-361) */
+        for (int y = 0; y < n; y++) {
+            hwr[y] = -beta * h1i[y] + alfa * h3i[y];
+            hwi[y] = beta * h1r[y] - alfa * h3r[y];
+            fwr[y] = pwr[y] - c1 * hwr[y];
+            fwi[y] = pwi[y] - c1 * hwi[y];
+            gr[y] = -alfa * h1i[y] - beta * h3i[y];
+            gi[y] = alfa * h1r[y] + beta * h3r[y];
+        }
+        dcheb(n, gr, tmp);
+        for (int y = 0; y < n; y++)
+            hvr[y] = -(tmp[y] + k2 * h2r[y]);
+        dcheb(n, gi, tmp);
+        for (int y = 0; y < n; y++) {
+            hvi[y] = -(tmp[y] + k2 * h2i[y]);
+            fvr[y] = pvr[y] - c1 * hvr[y];
+            fvi[y] = pvi[y] - c1 * hvi[y];
+        }
+        double fuw1[1024], fuw2[1024], huw1[1024], huw2[1024];
+        if (k0) {                                   /* linearbl.f:303-319 */
+            for (int y = 0; y < n; y++) {
+                fuw1[y] = puw[y] - c1 * h1r[y];
+                fuw2[y] = puw[n + y] - c1 * h3r[y];
+                huw1[y] = h1r[y];
+                huw2[y] = h3r[y];
+                fvr[y] = fvi[y] = fwr[y] = fwi[y] = 0.;
+            }
+            fuw1[0] += 2. * re * o->px;
+        }
+        /* Helmholtz solves with lam^2 (linearbl.f:625-626): particular
+         * phi (bc 0,0), homogeneous phi_h (bc 1,1), omega_y (bc 0,0) */
+        tau_solve(n, lam2, fvr, 0., 0., phr, W->gam, W->rho);
+        tau_solve(n, lam2, fvi, 0., 0., phi, W->gam, W->rho);
+        tau_solve(n, lam2, zero, 1., 1., phh, W->gam, W->rho);
+        tau_solve(n, lam2, fwr, 0., 0., omr, W->gam, W->rho);
+        tau_solve(n, lam2, fwi, 0., 0., omi, W->gam, W->rho);
+        /* v from phi with k^2 (linearbl.f:1000-1001) */
+        tau_solve(n, k2, phr, 0., 0., vr, W->gam, W->rho);
+        tau_solve(n, k2, phi, 0., 0., vi, W->gam, W->rho);
+        tau_solve(n, k2, phh, 0., 0., vh, W->gam, W->rho);
+        /* superpose the homogeneous solution per parity so that dv/dy = 0
+         * at both walls (linearbl.f:1023-1070) */
+        dcheb(n, vr, dvr);
+        dcheb(n, vi, dvi);
+        dcheb(n, vh, dvh);
+        double Sr[2] = {0, 0}, Si[2] = {0, 0}, Sh[2] = {0, 0};
+        for (int y = 0; y < n; y++) {
+            Sr[y & 1] += dvr[y];
+            Si[y & 1] += dvi[y];
+            Sh[y & 1] += dvh[y];
+        }
+        double cr[2], ci[2];
+        for (int s = 0; s < 2; s++) {
+            cr[s] = (0. - Sr[1 - s]) * (1. / Sh[1 - s]);
+            ci[s] = (0. - Si[1 - s]) * (1. / Sh[1 - s]);
+        }
+        for (int y = 0; y < n; y++) {
+            vr[y] += cr[y & 1] * vh[y];
+            vi[y] += ci[y & 1] * vh[y];
+        }
+        /* ibc = 0: the particular solutions satisfy u=v=w=0 at the walls,
+         * so the 3x3 boundary system (evald/evalbc/solvbc) has a zero RHS
+         * and adds nothing. */
+        /* dv, phi = D^2 v - k^2 v (linearbl.f:1151-1166), d omega */
+        dcheb(n, vr, dvr);
+        dcheb(n, dvr, tmp);
+        for (int y = 0; y < n; y++)
+            phr[y] = tmp[y] - k2 * vr[y];
+        dcheb(n, vi, dvi);
+        dcheb(n, dvi, tmp);
+        for (int y = 0; y < n; y++)
+            phi[y] = tmp[y] - k2 * vi[y];
+        dcheb(n, omr, domr);
+        dcheb(n, omi, domi);
+        /* u, w, omega_x, omega_z (linearbl.f:1190-1212) */
+        for (int y = 0; y < n; y++) {
+            ur_[y] = (-alfa * dvi[y] + beta * omi[y]) * k2i;
+            ui_[y] = (alfa * dvr[y] - beta * omr[y]) * k2i;
+            wr_[y] = (-alfa * omi[y] - beta * dvi[y]) * k2i;
+            wi_[y] = (alfa * omr[y] + beta * dvr[y]) * k2i;
+            ozr[y] = (-beta * domi[y] + alfa * phi[y]) * k2i;
+            ozi[y] = (beta * domr[y] - alfa * phr[y]) * k2i;
+            oxr[y] = (-alfa * domi[y] - beta * phi[y]) * k2i;
+            oxi[y] = (alfa * domr[y] + beta * phr[y]) * k2i;
+        }
+        if (k0) {
+            /* mean flow (linearbl.f:1216-1692, tau branch, ibc = 0) */
+            double *u0 = W->v[43], *w0 = W->v[44], *du0 = W->v[45], *dw0 = W->v[46];
+            double bu_e = .5 * (o->u0upp + o->u0low), bu_o = .5 * (o->u0upp - o->u0low);
+            double bw_e = .5 * (o->w0upp + o->w0low), bw_o = .5 * (o->w0upp - o->w0low);
+            tau_solve(n, lam2, fuw1, bu_e, bu_o, u0, W->gam, W->rho);
+            tau_solve(n, lam2, fuw2, bw_e, bw_o, w0, W->gam, W->rho);
+            if (o->cflux) {
+                /* secant step on the pressure gradient (linearbl.f:1355-1417) */
+                double m1 = mass_flux(n, u0);
+                fuw1[0] += 2. * re * o->px;
+                tau_solve(n, lam2, fuw1, bu_e, bu_o, u0, W->gam, W->rho);
+                double m2 = mass_flux(n, u0);
+                fuw1[0] -= 4. * re * o->px;
+                o->px = o->px * (1. + (o->mflux - m1) / (m2 - m1));
+                fuw1[0] += 2. * re * o->px;
+                tau_solve(n, lam2, fuw1, bu_e, bu_o, u0, W->gam, W->rho);
+            }
+            dcheb(n, u0, du0);
+            dcheb(n, w0, dw0);
+            for (int y = 0; y < n; y++) {
+                oxr[y] = dw0[y];
+                ozr[y] = -du0[y];
+                ur_[y] = u0[y];
+                wr_[y] = w0[y];
+                vr[y] = 0.;
+                vi[y] = 0.;
+                ui_[y] = wi_[y] = oxi[y] = ozi[y] = 0.;
+            }
+            /* next partial RHS for the mean flow (linearbl.f:1734-1746) */
+            dcheb(n, du0, tmp);
+            for (int y = 0; y < n; y++)
+                puw[y] = -tmp[y] - c1n * u0[y] - c2 * huw1[y];
+            dcheb(n, dw0, tmp);
+            for (int y = 0; y < n; y++)
+                puw[n + y] = -tmp[y] - c1n * w0[y] - c2 * huw2[y];
+        }
+        /* next partial RHS (linearbl.f:1761-1784, tau branch) */
+        double k2m = k2 - c1n;
+        double *d1 = W->v[42];
+        dcheb(n, phr, d1);
+        dcheb(n, d1, tmp);
+        for (int y = 0; y < n; y++)
+            pvr[y] = -tmp[y] + k2m * phr[y] - c2 * hvr[y];
+        dcheb(n, phi, d1);
+        dcheb(n, d1, tmp);
+        for (int y = 0; y < n; y++)
+            pvi[y] = -tmp[y] + k2m * phi[y] - c2 * hvi[y];
+        dcheb(n, domr, tmp);
+        for (int y = 0; y < n; y++)
+            pwr[y] = -tmp[y] + k2m * omr[y] - c2 * hwr[y];
+        dcheb(n, domi, tmp);
+        for (int y = 0; y < n; y++)
+            pwi[y] = -tmp[y] + k2m * omi[y] - c2 * hwi[y];
+        /* back to physical y and store (linearbl.f:1830-1867) */
+        double *outs[10] = {ur_, ui_, vr, vi, wr_, wi_, oxr, oxi, ozr, ozi};
+        for (int c = 0; c < 5; c++) {
+            chbb_tab(n, o->ct, outs[2 * c], tmp);
+            for (int y = 0; y < n; y++)
+                ur[IDX(o, c, z, y, x)] = tmp[y];
+            chbb_tab(n, o->ct, outs[2 * c + 1], tmp);
+            for (int y = 0; y < n; y++)
+                ui[IDX(o, c, z, y, x)] = tmp[y];
+        }
+        for (int y = 0; y < n; y++) {
+            ur[IDX(o, 5, z, y, x)] = pvr[y];
+            ui[IDX(o, 5, z, y, x)] = pvi[y];
+            ur[IDX(o, 6, z, y, x)] = pwr[y];
+            ui[IDX(o, 6, z, y, x)] = pwi[y];
+        }
+    }
+}
+
+void orc_linear(orc_t *o, double *ur, double *ui, double *puw,
+                double an, double bn, double anp1, double bnp1)
+{
+#pragma omp parallel
+    {
+        work_t W;
+        work_alloc(&W, o->ny);
+#pragma omp for schedule(dynamic)
+        for (int z = 0; z < o->nz; z++) {
+            if (z == o->nz / 2)          /* linearbl.f:227, z oddball */
+                continue;
+            linear_plane(o, ur, ui, puw, z, an, bn, anp1, bnp1, &W);
+        }
+        work_free(&W);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* prhs (prhs.f:93-709)                                                */
+/* ------------------------------------------------------------------ */
+
+static void prhs_plane(orc_t *o, double *ur, double *ui, double *puw, int z,
+                       double an, work_t *W)
+{
+    int n = o->ny, nxh = o->nxh;
+    double re = o->re, beta = o->beta[z];
+    double cn = 2. / (double)(n - 1);
+    double *tmp = W->v[47];
+    for (int x = 0; x < nxh; x++) {
+        double alfa = o->alfa[x];
+        double k2 = alfa * alfa + beta * beta;
+        double lam = k2 - 2. * re / an;
+        double *U[6];
+        for (int c = 0; c < 6; c++)
+            U[c] = W->v[c];
+        for (int c = 0; c < 3; c++) {
+            for (int y = 0; y < n; y++) {
+                W->v[40][y] = ur[IDX(o, c, z, y, x)];
+                W->v[41][y] = ui[IDX(o, c, z, y, x)];
+            }
+            chbf_tab(n, o->ct, W->v[40], U[2 * c], cn);
+            chbf_tab(n, o->ct, W->v[41], U[2 * c + 1], cn);
+        }
+        double *uR = U[0], *uI = U[1], *vR = U[2], *vI = U[3], *wR = U[4], *wI = U[5];
+        double *oyr = W->v[6], *oyi = W->v[7], *oxr = W->v[8], *oxi = W->v[9];
+        double *ozr = W->v[10], *ozi = W->v[11], *phr = W->v[12], *phi = W->v[13];
+        double *d1 = W->v[14], *d2 = W->v[15];
+        if (z == 0 && x == 0) {
+            dcheb(n, uR, d1);
+            dcheb(n, d1, d2);
+            for (int y = 0; y < n; y++)
+                puw[y] = -2. * re / an * uR[y] - d2[y];
+            dcheb(n, wR, d1);
+            dcheb(n, d1, d2);
+            for (int y = 0; y < n; y++)
+                puw[n + y] = -2. * re / an * wR[y] - d2[y];
+        }
+        for (int y = 0; y < n; y++) {
+            oyr[y] = -uI[y] * beta + wI[y] * alfa;
+            oyi[y] = uR[y] * beta - wR[y] * alfa;
+        }
+        dcheb(n, wR, oxr);
+        dcheb(n, wI, oxi);
+        dcheb(n, uR, ozr);
+        dcheb(n, uI, ozi);
+        for (int y = 0; y < n; y++) {
+            oxr[y] = oxr[y] + vI[y] * beta;
+            oxi[y] = oxi[y] - vR[y] * beta;
+            ozr[y] = -ozr[y] - vI[y] * alfa;
+            ozi[y] = -ozi[y] + vR[y] * alfa;
+        }
+        /* phi = D^2 v - k^2 v, p_omega = -D^2 omega_y + lam omega_y,
+         * p_v = -D^2 phi + lam phi (prhs.f:595-672) */
+        double *pvr = W->v[16], *pvi = W->v[17], *pwr = W->v[18], *pwi = W->v[19];
+        dcheb(n, vR, d1);
+        dcheb(n, d1, phr);
+        dcheb(n, vI, d1);
+        dcheb(n, d1, phi);
+        dcheb(n, oyr, d1);
+        dcheb(n, d1, pwr);
+        dcheb(n, oyi, d1);
+        dcheb(n, d1, pwi);
+        for (int y = 0; y < n; y++) {
+            phr[y] = phr[y] - k2 * vR[y];
+            phi[y] = phi[y] - k2 * vI[y];
+            pwr[y] = -pwr[y] + lam * oyr[y];
+            pwi[y] = -pwi[y] + lam * oyi[y];
+        }
+        dcheb(n, phr, d1);
+        dcheb(n, d1, pvr);
+        dcheb(n, phi, d1);
+        dcheb(n, d1, pvi);
+        for (int y = 0; y < n; y++) {
+            pvr[y] = -pvr[y] + lam * phr[y];
+            pvi[y] = -pvi[y] + lam * phi[y];
+        }
+        double *outs[4] = {oxr, oxi, ozr, ozi};
+        for (int c = 0; c < 2; c++) {
+            chbb_tab(n, o->ct, outs[2 * c], tmp);
+            for (int y = 0; y < n; y++)
+                ur[IDX(o, 3 + c, z, y, x)] = tmp[y];
+            chbb_tab(n, o->ct, outs[2 * c + 1], tmp);
+            for (int y = 0; y < n; y++)
+                ui[IDX(o, 3 + c, z, y, x)] = tmp[y];
+        }
+        for (int y = 0; y < n; y++) {
+            ur[IDX(o, 5, z, y, x)] = pvr[y];
+            ui[IDX(o, 5, z, y, x)] = pvi[y];
+            ur[IDX(o, 6, z, y, x)] = pwr[y];
+            ui[IDX(o, 6, z, y, x)] = pwi[y];
+        }
+    }
+}
+
+void orc_prhs(orc_t *o, double *ur, double *ui, double *puw, double an)
+{
+    /* u0upp etc. from the wall values of the mean mode (rdiscbl.f:319-323,
+     * thresholded as in bla.f:1041-1044) */
+    double v[4] = {ur[IDX(o, 0, 0, 0, 0)], ur[IDX(o, 0, 0, o->ny - 1, 0)],
+                   ur[IDX(o, 2, 0, 0, 0)], ur[IDX(o, 2, 0, o->ny - 1, 0)]};
+    for (int i = 0; i < 4; i++)
+        if (fabs(v[i]) < 1e-8)
+            v[i] = 0.;
+    o->u0upp = v[0];
+    o->u0low = v[1];
+    o->w0upp = v[2];
+    o->w0low = v[3];
+#pragma omp parallel
+    {
+        work_t W;
+        work_alloc(&W, o->ny);
+#pragma omp for schedule(dynamic)
+        for (int z = 0; z < o->nz; z++)
+            prhs_plane(o, ur, ui, puw, z, an, &W);
+        work_free(&W);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Driver (bla.f:1465-1477, 1790-2610 with fixed dt)                   */
+/* ------------------------------------------------------------------ */
+
+int orc_run(orc_t *o, double *ur, double *ui, double *puw, int nsteps,
+            double dt, int icfl, double *cfl_out)
+{
+    static const double anrk[3] = {8. / 15., 5. / 12., 3. / 4.};
+    static const double bnrk[3] = {0., -17. / 60., -5. / 12.};
+    const int nst = 3;
+    double dtn = dt, an = dtn * anrk[0], bn, anp1 = an, bnp1 = 0., dtnp1 = dtn;
+    orc_prhs(o, ur, ui, puw, anp1);
+    int status = 0;
+    for (int it = 1; it <= nst * nsteps; it++) {
+        int do_cfl = icfl > 0 && ((it - 1) % icfl) == 0;
+        double cflp = orc_nonlin(o, ur, ui, do_cfl);
+        if (do_cfl) {
+            double cfl = (cflp * PI + 2. * fabs(o->rot)) * dtn;   /* wcflbl.f:87 */
+            if (cfl_out)
+                *cfl_out = cfl;
+            if (cfl > 10.)
+                return 324234;
+        }
+        an = anp1;
+        bn = bnp1;
+        anp1 = dtnp1 * anrk[it % nst];
+        bnp1 = dtnp1 * bnrk[it % nst];
+        orc_linear(o, ur, ui, puw, an, bn, anp1, bnp1);
+    }
+    return status;
+}

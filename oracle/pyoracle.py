@@ -1,0 +1,103 @@
+"""ctypes wrapper of the CPU oracle (``oracle/_build/libblaref.so``).
+
+TEST INFRASTRUCTURE ONLY: imported by tests, the smoke check and the CPU
+baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "_build", "libblaref.so")
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"oracle not built: {LIBPATH} (run make -C oracle)")
+        L = C.CDLL(LIBPATH)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                 C.c_double, C.c_double, C.c_int, C.c_double]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_get_px.restype = C.c_double
+        L.orc_get_px.argtypes = [C.c_void_p]
+        L.orc_set_px.argtypes = [C.c_void_p, C.c_double]
+        L.orc_prhs.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_double]
+        L.orc_nonlin.restype = C.c_double
+        L.orc_nonlin.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
+        L.orc_linear.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_double, C.c_double,
+                                 C.c_double, C.c_double]
+        L.orc_run.restype = C.c_int
+        L.orc_run.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int,
+                              C.POINTER(C.c_double)]
+        L.orc_cdft.argtypes = [C.c_int, _dp, _dp, C.c_int]
+        L.orc_rdft_bwd.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.orc_rdft_fwd.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.orc_chbf.argtypes = [C.c_int, _dp, _dp]
+        L.orc_chbb.argtypes = [C.c_int, _dp, _dp]
+        L.orc_dcheb.argtypes = [C.c_int, _dp, _dp]
+        L.orc_icheb.argtypes = [C.c_int, _dp, _dp, C.c_double]
+        L.orc_tau_solve.argtypes = [C.c_int, C.c_double, _dp, C.c_double, C.c_double, _dp]
+        _lib = L
+    return _lib
+
+
+class Oracle:
+    """One channel configuration; state arrays are owned by the caller."""
+
+    NSLOT = 7
+
+    def __init__(self, nx, ny, nz, re, xl, zl, rot=0.0, cflux=True, retau=0.0):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.h = lib().orc_create(nx, ny, nz, re, xl, zl, rot, int(cflux), retau)
+
+    def close(self):
+        if self.h:
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def px(self):
+        return lib().orc_get_px(self.h)
+
+    @px.setter
+    def px(self, v):
+        lib().orc_set_px(self.h, v)
+
+    def new_state(self, u: np.ndarray):
+        """Main storage ur, ui ([slot][z][y][x]) and puw from velocities."""
+        ur = np.zeros((self.NSLOT, self.nz, self.ny, self.nx // 2))
+        ui = np.zeros_like(ur)
+        ur[:3] = u.real
+        ui[:3] = u.imag
+        puw = np.zeros((2, self.ny))
+        return ur, ui, puw
+
+    def prhs(self, ur, ui, puw, an):
+        lib().orc_prhs(self.h, ur, ui, puw, an)
+
+    def nonlin(self, ur, ui, do_cfl=True):
+        return lib().orc_nonlin(self.h, ur, ui, int(do_cfl))
+
+    def linear(self, ur, ui, puw, an, bn, anp1, bnp1):
+        lib().orc_linear(self.h, ur, ui, puw, an, bn, anp1, bnp1)
+
+    def run(self, ur, ui, puw, nsteps, dt, icfl=0):
+        cfl = C.c_double(0.0)
+        st = lib().orc_run(self.h, ur, ui, puw, nsteps, dt, icfl, C.byref(cfl))
+        return st, cfl.value
+
+
+def velocities(ur, ui):
+    return ur[:3] + 1j * ui[:3]

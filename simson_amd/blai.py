@@ -1,0 +1,285 @@
+"""Reader and writer for ``bla.i`` runtime parameters (version 20070716).
+
+The grammar follows ``bla/rparambl.f:160-1860``: one value per line, lines
+starting with ``#`` are skipped (``bla/comment.f``), numbers and logicals are
+read list-directed (first token of the line), file names with ``(a)``.
+Conditional reads (``retau`` only when ``cflux`` is false, the amplitude
+file name only when ``iamp > 0`` ...) are reproduced for the temporal
+channel subset this library runs; any parameter that switches on a feature
+outside that subset raises ``BlaInputError``.
+
+The post-read adjustments of ``rparambl.f:1860-2033`` are applied: ``icfl``
+rounded to multiples of ``nst``, ``maxit`` rounded down to a multiple of
+``nst``, ``vart = dt <= 0`` and ``cflmax = cflmaxin * sqrt(3)`` for RK3.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+VERSION = 20070716
+
+
+class BlaInputError(ValueError):
+    """Raised for malformed or unsupported ``bla.i`` content.
+
+    ``code`` is the reference stop code where one exists (13 for a bad
+    parameter file, ``rparambl.f:2001``; 345431 for a wrong version).
+    """
+
+    def __init__(self, msg: str, code: int = 13):
+        super().__init__(msg)
+        self.code = code
+
+
+@dataclass
+class BlaParams:
+    namnin: str = "init.u"
+    namnut: str = "end.u"
+    tmax: float = 0.0
+    maxit: int = 0
+    cpumax: float = 0.0
+    wallmax: float = 0.0
+    write_inter: bool = False
+    dt: float = 0.01
+    nst: int = 3
+    cflmaxin: float = 0.8
+    xl: float = 4 * math.pi
+    varsiz: bool = False
+    rot: float = 0.0
+    cflux: bool = True
+    retau: float = 0.0
+    pert: bool = False
+    ibc: int = 0
+    cim: bool = False
+    gall: bool = False
+    suction: bool = False
+    spat: bool = False
+    cdev: float = 0.0
+    sgs: bool = False
+    isfd: int = 0
+    imhd: int = 0
+    loctyp: int = 0
+    tripf: bool = False
+    wbci: int = 0
+    icfl: int = 0
+    iamp: int = 0
+    namamp: str = "amp.stat"
+    fileurms: bool = False
+    longli: bool = False
+    iext: int = 0
+    ixys: int = 0
+    msave: int = 0
+    tsave: list = field(default_factory=list)
+    nmsave: list = field(default_factory=list)
+    mwave: int = 0
+    npl: int = 0
+
+    # derived (rparambl.f:1957-2033)
+    @property
+    def vart(self) -> bool:
+        return self.dt <= 0
+
+    @property
+    def cflmax(self) -> float:
+        base = {1: 1.0, 3: math.sqrt(3.0), 4: math.sqrt(8.0)}[self.nst]
+        return self.cflmaxin * base
+
+    def adjusted(self) -> "BlaParams":
+        p = BlaParams(**{k: getattr(self, k) for k in self.__dataclass_fields__})
+        if 0 < p.icfl < p.nst:
+            p.icfl = p.nst
+        if p.icfl > 0:
+            p.icfl = min(p.icfl // p.nst * p.nst, 3 * p.nst)
+        if p.dt <= 0 and p.icfl <= 0:
+            p.icfl = p.nst
+        p.maxit = max(p.maxit, 0) // p.nst * p.nst
+        return p
+
+
+def _lines(text: str):
+    for raw in text.splitlines():
+        if raw.startswith("#"):
+            continue
+        yield raw
+
+
+def _logical(tok: str) -> bool:
+    t = tok.strip().lower().strip(".")
+    if t.startswith("t"):
+        return True
+    if t.startswith("f"):
+        return False
+    raise BlaInputError(f"bad logical {tok!r}")
+
+
+def _num(tok: str) -> float:
+    return float(tok.lower().replace("d", "e"))
+
+
+def parse(text: str) -> BlaParams:
+    it = _lines(text)
+
+    def nxt() -> str:
+        for line in it:
+            if line.strip() == "":
+                # list-directed reads skip blank records
+                continue
+            return line
+        raise BlaInputError("bla.i ended early")
+
+    def tok() -> str:
+        return nxt().split()[0]
+
+    def name() -> str:
+        return nxt().strip().split()[0]
+
+    p = BlaParams()
+    version = int(_num(tok()))
+    if version != VERSION:
+        raise BlaInputError(f"wrong bla.i version {version}", code=345431)
+    p.namnin = name()
+    p.namnut = name()
+    p.tmax = max(_num(tok()), 0.0)
+    p.maxit = max(int(_num(tok())), 0)
+    p.cpumax = max(_num(tok()), 0.0)
+    p.wallmax = max(_num(tok()), 0.0)
+    p.write_inter = _logical(tok())
+    p.dt = _num(tok())
+    p.nst = int(_num(tok()))
+    p.cflmaxin = _num(tok())
+    p.xl = _num(tok())
+    p.varsiz = _logical(tok())
+    p.rot = _num(tok())
+    p.cflux = _logical(tok())
+    if not p.cflux:
+        p.retau = _num(tok())
+    p.pert = _logical(tok())
+    if p.pert:
+        raise BlaInputError("perturbation mode (pert) is not supported")
+    p.ibc = int(_num(tok()))
+    p.cim = _logical(tok())
+    if p.cim:
+        # setmat.f:32 aborts unconditionally in the reference as well
+        raise BlaInputError("Chebyshev integration method is disabled", code=434342)
+    p.gall = _logical(tok())
+    p.suction = _logical(tok())
+    if p.suction:
+        raise BlaInputError("wall suction is not supported")
+    p.spat = _logical(tok())
+    if p.spat:
+        raise BlaInputError("spatial simulations are not supported")
+    p.cdev = _num(tok())
+    p.sgs = _logical(tok())
+    if p.sgs:
+        raise BlaInputError("LES (sgs) is not supported by the step library")
+    p.isfd = int(_num(tok()))
+    if p.isfd != 0:
+        raise BlaInputError("selective frequency damping is not supported")
+    p.imhd = int(_num(tok()))
+    if p.imhd != 0:
+        raise BlaInputError("MHD is not supported")
+    p.loctyp = int(_num(tok()))
+    if p.loctyp != 0:
+        raise BlaInputError("localized forcing is not supported")
+    p.tripf = _logical(tok())
+    if p.tripf:
+        raise BlaInputError("trip forcing is not supported")
+    p.wbci = int(_num(tok()))
+    if p.wbci != 0:
+        raise BlaInputError("wall boundary forcing (wbci) is not supported")
+    p.icfl = int(_num(tok()))
+    p.iamp = int(_num(tok()))
+    if p.iamp > 0:
+        p.namamp = name()
+        p.fileurms = _logical(tok())
+    p.longli = _logical(tok())
+    p.iext = int(_num(tok()))
+    if p.iext > 0:
+        name()
+    p.ixys = max(0, int(_num(tok())))
+    if p.ixys > 0:
+        raise BlaInputError("xy statistics (ixys) are not supported yet")
+    p.msave = int(_num(tok()))
+    if p.msave < 0 or p.msave == 100000:
+        raise BlaInputError("periodic save modes are not supported")
+    for _ in range(p.msave):
+        p.tsave.append(_num(tok()))
+        p.nmsave.append(name())
+    p.mwave = max(0, int(_num(tok())))
+    if p.mwave > 0:
+        raise BlaInputError("wavenumber saving (mwave) is not supported")
+    p.npl = max(0, int(_num(tok())))
+    if p.npl > 0:
+        raise BlaInputError("plane saving (npl) is not supported")
+    if p.nst not in (3, 4):
+        raise BlaInputError("nst must be 3 or 4")
+    if p.xl <= 0:
+        raise BlaInputError("xl must be > 0")
+    if p.ibc != 0:
+        raise BlaInputError("only ibc=0 (no-slip channel walls) is supported")
+    return p
+
+
+def read(path: str) -> BlaParams:
+    with open(path) as f:
+        return parse(f.read())
+
+
+def format_channel(p: BlaParams) -> str:
+    """Write a channel ``bla.i`` that the reference ``bla`` accepts."""
+    def L(v):
+        return ".true." if v else ".false."
+    out = [
+        "# bla.i written by simson_amd",
+        f"{VERSION}        Version of input file",
+        p.namnin,
+        p.namnut,
+        f"{p.tmax!r}      Maximum simulation time (tmax)",
+        f"{p.maxit}       Maximum iterations (maxit)",
+        f"{p.cpumax!r}    Maximum CPU time (cpumax)",
+        f"{p.wallmax!r}   Maximum wall-clock time (wallmax)",
+        f"{L(p.write_inter)}  Save intermediate files (write_inter)",
+        f"{p.dt!r}        Time step (dt)",
+        f"{p.nst}         number of RK stages (nst)",
+        f"{p.cflmaxin!r}  CFL number (cflmaxin)",
+        f"{p.xl!r}        Box length (xl)",
+        f"{L(p.varsiz)}   Variable size (varsiz)",
+        f"{p.rot!r}       Rotation rate (rot)",
+        f"{L(p.cflux)}    Constant mass flux (cflux)",
+    ]
+    if not p.cflux:
+        out.append(f"{p.retau!r}   Re_tau")
+    out += [
+        ".false.         Perturbation mode (pert)",
+        f"{p.ibc}         Boundary condition number (ibc)",
+        ".false.         Chebyshev integration method (cim)",
+        f"{L(p.gall)}    Galilei transformation (gall)",
+        ".false.         Constant wall-suction (suction)",
+        ".false.         Spatial simulation (spat)",
+        f"{p.cdev!r}      Temporal development vel. (cdev)",
+        ".false.         Read sgs.i (sgs)",
+        "0               SFD (isfd)",
+        "0               MHD (imhd)",
+        "0               Type of localized disturbance (loctyp)",
+        ".false.         Trip forcing (tripf)",
+        "0               Boundary condition at lower wall (wbci)",
+        f"{p.icfl}        CFL calculation interval (icfl)",
+        f"{p.iamp}        Amplitude calculation interval (iamp)",
+    ]
+    if p.iamp > 0:
+        out += [p.namamp, f"{L(p.fileurms)}  Write urms velocities (fileurms)"]
+    out += [
+        ".false.         y-dependent statistics (longli)",
+        "0               Extremum calculation interval (iext)",
+        "0               xy-statistics calculation interval (ixys)",
+        f"{len(p.tsave)}  Number of 3D fields to save (msave)",
+    ]
+    for t, n in zip(p.tsave, p.nmsave):
+        out += [f"{t!r}", n]
+    out += [
+        "0               Number of wavenumbers to save (mwave)",
+        "0               Number of planes to save (npl)",
+        "",
+    ]
+    return "\n".join(out)

@@ -1,0 +1,85 @@
+"""Generate the golden fixtures from the reference Simson `bla`.
+
+Runs in the build container only (it needs /root/reference and amdflang):
+
+    python tests/golden/make_golden.py
+
+1. builds the reference `bla` for 32x33x32 (serial, pressure off) with
+   oracle/build_ref.sh into oracle/_ref/;
+2. writes the seeded plane-channel field (Re_cl 3000, lx 4 pi, lz 2 pi,
+   |kx|,|kz| <= 8, rms 1e-3, seed 1, Poiseuille mean) with simson_amd.fields;
+3. runs 100 RK3 steps at dt = 0.01 under constant flux, saving the velocity
+   field after 1, 10 and 100 steps, with amp.stat written every full step
+   (its column 12 is the pressure gradient px, wampbl.f:276-278) and the CFL
+   evaluated every substep;
+4. stores the reference outputs in c1_32x33x32.npz: the three fields
+   (complex128 [comp, z, y, kx], exactly as read from bla's .u files), the
+   px trajectory, the CFL*dt log lines and the sha256 of the input file.
+
+The fixture holds data produced by running the reference, not its source.
+"""
+from __future__ import annotations
+
+import hashlib
+import math
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+
+from simson_amd import blai, fields, ufile  # noqa: E402
+
+NX, NY, NZ = 32, 33, 32
+BIN = os.path.join(REPO, "oracle", "_ref", "bla_32x33x32_serial")
+
+
+def c1_field():
+    return fields.channel_field(NX, NY, NZ, 4 * math.pi, 2 * math.pi, 3000.0,
+                                kmax=8, rms=1e-3, seed=1)
+
+
+def main():
+    if not os.path.exists(BIN):
+        subprocess.check_call([os.path.join(REPO, "oracle", "build_ref.sh"),
+                               str(NX), str(NY), str(NZ), "1", "no", "0",
+                               os.path.basename(BIN)])
+    work = tempfile.mkdtemp(prefix="simson_golden.")
+    try:
+        ufile.write_u(os.path.join(work, "init.u"), c1_field())
+        p = blai.BlaParams(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=300,
+                           dt=0.01, nst=3, xl=4 * math.pi, cflux=True, icfl=3,
+                           iamp=3, namamp="amp.stat",
+                           tsave=[0.01, 0.1], nmsave=["t1.u", "t10.u"])
+        with open(os.path.join(work, "bla.i"), "w") as f:
+            f.write(blai.format_channel(p))
+        with open(os.path.join(work, "out.txt"), "w") as out:
+            subprocess.check_call([BIN], cwd=work, stdout=out, stderr=subprocess.STDOUT)
+        log = open(os.path.join(work, "out.txt")).read()
+        if "bla terminated normally" not in log:
+            raise RuntimeError("reference bla did not terminate normally")
+        cfl = [float(m) for m in re.findall(r"CFL\*dt\s+([0-9.Ee+-]+)", log)]
+        amp = np.loadtxt(os.path.join(work, "amp.stat"))
+        sha = hashlib.sha256(open(os.path.join(work, "init.u"), "rb").read()).hexdigest()
+        u1 = ufile.read_u(os.path.join(work, "t1.u")).u
+        u10 = ufile.read_u(os.path.join(work, "t10.u")).u
+        u100 = ufile.read_u(os.path.join(work, "end.u")).u
+        np.savez_compressed(os.path.join(HERE, "c1_32x33x32.npz"),
+                            u1=u1, u10=u10, u100=u100,
+                            amp_t=amp[:, 0], amp_px=amp[:, 11],
+                            cfl=np.array(cfl), init_sha256=np.array(sha))
+        print("wrote", os.path.join(HERE, "c1_32x33x32.npz"), "cfl lines", len(cfl),
+              "amp rows", amp.shape)
+    finally:
+        shutil.rmtree(work)
+
+
+if __name__ == "__main__":
+    main()
