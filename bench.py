@@ -1,0 +1,193 @@
+"""Benchmark of the MI355X bla time step (one JSON line on stdout).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Workload: the Re_tau ~ 180 plane channel, 128 x 97 x 128 (nx, ny = nyp, nz),
+lx = 4 pi, lz = 4 pi/3, Re_cl = 4200, constant mass flux, RK3-CN at fixed
+dt = 0.005, CFL evaluated every full step as bla.i icfl=3 does.  The initial
+field is the seeded divergence-free perturbed Poiseuille flow (10% rms over
+|kx|,|kz| <= 16, seed 2); the state is resident on the GPU before timing
+starts.  A step is one full RK3 step (three substeps, each step 2 over all
+y planes and step 3 over all z planes).
+
+Roofline: per-kernel durations come from HIP events recorded on the
+library's stream around every launch of the timed region; the dominant
+kernel family is reported with its algorithmic bytes (HBM-bound kernels) or
+flops (the Chebyshev transform products) per launch.
+
+CPU baseline: the reference Simson bla built from its Fortran sources
+(oracle/_ref, OpenMP, 16 threads) on the same seeded field, timed by bla's
+own "Average time step" over a bounded sample of steps on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "channel time-steps/s @128x97x128: FP32-mixed vs FP64 GPU at ≤1e-5 rel err"
+NX, NY, NZ = 128, 97, 128
+RE, XL, ZL, DT = 4200.0, 4 * math.pi, 4 * math.pi / 3, 0.005
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+REF_BIN = os.path.join(HERE, "oracle", "_ref", "bla_128x97x128_omp16")
+
+
+def alg_per_launch():
+    """Algorithmic HBM bytes (or flops) of one launch of each kernel family.
+
+    F = nz*ny*nx/2 complex values per field, Fp = ny*nzp*nx/2 per padded
+    z-physical field, 16 bytes per FP64 complex value; the z-oddball plane
+    is skipped by step 3 (nz-1 planes).
+    """
+    nxh, nzp = NX // 2, NZ + NZ // 2
+    F = NZ * NY * nxh
+    Fp = NY * nzp * nxh
+    F3 = (NZ - 1) * NY * nxh
+    b = 16
+    return {
+        # read u,v,w,om_x,om_z; write six z-synthesised fields
+        "zsyn": ("hbm", (5 * F + 6 * Fp) * b),
+        # read six fields, write three products
+        "xrow": ("hbm", (6 * Fp + 3 * Fp) * b),
+        # read three products, write H into three storage slots
+        "zana": ("hbm", (3 * Fp + 3 * F) * b),
+        # read H (3 fields) and p_v, p_om; write u,v,w,om_x,om_z and p_v, p_om
+        "tau": ("hbm", (5 * F3 + 7 * F3) * b),
+        # dense ny x ny transform of 3 (fwd) / 5 (bwd) complex fields
+        "cheb_fwd": ("flops", 2.0 * NY * NY * 2 * nxh * (NZ - 1) * 3),
+        "cheb_bwd": ("flops", 2.0 * NY * NY * 2 * nxh * (NZ - 1) * 5),
+    }
+
+
+def cpu_baseline(nsteps: int = 40, threads: int = 16):
+    """Time the reference bla on the host (bounded sample of full steps)."""
+    if not os.path.exists(REF_BIN):
+        return None
+    from simson_amd import blai, fields, ufile
+    work = tempfile.mkdtemp(prefix="simson_cpu.")
+    try:
+        f = fields.channel_field(NX, NY, NZ, XL, ZL, RE, kmax=16, rms=0.1, seed=2)
+        ufile.write_u(os.path.join(work, "init.u"), f)
+        p = blai.BlaParams(namnin="init.u", namnut="end.u", tmax=1e6, maxit=3 * nsteps,
+                           dt=DT, nst=3, xl=XL, cflux=True, icfl=3)
+        with open(os.path.join(work, "bla.i"), "w") as fh:
+            fh.write(blai.format_channel(p))
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        cmd = f"ulimit -s unlimited; exec {REF_BIN}"
+        r = subprocess.run(["bash", "-c", cmd], cwd=work, env=env, capture_output=True,
+                           text=True, timeout=600)
+        m = re.search(r"Average time step\s*:\s*([0-9.Ee+-]+)", r.stdout)
+        if r.returncode != 0 or not m:
+            return None
+        sec = float(m.group(1))
+        return {"value": 1.0 / sec, "unit": "time-steps/s", "cores": threads,
+                "kind": "reference",
+                "sample": f"reference bla (Fortran, amdflang -O3 OpenMP) {NX}x{NY}x{NZ}, "
+                          f"{nsteps} RK3 steps, bla's 'Average time step' = {sec:.4f} s"}
+    except (subprocess.TimeoutExpired, OSError):
+        return None
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the step is not sharded across GPUs).", file=sys.stderr)
+        sys.exit(2)
+
+    # the CPU leg runs first, before this process touches the GPU
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_steps)
+
+    from simson_amd import fields
+    from simson_amd.gpu import BlaGPU
+
+    f = fields.channel_field(NX, NY, NZ, XL, ZL, RE, kmax=16, rms=0.1, seed=2)
+    g = BlaGPU(NX, NY, NZ, RE, XL, ZL, cflux=True)
+    g.upload_velocity(f.u, -2.0 / RE)
+    g.run(args.warmup, DT, icfl=3, first=True)
+    g.sync()
+    g.timing(True)
+    g.timing_read()                     # drop anything recorded so far
+    t0 = time.perf_counter()
+    cfl = g.run(args.steps, DT, icfl=3, first=False)
+    g.sync()
+    t1 = time.perf_counter()
+    kern = g.timing_read()
+    g.timing(False)
+    g.close()
+
+    elapsed = t1 - t0
+    steps_per_s = args.steps / elapsed
+    alg = alg_per_launch()
+    kernels = {}
+    for name, (ms, cnt) in kern.items():
+        avg_us = 1e3 * ms / cnt
+        entry = {"launches": cnt, "avg_us": round(avg_us, 3),
+                 "share": round(ms / max(1e-12, sum(v[0] for v in kern.values())), 4)}
+        if name in alg:
+            kind, amount = alg[name]
+            if kind == "hbm":
+                gbs = amount / (avg_us * 1e-6) / 1e9
+                entry.update(bound="hbm", alg_bytes=int(amount), achieved_GBs=round(gbs, 1),
+                             frac=round(gbs / HBM_PEAK_GBS, 4))
+            else:
+                tf = amount / (avg_us * 1e-6) / 1e12
+                entry.update(bound="fp64", alg_flops=int(amount), achieved_TFLOPs=round(tf, 3),
+                             frac=round(tf / FP64_PEAK_TFLOPS, 4))
+        kernels[name] = entry
+    dom = max((k for k in kernels if k in alg), key=lambda k: kern[k][0])
+    d = kernels[dom]
+    if d["bound"] == "hbm":
+        roofline = {"kernel": dom, "bound": "hbm", "achieved": d["achieved_GBs"],
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["frac"], "traffic": None}
+    else:
+        roofline = {"kernel": dom, "bound": "mfma", "achieved": d["achieved_TFLOPs"],
+                    "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": d["frac"],
+                    "traffic": None}
+    out = {
+        "metric": METRIC,
+        "value": round(steps_per_s, 3),
+        "unit": "time-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "Re_tau~180 plane channel 128x97x128, RK3-CN dt=0.005, "
+                               "constant flux, FP64 step",
+                   "nx": NX, "ny": NY, "nz": NZ, "re_cl": RE, "lx": "4pi", "lz": "4pi/3",
+                   "dt": DT, "icfl": 3, "form": "fp64", "parallelism": "single GPU"},
+        "grid_points_steps_per_s": round(steps_per_s * NX * NY * NZ, 1),
+        "cfl_dt": cfl,
+        "roofline": roofline,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,119 @@
+/*
+ * simson_gpu.h -- C ABI of the MI355X Simson `bla` time step.
+ *
+ * The reference has no plugin or FFI layer: its step is the body of the
+ * main loop of bla/bla.f:1790-2885, i.e. per RK substep one call of
+ * nonlinbl per y plane (bla.f:2251-2288) followed by one call of linearbl
+ * per z plane (bla.f:2433-2475), with prhs once at start-up (bla.f:1562).
+ * These entry points replace exactly those calls; a host that keeps
+ * bla's par.f/bla.i parameters, its .u file reader/writer and its RK
+ * bookkeeping (bla.f:1465-1477, 2384-2397) calls them instead.
+ *
+ * Conventions
+ *  - All host arrays are FP64 in the reference main-storage layout
+ *    ur(memnx, memny, memnz, memnxyz) / ui(...) of bla.f:67, i.e. C arrays
+ *    [slot][z][y][x] with x < nx/2, y < ny, z < nz.  Slots: 0-2 u,v,w;
+ *    3-4 omega_x, omega_z; 5-6 partial right-hand sides p_v, p_omega.
+ *  - puw is the k=0 partial right-hand side puw(ny, 2) of bla.f:131, a C
+ *    array [2][ny].
+ *  - Status: 0 on success; otherwise the reference stop code the reference
+ *    would have raised (e.g. 324234 for CFL*dt > 10, wcflbl.f:97) or a
+ *    negative SIMSON_E_* code for errors of this library.
+ *  - The state lives on the GPU between calls.  No host pointer is kept
+ *    after a call returns.  A handle is used from one host thread at a time.
+ */
+#ifndef SIMSON_GPU_H
+#define SIMSON_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SIMSON_E_ARG (-1)       /* invalid argument / unsupported setting */
+#define SIMSON_E_HIP (-2)       /* HIP runtime error */
+#define SIMSON_E_NODEV (-3)     /* no HIP device */
+
+#define SIMSON_FP64 0           /* all arithmetic in FP64 */
+
+typedef struct simson_params {
+    int nx, ny, nz;      /* par.f nx, ny (= nyp, nfyd = 0), nz; nfxd = nfzd = 1 */
+    int precision;       /* SIMSON_FP64 */
+    int ibc;             /* bla.i ibc; only 0 (no-slip walls) */
+    int cflux;           /* bla.i cflux (constant mass flux) */
+    double re;           /* Reynolds number from the .u header */
+    double xl, zl;       /* box lengths (bla.i xl, .u zl) */
+    double rot;          /* bla.i rot */
+    double retau;        /* bla.i retau (used when cflux == 0) */
+} simson_params_t;
+
+typedef struct simson_gpu simson_gpu_t;
+
+/* Allocate the device state for one channel configuration.
+ * Replaces the set-up of bla.f:1080-1082 (preprbl) and the pressure
+ * gradient initialisation of bla.f:1622-1662. */
+int simson_gpu_init(const simson_params_t *p, simson_gpu_t **out);
+
+/* Upload nslots (3 or 7) slots of main storage, the k=0 partial RHS puw
+ * (may be NULL when nslots == 3) and the pressure gradient px.  With
+ * nslots == 3 the wall values u0low/u0upp/w0low/w0upp are taken from the
+ * mean mode as rdiscbl.f:319-323 and bla.f:1041-1044 do. */
+int simson_gpu_upload(simson_gpu_t *g, const double *ur, const double *ui,
+                      int nslots, const double *puw, double px);
+
+/* prhs (bla.f:1562, prhs.f:93-709): partial RHS, omega_x, omega_z and puw
+ * from the velocities, for a first stage of weight an. */
+int simson_gpu_prhs(simson_gpu_t *g, double an);
+
+/* Step 2 of a substep alone: nonlinbl over all y planes (bla.f:2251-2288).
+ * Slots 0-2 receive H_1..3 (times nxp*nzp, unnormalised as in the
+ * reference).  When do_cfl is non-zero, *cfl_out receives the boxcfl
+ * partial max (synchronises). */
+int simson_gpu_nonlin(simson_gpu_t *g, int do_cfl, double *cfl_out);
+
+/* Step 3 of a substep alone: linearbl over all z planes (bla.f:2433-2475)
+ * with the weights an, bn, anp1, bnp1.  px_out (may be NULL) receives the
+ * pressure gradient afterwards (synchronises). */
+int simson_gpu_linear(simson_gpu_t *g, double an, double bn, double anp1,
+                      double bnp1, double *px_out);
+
+/* One RK substep: nonlinbl over all y planes (bla.f:2251-2288), then
+ * linearbl over all z planes (bla.f:2433-2475) with the weights an, bn of
+ * this substep and anp1, bnp1 of the next (bla.f:2384-2397).  When do_cfl
+ * is non-zero, *cfl_out receives the partial CFL max of boxcfl.f (multiply
+ * by pi and dt as wcflbl.f:87) and the call synchronises.  px_out (may be
+ * NULL) receives the pressure gradient after the substep (synchronises). */
+int simson_gpu_substep(simson_gpu_t *g, double an, double bn, double anp1,
+                       double bnp1, int do_cfl, double *cfl_out, double *px_out);
+
+/* nsteps full RK3 steps at fixed dt (bla.f:1465-1477 and the substep loop
+ * 1790-2610 with vart = .false., nst = 3), prhs included when first != 0.
+ * CFL*dt is evaluated every icfl substeps (icfl <= 0: never) and the run
+ * returns 324234 if it exceeded 10 (checked when the run ends); *cfl_out
+ * receives the last CFL*dt.  Launches are asynchronous until the end. */
+int simson_gpu_run(simson_gpu_t *g, int nsteps, double dt, int icfl, int first,
+                   double *cfl_out);
+
+/* Download nslots slots of main storage, puw and px (any may be NULL). */
+int simson_gpu_download(simson_gpu_t *g, double *ur, double *ui, int nslots,
+                        double *puw, double *px);
+
+/* Wait for all queued work. */
+int simson_gpu_sync(simson_gpu_t *g);
+
+/* Per-kernel timing with HIP events on the library's stream.  While
+ * enabled, every launch is bracketed by events; simson_gpu_timing_read
+ * fills up to cap entries of per-kernel-family totals (ms) and launch
+ * counts and returns the number of families.  Names are static strings. */
+int simson_gpu_timing_enable(simson_gpu_t *g, int on);
+int simson_gpu_timing_read(simson_gpu_t *g, const char **names, double *total_ms,
+                           long *count, int cap);
+
+/* Last HIP error text of this handle (static storage, never NULL). */
+const char *simson_gpu_last_error(const simson_gpu_t *g);
+
+void simson_gpu_finalize(simson_gpu_t *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,599 @@
+// C ABI of the MI355X Simson step library (declared in include/simson_gpu.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sg_internal.h"
+
+using namespace sg;
+
+namespace {
+
+const char *const kFamilies[] = {"zsyn", "xrow", "zana", "cfl_reduce", "cheb_fwd",
+                                 "tau", "cheb_bwd", "prhs_cheb_fwd", "prhs",
+                                 "prhs_cheb_bwd"};
+enum Fam { F_ZSYN, F_XROW, F_ZANA, F_CFL, F_CHF, F_TAU, F_CHB, F_PCHF, F_PRHS, F_PCHB, F_N };
+
+struct TimedLaunch {
+    int fam;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct simson_gpu {
+    simson_params_t p{};
+    int nx = 0, ny = 0, nz = 0, nxh = 0, nxp = 0, nzp = 0;
+    hipStream_t stream = nullptr;
+    double2 *U = nullptr, *P = nullptr, *C = nullptr, *twz = nullptr, *twx = nullptr;
+    double *W = nullptr, *puw = nullptr, *px = nullptr, *mf = nullptr;
+    double *cfl_blk = nullptr, *cfl = nullptr;
+    double *Tf = nullptr, *Tp = nullptr, *Tb = nullptr;
+    double *alfa = nullptr, *beta = nullptr, *deta = nullptr;
+    FftPlan pz, pm;
+    double mflux = 0., u0upp = 0., u0low = 0., w0upp = 0., w0low = 0.;
+    bool timing = false;
+    std::vector<TimedLaunch> launches;
+    std::vector<hipEvent_t> pool;
+    double tot_ms[F_N] = {};
+    long cnt[F_N] = {};
+    std::string err;
+};
+
+namespace {
+
+#define CK(expr)                                                                  \
+    do {                                                                          \
+        hipError_t e_ = (expr);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            g->err = std::string(#expr) + ": " + hipGetErrorString(e_);         \
+            return SIMSON_E_HIP;                                                  \
+        }                                                                         \
+    } while (0)
+
+hipEvent_t get_event(simson_gpu_t *g)
+{
+    if (!g->pool.empty()) {
+        hipEvent_t e = g->pool.back();
+        g->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess)
+        return nullptr;
+    return e;
+}
+
+template <class F>
+void timed(simson_gpu_t *g, int fam, F &&launch)
+{
+    if (!g->timing) {
+        launch();
+        return;
+    }
+    TimedLaunch t{fam, get_event(g), get_event(g)};
+    (void)hipEventRecord(t.a, g->stream);
+    launch();
+    (void)hipEventRecord(t.b, g->stream);
+    g->launches.push_back(t);
+}
+
+// exp(+2 pi i t/n) for t < n with the angle reduced to the first octant
+std::vector<double2> roots(int n)
+{
+    std::vector<double2> w(n);
+    for (int t = 0; t < n; t++) {
+        long num = 4L * t, q = num / n, r = num % n;
+        double a = 2.0 * M_PI * (double)r / (4.0 * (double)n);
+        double c = std::cos(a), s = std::sin(a);
+        switch (q & 3) {
+        case 0: w[t] = make_double2(c, s); break;
+        case 1: w[t] = make_double2(-s, c); break;
+        case 2: w[t] = make_double2(-c, -s); break;
+        default: w[t] = make_double2(s, -c); break;
+        }
+    }
+    return w;
+}
+
+bool make_radices(int n, FftPlan &pl)
+{
+    pl.n = n;
+    pl.nrad = 0;
+    int m = n;
+    const int order[4] = {4, 2, 3, 5};
+    for (int r : order) {
+        while (m % r == 0) {
+            if (pl.nrad >= kMaxRad)
+                return false;
+            pl.rad[pl.nrad++] = r;
+            m /= r;
+        }
+    }
+    return m == 1;
+}
+
+void free_all(simson_gpu_t *g)
+{
+    void *ptrs[] = {g->U, g->P, g->C, g->twz, g->twx, g->W, g->puw, g->px, g->mf,
+                    g->cfl_blk, g->cfl, g->Tf, g->Tp, g->Tb, g->alfa, g->beta, g->deta};
+    for (void *q : ptrs)
+        if (q)
+            (void)hipFree(q);
+    for (auto &t : g->launches) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    for (auto e : g->pool)
+        (void)hipEventDestroy(e);
+    if (g->stream)
+        (void)hipStreamDestroy(g->stream);
+}
+
+template <class T>
+int dalloc(simson_gpu_t *g, T **p, size_t n)
+{
+    CK(hipMalloc((void **)p, n * sizeof(T)));
+    CK(hipMemsetAsync(*p, 0, n * sizeof(T), g->stream));
+    return 0;
+}
+
+template <class T>
+int h2d(simson_gpu_t *g, T *d, const T *h, size_t n)
+{
+    CK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, g->stream));
+    return 0;
+}
+
+NonlinArgs nonlin_args(simson_gpu_t *g, int do_cfl)
+{
+    NonlinArgs a;
+    a.U = g->U;
+    a.Uw = g->U;
+    a.P = g->P;
+    a.alfa = g->alfa;
+    a.beta = g->beta;
+    a.deta = g->deta;
+    a.cfl_blk = g->cfl_blk;
+    a.nx = g->nx;
+    a.ny = g->ny;
+    a.nz = g->nz;
+    a.nxh = g->nxh;
+    a.nxp = g->nxp;
+    a.nzp = g->nzp;
+    a.rot = g->p.rot;
+    a.xl = g->p.xl;
+    a.zl = g->p.zl;
+    a.do_cfl = do_cfl;
+    return a;
+}
+
+ColArgs col_args(simson_gpu_t *g)
+{
+    ColArgs a;
+    a.U = g->U;
+    a.C = g->C;
+    a.W = g->W;
+    a.puw = g->puw;
+    a.px = g->px;
+    a.mf = g->mf;
+    a.alfa = g->alfa;
+    a.beta = g->beta;
+    a.ny = g->ny;
+    a.nz = g->nz;
+    a.nxh = g->nxh;
+    a.re = g->p.re;
+    a.mflux = g->mflux;
+    a.cflux = g->p.cflux;
+    a.u0upp = g->u0upp;
+    a.u0low = g->u0low;
+    a.w0upp = g->w0upp;
+    a.w0low = g->w0low;
+    return a;
+}
+
+GemmIO gemm_io(simson_gpu_t *g, const void *in, void *out, std::initializer_list<int> sin,
+               std::initializer_list<int> sout, int skip_z)
+{
+    GemmIO io{};
+    io.in = (const double *)in;
+    io.out = (double *)out;
+    int i = 0;
+    for (int s : sin)
+        io.in_slot[i++] = s;
+    io.nf = i;
+    i = 0;
+    for (int s : sout)
+        io.out_slot[i++] = s;
+    io.nz = g->nz;
+    io.ny = g->ny;
+    io.ncols = 2 * g->nxh;
+    io.skip_z = skip_z;
+    return io;
+}
+
+int do_nonlin(simson_gpu_t *g, int do_cfl)
+{
+    NonlinArgs a = nonlin_args(g, do_cfl);
+    timed(g, F_ZSYN, [&] { launch_zsyn(a, g->pz, g->stream); });
+    timed(g, F_XROW, [&] { launch_xrow(a, g->pm, g->twx, g->stream); });
+    if (do_cfl)
+        timed(g, F_CFL, [&] { launch_cfl_reduce(g->cfl_blk, xrow_blocks(a), g->cfl, g->stream); });
+    timed(g, F_ZANA, [&] { launch_zana(a, g->pz, g->stream); });
+    CK(hipGetLastError());
+    return 0;
+}
+
+int do_linear(simson_gpu_t *g, double an, double bn, double anp1, double bnp1)
+{
+    const int odd = g->nz / 2;
+    GemmIO fw = gemm_io(g, g->U, g->C, {0, 1, 2}, {0, 1, 2}, odd);
+    timed(g, F_CHF, [&] { launch_cheb_gemm(g->Tf, fw, g->stream); });
+    ColArgs ca = col_args(g);
+    timed(g, F_TAU, [&] { launch_tau(ca, an, bn, anp1, bnp1, g->stream); });
+    GemmIO bw = gemm_io(g, g->C, g->U, {0, 1, 2, 3, 4}, {0, 1, 2, 3, 4}, odd);
+    timed(g, F_CHB, [&] { launch_cheb_gemm(g->Tb, bw, g->stream); });
+    CK(hipGetLastError());
+    return 0;
+}
+
+int do_prhs(simson_gpu_t *g, double an)
+{
+    GemmIO fw = gemm_io(g, g->U, g->C, {0, 1, 2}, {0, 1, 2}, -1);
+    timed(g, F_PCHF, [&] { launch_cheb_gemm(g->Tp, fw, g->stream); });
+    ColArgs ca = col_args(g);
+    timed(g, F_PRHS, [&] { launch_prhs(ca, an, g->stream); });
+    GemmIO bw = gemm_io(g, g->C, g->U, {3, 4}, {3, 4}, -1);
+    timed(g, F_PCHB, [&] { launch_cheb_gemm(g->Tb, bw, g->stream); });
+    CK(hipGetLastError());
+    return 0;
+}
+
+int valid_fft_len(int n)
+{
+    FftPlan pl;
+    return make_radices(n, pl);
+}
+
+}  // namespace
+
+extern "C" {
+
+int simson_gpu_init(const simson_params_t *p, simson_gpu_t **out)
+{
+    if (!p || !out)
+        return SIMSON_E_ARG;
+    *out = nullptr;
+    if (p->precision != SIMSON_FP64 || p->ibc != 0 || p->nx < 4 || p->nz < 4 ||
+        p->nx % 4 != 0 || p->nz % 2 != 0 || p->ny < 9 || p->ny % 2 == 0 ||
+        p->re <= 0 || p->xl <= 0 || p->zl <= 0)
+        return SIMSON_E_ARG;
+    const int nxp = p->nx + p->nx / 2, nzp = p->nz + p->nz / 2;
+    if (!valid_fft_len(nzp) || !valid_fft_len(nxp / 2))
+        return SIMSON_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return SIMSON_E_NODEV;
+    simson_gpu_t *g = new simson_gpu_t;
+    g->p = *p;
+    g->nx = p->nx;
+    g->ny = p->ny;
+    g->nz = p->nz;
+    g->nxh = p->nx / 2;
+    g->nxp = nxp;
+    g->nzp = nzp;
+    auto fail = [&](int code) {
+        fprintf(stderr, "simson_gpu_init: %s\n", g->err.c_str());
+        free_all(g);
+        delete g;
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) {
+        g->err = "hipStreamCreate failed";
+        return fail(SIMSON_E_HIP);
+    }
+    const int nx = g->nx, ny = g->ny, nz = g->nz, nxh = g->nxh;
+    const size_t F = (size_t)nz * ny * nxh;
+    const size_t ncol = (size_t)nz * nxh;
+    const int xblk = nzp * ny;                  // >= xrow blocks for any RW >= 1
+    if (dalloc(g, &g->U, kNSlot * F) || dalloc(g, &g->P, (size_t)6 * ny * nzp * nxh) ||
+        dalloc(g, &g->C, 5 * F) || dalloc(g, &g->W, (size_t)kTauArrays * ny * ncol) ||
+        dalloc(g, &g->puw, 2 * (size_t)ny) || dalloc(g, &g->px, 1) ||
+        dalloc(g, &g->mf, 8 * (size_t)ny) || dalloc(g, &g->cfl_blk, (size_t)xblk) ||
+        dalloc(g, &g->cfl, 2) || dalloc(g, &g->Tf, (size_t)ny * ny) ||
+        dalloc(g, &g->Tp, (size_t)ny * ny) || dalloc(g, &g->Tb, (size_t)ny * ny) ||
+        dalloc(g, &g->alfa, (size_t)nxh) || dalloc(g, &g->beta, (size_t)nz) ||
+        dalloc(g, &g->deta, (size_t)ny) || dalloc(g, &g->twz, (size_t)nzp) ||
+        dalloc(g, &g->twx, (size_t)nxp))
+        return fail(SIMSON_E_HIP);
+    // wavenumbers and grid (preprbl.f:34-55)
+    std::vector<double> alfa(nxh), beta(nz), eta(ny), deta(ny);
+    const double pi = 3.14159265358979323846;
+    for (int x = 0; x < nxh; x++)
+        alfa[x] = 2. * pi / p->xl * (double)x;
+    beta[0] = 0.;
+    for (int z = 1; z <= nz / 2; z++) {
+        beta[z] = 2. * pi / p->zl * (double)z;
+        beta[nz - z] = -2. * pi / p->zl * (double)z;
+    }
+    std::vector<double2> rc = roots(2 * (ny - 1));   // exp(i pi m/(ny-1))
+    for (int y = 0; y < ny; y++)
+        eta[y] = rc[y].x;
+    deta[0] = 1. - eta[1];
+    deta[ny - 1] = deta[0];
+    for (int y = 1; y < ny - 1; y++)
+        deta[y] = (eta[y - 1] - eta[y + 1]) * .5;
+    // Chebyshev matrices: analysis (vchbf with c_0 = c_{n-1} = 1/2) times the
+    // linearbl scaling 2/(nxp (ny-1) nzp) or the prhs scaling 2/(ny-1);
+    // synthesis (vchbb) cos(pi j k/(ny-1)).
+    std::vector<double> Tf((size_t)ny * ny), Tp((size_t)ny * ny), Tb((size_t)ny * ny);
+    const double cn = 2. / ((double)nxp * (double)(ny - 1) * (double)nzp);
+    const double cp = 2. / (double)(ny - 1);
+    const int M2 = 2 * (ny - 1);
+    for (int k = 0; k < ny; k++) {
+        for (int j = 0; j < ny; j++) {
+            const double cs = rc[(long)k * j % M2].x;
+            const double ck = (k == 0 || k == ny - 1) ? 0.5 : 1.0;
+            const double cj = (j == 0 || j == ny - 1) ? 0.5 : 1.0;
+            Tf[(size_t)k * ny + j] = ck * cj * cs * cn;
+            Tp[(size_t)k * ny + j] = ck * cj * cs * cp;
+            Tb[(size_t)k * ny + j] = cs;
+        }
+    }
+    if (!make_radices(nzp, g->pz) || !make_radices(nxp / 2, g->pm)) {
+        g->err = "unsupported FFT length";
+        return fail(SIMSON_E_ARG);
+    }
+    std::vector<double2> twz = roots(nzp), twx = roots(nxp);
+    g->pz.tw = g->twz;
+    g->pz.tabn = nzp;
+    g->pm.tw = g->twx;         // length-nxp/2 transform uses every other root
+    g->pm.tabn = nxp;
+    if (h2d(g, g->alfa, alfa.data(), alfa.size()) || h2d(g, g->beta, beta.data(), beta.size()) ||
+        h2d(g, g->deta, deta.data(), deta.size()) || h2d(g, g->Tf, Tf.data(), Tf.size()) ||
+        h2d(g, g->Tp, Tp.data(), Tp.size()) || h2d(g, g->Tb, Tb.data(), Tb.size()) ||
+        h2d(g, g->twz, twz.data(), twz.size()) || h2d(g, g->twx, twx.data(), twx.size()))
+        return fail(SIMSON_E_HIP);
+    // pressure gradient set-up (bla.f:1622-1662)
+    double px;
+    if (p->cflux) {
+        g->mflux = 4. / 3.;
+        px = -2. / p->re;
+    } else {
+        g->mflux = 0.;
+        px = -(p->retau / p->re) * (p->retau / p->re);
+    }
+    if (h2d(g, g->px, &px, 1) || hipStreamSynchronize(g->stream) != hipSuccess)
+        return fail(SIMSON_E_HIP);
+    (void)nx;
+    *out = g;
+    return 0;
+}
+
+int simson_gpu_upload(simson_gpu_t *g, const double *ur, const double *ui, int nslots,
+                      const double *puw, double px)
+{
+    if (!g || !ur || !ui || (nslots != 3 && nslots != kNSlot) || (nslots == kNSlot && !puw))
+        return SIMSON_E_ARG;
+    const size_t F = (size_t)g->nz * g->ny * g->nxh;
+    std::vector<double2> h(nslots * F);
+    for (size_t i = 0; i < nslots * F; i++)
+        h[i] = make_double2(ur[i], ui[i]);
+    if (h2d(g, g->U, h.data(), h.size()))
+        return SIMSON_E_HIP;
+    if (puw && h2d(g, g->puw, puw, 2 * (size_t)g->ny))
+        return SIMSON_E_HIP;
+    if (h2d(g, g->px, &px, 1))
+        return SIMSON_E_HIP;
+    // wall values of the mean mode (rdiscbl.f:319-323), zeroed below 1e-8
+    // (bla.f:1041-1044)
+    const size_t ylast = (size_t)(g->ny - 1) * g->nxh;
+    double v[4] = {ur[0], ur[ylast], ur[2 * F], ur[2 * F + ylast]};
+    for (double &q : v)
+        if (std::fabs(q) < 1e-8)
+            q = 0.;
+    g->u0upp = v[0];
+    g->u0low = v[1];
+    g->w0upp = v[2];
+    g->w0low = v[3];
+    CK(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int simson_gpu_prhs(simson_gpu_t *g, double an)
+{
+    if (!g || an == 0.)
+        return SIMSON_E_ARG;
+    return do_prhs(g, an);
+}
+
+int simson_gpu_nonlin(simson_gpu_t *g, int do_cfl, double *cfl_out)
+{
+    if (!g)
+        return SIMSON_E_ARG;
+    int st = do_nonlin(g, do_cfl);
+    if (st)
+        return st;
+    if (do_cfl && cfl_out) {
+        double c[2];
+        CK(hipMemcpyAsync(c, g->cfl, 2 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+        CK(hipStreamSynchronize(g->stream));
+        *cfl_out = c[0];
+    }
+    return 0;
+}
+
+int simson_gpu_linear(simson_gpu_t *g, double an, double bn, double anp1, double bnp1,
+                      double *px_out)
+{
+    if (!g || an + bn == 0. || anp1 + bnp1 == 0.)
+        return SIMSON_E_ARG;
+    int st = do_linear(g, an, bn, anp1, bnp1);
+    if (st)
+        return st;
+    if (px_out) {
+        CK(hipMemcpyAsync(px_out, g->px, sizeof(double), hipMemcpyDeviceToHost, g->stream));
+        CK(hipStreamSynchronize(g->stream));
+    }
+    return 0;
+}
+
+int simson_gpu_substep(simson_gpu_t *g, double an, double bn, double anp1, double bnp1,
+                       int do_cfl, double *cfl_out, double *px_out)
+{
+    if (!g || an + bn == 0. || anp1 + bnp1 == 0.)
+        return SIMSON_E_ARG;
+    int st = do_nonlin(g, do_cfl);
+    if (st)
+        return st;
+    st = do_linear(g, an, bn, anp1, bnp1);
+    if (st)
+        return st;
+    if (do_cfl && cfl_out) {
+        double c[2];
+        CK(hipMemcpyAsync(c, g->cfl, 2 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+        CK(hipStreamSynchronize(g->stream));
+        *cfl_out = c[0];
+    }
+    if (px_out) {
+        CK(hipMemcpyAsync(px_out, g->px, sizeof(double), hipMemcpyDeviceToHost, g->stream));
+        CK(hipStreamSynchronize(g->stream));
+    }
+    return 0;
+}
+
+int simson_gpu_run(simson_gpu_t *g, int nsteps, double dt, int icfl, int first, double *cfl_out)
+{
+    if (!g || nsteps < 0 || !(dt > 0.))
+        return SIMSON_E_ARG;
+    static const double anrk[3] = {8. / 15., 5. / 12., 3. / 4.};
+    static const double bnrk[3] = {0., -17. / 60., -5. / 12.};
+    const int nst = 3;
+    const double dtn = dt, dtnp1 = dt;
+    double an = dtn * anrk[0], bn = 0., anp1 = an, bnp1 = 0.;
+    CK(hipMemsetAsync(g->cfl, 0, 2 * sizeof(double), g->stream));
+    if (first) {
+        int st = do_prhs(g, anp1);
+        if (st)
+            return st;
+    }
+    std::vector<double> cfls;
+    for (int it = 1; it <= nst * nsteps; it++) {
+        const int do_cfl = icfl > 0 && ((it - 1) % icfl) == 0;
+        int st = do_nonlin(g, do_cfl);
+        if (st)
+            return st;
+        an = anp1;
+        bn = bnp1;
+        anp1 = dtnp1 * anrk[it % nst];
+        bnp1 = dtnp1 * bnrk[it % nst];
+        st = do_linear(g, an, bn, anp1, bnp1);
+        if (st)
+            return st;
+    }
+    double c[2] = {0., 0.};
+    CK(hipMemcpyAsync(c, g->cfl, 2 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    CK(hipStreamSynchronize(g->stream));
+    const double pi = 3.14159265358979323846;
+    const double last = (c[0] * pi + 2. * std::fabs(g->p.rot)) * dt;   // wcflbl.f:87
+    const double worst = (c[1] * pi + 2. * std::fabs(g->p.rot)) * dt;
+    if (cfl_out)
+        *cfl_out = last;
+    if (icfl > 0 && worst > 10.)
+        return 324234;                                                  // wcflbl.f:93-98
+    return 0;
+}
+
+int simson_gpu_download(simson_gpu_t *g, double *ur, double *ui, int nslots, double *puw,
+                        double *px)
+{
+    if (!g || nslots < 0 || nslots > kNSlot || (nslots > 0 && (!ur || !ui)))
+        return SIMSON_E_ARG;
+    const size_t F = (size_t)g->nz * g->ny * g->nxh;
+    if (nslots > 0) {
+        std::vector<double2> h(nslots * F);
+        CK(hipMemcpyAsync(h.data(), g->U, h.size() * sizeof(double2), hipMemcpyDeviceToHost,
+                          g->stream));
+        CK(hipStreamSynchronize(g->stream));
+        for (size_t i = 0; i < nslots * F; i++) {
+            ur[i] = h[i].x;
+            ui[i] = h[i].y;
+        }
+    }
+    if (puw)
+        CK(hipMemcpyAsync(puw, g->puw, 2 * (size_t)g->ny * sizeof(double), hipMemcpyDeviceToHost,
+                          g->stream));
+    if (px)
+        CK(hipMemcpyAsync(px, g->px, sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    CK(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int simson_gpu_sync(simson_gpu_t *g)
+{
+    if (!g)
+        return SIMSON_E_ARG;
+    CK(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int simson_gpu_timing_enable(simson_gpu_t *g, int on)
+{
+    if (!g)
+        return SIMSON_E_ARG;
+    g->timing = on != 0;
+    return 0;
+}
+
+int simson_gpu_timing_read(simson_gpu_t *g, const char **names, double *total_ms, long *count,
+                           int cap)
+{
+    if (!g)
+        return SIMSON_E_ARG;
+    CK(hipStreamSynchronize(g->stream));
+    for (auto &t : g->launches) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            g->tot_ms[t.fam] += ms;
+            g->cnt[t.fam] += 1;
+        }
+        g->pool.push_back(t.a);
+        g->pool.push_back(t.b);
+    }
+    g->launches.clear();
+    int n = 0;
+    for (int f = 0; f < F_N && n < cap; f++) {
+        if (names)
+            names[n] = kFamilies[f];
+        if (total_ms)
+            total_ms[n] = g->tot_ms[f];
+        if (count)
+            count[n] = g->cnt[f];
+        n++;
+    }
+    for (int f = 0; f < F_N; f++) {
+        g->tot_ms[f] = 0.;
+        g->cnt[f] = 0;
+    }
+    return n;
+}
+
+const char *simson_gpu_last_error(const simson_gpu_t *g)
+{
+    return g ? g->err.c_str() : "null handle";
+}
+
+void simson_gpu_finalize(simson_gpu_t *g)
+{
+    if (!g)
+        return;
+    (void)hipStreamSynchronize(g->stream);
+    free_all(g);
+    delete g;
+}
+
+}  // extern "C"

@@ -1,0 +1,203 @@
+"""Host-side mirror of bla's step structure over the C ABI (ctypes).
+
+``BlaGPU`` owns one device state and exposes the calls bla.f makes on the hot
+path: ``prhs`` (bla.f:1562), ``substep`` (one RK substep = nonlinbl over all
+y planes then linearbl over all z planes, bla.f:2251-2475) and the fixed-dt
+driver ``run`` (bla.f:1465-1477, 2384-2397).  The library must be built
+(``python -m simson_amd.build``); there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .ufile import UField
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "_lib", "libsimson_gpu.so")
+
+SIMSON_FP64 = 0
+NSLOT = 7
+
+# Every entry point declared in include/simson_gpu.h.
+EXPORTS = (
+    "simson_gpu_init", "simson_gpu_upload", "simson_gpu_prhs", "simson_gpu_nonlin",
+    "simson_gpu_linear", "simson_gpu_substep",
+    "simson_gpu_run", "simson_gpu_download", "simson_gpu_sync",
+    "simson_gpu_timing_enable", "simson_gpu_timing_read", "simson_gpu_last_error",
+    "simson_gpu_finalize",
+)
+
+
+class SimsonParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("precision", C.c_int), ("ibc", C.c_int), ("cflux", C.c_int),
+        ("re", C.c_double), ("xl", C.c_double), ("zl", C.c_double),
+        ("rot", C.c_double), ("retau", C.c_double),
+    ]
+
+
+class SimsonError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"simson_gpu error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"HIP library missing: {LIBPATH} (python -m simson_amd.build)")
+        L = C.CDLL(LIBPATH)
+        vp = C.c_void_p
+        L.simson_gpu_init.argtypes = [C.POINTER(SimsonParams), C.POINTER(vp)]
+        L.simson_gpu_upload.argtypes = [vp, _dp, _dp, C.c_int, C.c_void_p, C.c_double]
+        L.simson_gpu_prhs.argtypes = [vp, C.c_double]
+        L.simson_gpu_nonlin.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
+        L.simson_gpu_linear.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                        C.POINTER(C.c_double)]
+        L.simson_gpu_substep.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                         C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.simson_gpu_run.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_int,
+                                     C.POINTER(C.c_double)]
+        L.simson_gpu_download.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                          C.POINTER(C.c_double)]
+        L.simson_gpu_sync.argtypes = [vp]
+        L.simson_gpu_timing_enable.argtypes = [vp, C.c_int]
+        L.simson_gpu_timing_read.argtypes = [vp, C.POINTER(C.c_char_p),
+                                             C.POINTER(C.c_double), C.POINTER(C.c_long),
+                                             C.c_int]
+        L.simson_gpu_last_error.restype = C.c_char_p
+        L.simson_gpu_last_error.argtypes = [vp]
+        L.simson_gpu_finalize.argtypes = [vp]
+        for name in EXPORTS:
+            getattr(L, name).restype = getattr(L, name).restype or C.c_int
+        L.simson_gpu_finalize.restype = None
+        _lib = L
+    return _lib
+
+
+class BlaGPU:
+    """Device-resident bla state for one channel configuration."""
+
+    def __init__(self, nx: int, ny: int, nz: int, re: float, xl: float, zl: float,
+                 rot: float = 0.0, cflux: bool = True, retau: float = 0.0):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        p = SimsonParams(nx, ny, nz, SIMSON_FP64, 0, int(cflux), re, xl, zl, rot, retau)
+        h = C.c_void_p()
+        self._check(lib().simson_gpu_init(C.byref(p), C.byref(h)))
+        self.h = h
+        self.rot = rot
+
+    @classmethod
+    def from_ufield(cls, f: UField, xl: float | None = None, **kw) -> "BlaGPU":
+        return cls(f.nx, f.nyp, f.nz, f.re, f.xl if xl is None else xl, f.zl, **kw)
+
+    def _check(self, st: int):
+        if st != 0:
+            msg = lib().simson_gpu_last_error(getattr(self, "h", None) or None)
+            raise SimsonError(st, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().simson_gpu_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # --- data movement -------------------------------------------------
+    def upload_velocity(self, u: np.ndarray, px: float):
+        """Slots 0-2 from complex velocities u[comp, z, y, kx]."""
+        ur = np.ascontiguousarray(u.real, dtype=np.float64)
+        ui = np.ascontiguousarray(u.imag, dtype=np.float64)
+        self._check(lib().simson_gpu_upload(self.h, ur, ui, 3, None, px))
+
+    def upload_state(self, ur: np.ndarray, ui: np.ndarray, puw: np.ndarray, px: float):
+        ur = np.ascontiguousarray(ur, dtype=np.float64)
+        ui = np.ascontiguousarray(ui, dtype=np.float64)
+        puw = np.ascontiguousarray(puw, dtype=np.float64)
+        self._check(lib().simson_gpu_upload(self.h, ur, ui, NSLOT,
+                                            puw.ctypes.data_as(C.c_void_p), px))
+
+    def download(self, nslots: int = NSLOT):
+        shape = (nslots, self.nz, self.ny, self.nx // 2)
+        ur = np.zeros(shape)
+        ui = np.zeros(shape)
+        puw = np.zeros((2, self.ny))
+        px = C.c_double()
+        self._check(lib().simson_gpu_download(self.h, ur.ctypes.data_as(C.c_void_p),
+                                              ui.ctypes.data_as(C.c_void_p), nslots,
+                                              puw.ctypes.data_as(C.c_void_p), C.byref(px)))
+        return ur, ui, puw, px.value
+
+    def velocities(self) -> np.ndarray:
+        ur, ui, _, _ = self.download(3)
+        return ur + 1j * ui
+
+    # --- the hot path ----------------------------------------------------
+    def prhs(self, an: float):
+        self._check(lib().simson_gpu_prhs(self.h, an))
+
+    def nonlin(self, do_cfl=False) -> float:
+        """Step 2 only (nonlinbl over all y planes); returns the CFL partial."""
+        cfl = C.c_double(0.0)
+        self._check(lib().simson_gpu_nonlin(self.h, int(do_cfl), C.byref(cfl)))
+        return cfl.value
+
+    def linear(self, an, bn, anp1, bnp1) -> float:
+        """Step 3 only (linearbl over all z planes); returns px afterwards."""
+        px = C.c_double(0.0)
+        self._check(lib().simson_gpu_linear(self.h, an, bn, anp1, bnp1, C.byref(px)))
+        return px.value
+
+    def substep(self, an, bn, anp1, bnp1, do_cfl=False):
+        cfl = C.c_double(0.0)
+        px = C.c_double(0.0)
+        self._check(lib().simson_gpu_substep(self.h, an, bn, anp1, bnp1, int(do_cfl),
+                                             C.byref(cfl), C.byref(px)))
+        return cfl.value, px.value
+
+    def run(self, nsteps: int, dt: float, icfl: int = 0, first: bool = True) -> float:
+        """nsteps fixed-dt RK3 steps; returns the last CFL*dt (0 if never)."""
+        cfl = C.c_double(0.0)
+        self._check(lib().simson_gpu_run(self.h, nsteps, dt, icfl, int(first), C.byref(cfl)))
+        return cfl.value
+
+    def sync(self):
+        self._check(lib().simson_gpu_sync(self.h))
+
+    # --- timing ------------------------------------------------------------
+    def timing(self, on: bool):
+        self._check(lib().simson_gpu_timing_enable(self.h, int(on)))
+
+    def timing_read(self) -> dict:
+        cap = 32
+        names = (C.c_char_p * cap)()
+        ms = (C.c_double * cap)()
+        cnt = (C.c_long * cap)()
+        n = lib().simson_gpu_timing_read(self.h, names, ms, cnt, cap)
+        if n < 0:
+            self._check(n)
+        return {names[i].decode(): (ms[i], cnt[i]) for i in range(n) if cnt[i] > 0}
+
+
+def rk3_weights(dt: float):
+    """Wray RK3 weights a, b, c (rkcoeff.f:21-32) scaled by dt."""
+    a = (8. / 15., 5. / 12., 3. / 4.)
+    b = (0., -17. / 60., -5. / 12.)
+    return [dt * x for x in a], [dt * x for x in b]
