@@ -15,8 +15,9 @@ namespace {
 
 const char *const kFamilies[] = {"zsyn", "xrow", "zana", "cfl_reduce", "cheb_fwd",
                                  "tau", "cheb_bwd", "prhs_cheb_fwd", "prhs",
-                                 "prhs_cheb_bwd"};
-enum Fam { F_ZSYN, F_XROW, F_ZANA, F_CFL, F_CHF, F_TAU, F_CHB, F_PCHF, F_PRHS, F_PCHB, F_N };
+                                 "prhs_cheb_bwd", "tau_factor"};
+enum Fam { F_ZSYN, F_XROW, F_ZANA, F_CFL, F_CHF, F_TAU, F_CHB, F_PCHF, F_PRHS, F_PCHB,
+           F_FAC, F_N };
 
 struct TimedLaunch {
     int fam;
@@ -29,9 +30,19 @@ struct simson_gpu {
     simson_params_t p{};
     int nx = 0, ny = 0, nz = 0, nxh = 0, nxp = 0, nzp = 0;
     hipStream_t stream = nullptr;
-    double2 *U = nullptr, *P = nullptr, *C = nullptr, *twz = nullptr, *twx = nullptr;
-    double *W = nullptr, *puw = nullptr, *px = nullptr, *mf = nullptr;
+    // U: slots 0-4 at physical y, [5][nz][ny][nx/2]; C: Chebyshev columns
+    // [5][nz][nx/2][ny]; Pp: partial right-hand sides p_v, p_omega as
+    // Chebyshev columns [2][nz][nx/2][ny]; P: step-2 planes.
+    double2 *U = nullptr, *P = nullptr, *C = nullptr, *Pp = nullptr;
+    double2 *twz = nullptr, *twx = nullptr;
+    double *W = nullptr, *puw = nullptr, *px = nullptr;
     double *cfl_blk = nullptr, *cfl = nullptr;
+    // tau factorisations: k^2 once, lam^2 for up to three values of a+b
+    LamFactor kfac;
+    LamFactor lam[3];
+    double lam_apb[3] = {0., 0., 0.};
+    long lam_used[3] = {0, 0, 0};
+    long clock = 0;
     double *Tf = nullptr, *Tp = nullptr, *Tb = nullptr;
     double *alfa = nullptr, *beta = nullptr, *deta = nullptr;
     FftPlan pz, pm;
@@ -119,8 +130,11 @@ bool make_radices(int n, FftPlan &pl)
 
 void free_all(simson_gpu_t *g)
 {
-    void *ptrs[] = {g->U, g->P, g->C, g->twz, g->twx, g->W, g->puw, g->px, g->mf,
-                    g->cfl_blk, g->cfl, g->Tf, g->Tp, g->Tb, g->alfa, g->beta, g->deta};
+    void *ptrs[] = {g->U, g->P, g->C, g->Pp, g->twz, g->twx, g->W, g->puw, g->px,
+                    g->cfl_blk, g->cfl, g->Tf, g->Tp, g->Tb, g->alfa, g->beta, g->deta,
+                    g->kfac.gp, g->lam[0].gp, g->lam[0].vh, g->lam[0].sh,
+                    g->lam[1].gp, g->lam[1].vh, g->lam[1].sh,
+                    g->lam[2].gp, g->lam[2].vh, g->lam[2].sh};
     for (void *q : ptrs)
         if (q)
             (void)hipFree(q);
@@ -177,10 +191,10 @@ ColArgs col_args(simson_gpu_t *g)
     ColArgs a;
     a.U = g->U;
     a.C = g->C;
+    a.Pp = g->Pp;
     a.W = g->W;
     a.puw = g->puw;
     a.px = g->px;
-    a.mf = g->mf;
     a.alfa = g->alfa;
     a.beta = g->beta;
     a.ny = g->ny;
@@ -196,8 +210,9 @@ ColArgs col_args(simson_gpu_t *g)
     return a;
 }
 
-GemmIO gemm_io(simson_gpu_t *g, const void *in, void *out, std::initializer_list<int> sin,
-               std::initializer_list<int> sout, int skip_z)
+// in_col / out_col: the input / output planes are Chebyshev columns
+GemmIO gemm_io(simson_gpu_t *g, const void *in, int in_col, void *out, int out_col,
+               std::initializer_list<int> sin, std::initializer_list<int> sout, int skip_z)
 {
     GemmIO io{};
     io.in = (const double *)in;
@@ -211,9 +226,32 @@ GemmIO gemm_io(simson_gpu_t *g, const void *in, void *out, std::initializer_list
         io.out_slot[i++] = s;
     io.nz = g->nz;
     io.ny = g->ny;
-    io.ncols = 2 * g->nxh;
+    io.nxh = g->nxh;
+    io.in_col = in_col;
+    io.out_col = out_col;
     io.skip_z = skip_z;
     return io;
+}
+
+// Factorisation slot for lam^2 = k^2 + 2 Re/apb; computed when apb is new
+// (three values recur with a fixed dt).
+int lam_slot(simson_gpu_t *g, double apb)
+{
+    g->clock++;
+    for (int i = 0; i < 3; i++)
+        if (g->lam_used[i] && g->lam_apb[i] == apb) {
+            g->lam_used[i] = g->clock;
+            return i;
+        }
+    int v = 0;
+    for (int i = 1; i < 3; i++)
+        if (g->lam_used[i] < g->lam_used[v])
+            v = i;
+    ColArgs ca = col_args(g);
+    timed(g, F_FAC, [&] { launch_factor_lam(ca, apb, g->lam[v], g->kfac, g->stream); });
+    g->lam_apb[v] = apb;
+    g->lam_used[v] = g->clock;
+    return v;
 }
 
 int do_nonlin(simson_gpu_t *g, int do_cfl)
@@ -231,11 +269,19 @@ int do_nonlin(simson_gpu_t *g, int do_cfl)
 int do_linear(simson_gpu_t *g, double an, double bn, double anp1, double bnp1)
 {
     const int odd = g->nz / 2;
-    GemmIO fw = gemm_io(g, g->U, g->C, {0, 1, 2}, {0, 1, 2}, odd);
+    const int slot = lam_slot(g, an + bn);
+    GemmIO fw = gemm_io(g, g->U, 0, g->C, 1, {0, 1, 2}, {0, 1, 2}, odd);
     timed(g, F_CHF, [&] { launch_cheb_gemm(g->Tf, fw, g->stream); });
     ColArgs ca = col_args(g);
-    timed(g, F_TAU, [&] { launch_tau(ca, an, bn, anp1, bnp1, g->stream); });
-    GemmIO bw = gemm_io(g, g->C, g->U, {0, 1, 2, 3, 4}, {0, 1, 2, 3, 4}, odd);
+    int st = 0;
+    timed(g, F_TAU, [&] {
+        st = launch_tau(ca, g->lam[slot], g->kfac, an, bn, anp1, bnp1, g->stream);
+    });
+    if (st) {
+        g->err = "unsupported ny for the column solver";
+        return st;
+    }
+    GemmIO bw = gemm_io(g, g->C, 1, g->U, 0, {0, 1, 2, 3, 4}, {0, 1, 2, 3, 4}, odd);
     timed(g, F_CHB, [&] { launch_cheb_gemm(g->Tb, bw, g->stream); });
     CK(hipGetLastError());
     return 0;
@@ -243,11 +289,11 @@ int do_linear(simson_gpu_t *g, double an, double bn, double anp1, double bnp1)
 
 int do_prhs(simson_gpu_t *g, double an)
 {
-    GemmIO fw = gemm_io(g, g->U, g->C, {0, 1, 2}, {0, 1, 2}, -1);
+    GemmIO fw = gemm_io(g, g->U, 0, g->C, 1, {0, 1, 2}, {0, 1, 2}, -1);
     timed(g, F_PCHF, [&] { launch_cheb_gemm(g->Tp, fw, g->stream); });
     ColArgs ca = col_args(g);
     timed(g, F_PRHS, [&] { launch_prhs(ca, an, g->stream); });
-    GemmIO bw = gemm_io(g, g->C, g->U, {3, 4}, {3, 4}, -1);
+    GemmIO bw = gemm_io(g, g->C, 1, g->U, 0, {3, 4}, {3, 4}, -1);
     timed(g, F_PCHB, [&] { launch_cheb_gemm(g->Tb, bw, g->stream); });
     CK(hipGetLastError());
     return 0;
@@ -300,10 +346,16 @@ int simson_gpu_init(const simson_params_t *p, simson_gpu_t **out)
     const size_t F = (size_t)nz * ny * nxh;
     const size_t ncol = (size_t)nz * nxh;
     const int xblk = nzp * ny;                  // >= xrow blocks for any RW >= 1
-    if (dalloc(g, &g->U, kNSlot * F) || dalloc(g, &g->P, (size_t)6 * ny * nzp * nxh) ||
-        dalloc(g, &g->C, 5 * F) || dalloc(g, &g->W, (size_t)kTauArrays * ny * ncol) ||
+    if (dalloc(g, &g->U, kNPhys * F) || dalloc(g, &g->P, (size_t)6 * ny * nzp * nxh) ||
+        dalloc(g, &g->C, 5 * F) || dalloc(g, &g->Pp, 2 * F) ||
+        dalloc(g, &g->W, (size_t)kScratch * ny * ncol) ||
+        dalloc(g, &g->kfac.gp, F) || dalloc(g, &g->lam[0].gp, F) ||
+        dalloc(g, &g->lam[1].gp, F) || dalloc(g, &g->lam[2].gp, F) ||
+        dalloc(g, &g->lam[0].vh, F) || dalloc(g, &g->lam[1].vh, F) ||
+        dalloc(g, &g->lam[2].vh, F) || dalloc(g, &g->lam[0].sh, ncol) ||
+        dalloc(g, &g->lam[1].sh, ncol) || dalloc(g, &g->lam[2].sh, ncol) ||
         dalloc(g, &g->puw, 2 * (size_t)ny) || dalloc(g, &g->px, 1) ||
-        dalloc(g, &g->mf, 8 * (size_t)ny) || dalloc(g, &g->cfl_blk, (size_t)xblk) ||
+        dalloc(g, &g->cfl_blk, (size_t)xblk) ||
         dalloc(g, &g->cfl, 2) || dalloc(g, &g->Tf, (size_t)ny * ny) ||
         dalloc(g, &g->Tp, (size_t)ny * ny) || dalloc(g, &g->Tb, (size_t)ny * ny) ||
         dalloc(g, &g->alfa, (size_t)nxh) || dalloc(g, &g->beta, (size_t)nz) ||
@@ -367,8 +419,15 @@ int simson_gpu_init(const simson_params_t *p, simson_gpu_t **out)
         g->mflux = 0.;
         px = -(p->retau / p->re) * (p->retau / p->re);
     }
-    if (h2d(g, g->px, &px, 1) || hipStreamSynchronize(g->stream) != hipSuccess)
+    if (h2d(g, g->px, &px, 1))
         return fail(SIMSON_E_HIP);
+    // tau factorisation for lam = k^2 (fixed for the run)
+    ColArgs ca = col_args(g);
+    launch_factor_k2(ca, g->kfac, g->stream);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(g->stream) != hipSuccess) {
+        g->err = "k^2 factorisation failed";
+        return fail(SIMSON_E_HIP);
+    }
     (void)nx;
     *out = g;
     return 0;
@@ -380,11 +439,28 @@ int simson_gpu_upload(simson_gpu_t *g, const double *ur, const double *ui, int n
     if (!g || !ur || !ui || (nslots != 3 && nslots != kNSlot) || (nslots == kNSlot && !puw))
         return SIMSON_E_ARG;
     const size_t F = (size_t)g->nz * g->ny * g->nxh;
-    std::vector<double2> h(nslots * F);
-    for (size_t i = 0; i < nslots * F; i++)
+    const int nphys = nslots < kNPhys ? nslots : kNPhys;
+    std::vector<double2> h(nphys * F);
+    for (size_t i = 0; i < nphys * F; i++)
         h[i] = make_double2(ur[i], ui[i]);
     if (h2d(g, g->U, h.data(), h.size()))
         return SIMSON_E_HIP;
+    if (nslots == kNSlot) {
+        // p_v, p_omega: main-storage [z][y][x] -> Chebyshev columns [z][x][y]
+        std::vector<double2> hp(2 * F);
+        const int nz = g->nz, ny = g->ny, nxh = g->nxh;
+        for (int s = 0; s < 2; s++)
+            for (int z = 0; z < nz; z++)
+                for (int y = 0; y < ny; y++)
+                    for (int x = 0; x < nxh; x++) {
+                        const size_t src = (((size_t)(kNPhys + s) * nz + z) * ny + y) * nxh + x;
+                        const size_t dst = (((size_t)s * nz + z) * nxh + x) * ny + y;
+                        hp[dst] = make_double2(ur[src], ui[src]);
+                    }
+        if (h2d(g, g->Pp, hp.data(), hp.size()))
+            return SIMSON_E_HIP;
+        CK(hipStreamSynchronize(g->stream));
+    }
     if (puw && h2d(g, g->puw, puw, 2 * (size_t)g->ny))
         return SIMSON_E_HIP;
     if (h2d(g, g->px, &px, 1))
@@ -515,14 +591,32 @@ int simson_gpu_download(simson_gpu_t *g, double *ur, double *ui, int nslots, dou
         return SIMSON_E_ARG;
     const size_t F = (size_t)g->nz * g->ny * g->nxh;
     if (nslots > 0) {
-        std::vector<double2> h(nslots * F);
+        const int nphys = nslots < kNPhys ? nslots : kNPhys;
+        std::vector<double2> h(nphys * F);
         CK(hipMemcpyAsync(h.data(), g->U, h.size() * sizeof(double2), hipMemcpyDeviceToHost,
                           g->stream));
         CK(hipStreamSynchronize(g->stream));
-        for (size_t i = 0; i < nslots * F; i++) {
+        for (size_t i = 0; i < nphys * F; i++) {
             ur[i] = h[i].x;
             ui[i] = h[i].y;
         }
+    }
+    if (nslots > kNPhys) {
+        // Chebyshev columns [z][x][y] -> main-storage [z][y][x]
+        std::vector<double2> hp(2 * F);
+        CK(hipMemcpyAsync(hp.data(), g->Pp, hp.size() * sizeof(double2), hipMemcpyDeviceToHost,
+                          g->stream));
+        CK(hipStreamSynchronize(g->stream));
+        const int nz = g->nz, ny = g->ny, nxh = g->nxh;
+        for (int s = 0; s < nslots - kNPhys; s++)
+            for (int z = 0; z < nz; z++)
+                for (int y = 0; y < ny; y++)
+                    for (int x = 0; x < nxh; x++) {
+                        const size_t dst = (((size_t)(kNPhys + s) * nz + z) * ny + y) * nxh + x;
+                        const size_t src = (((size_t)s * nz + z) * nxh + x) * ny + y;
+                        ur[dst] = hp[src].x;
+                        ui[dst] = hp[src].y;
+                    }
     }
     if (puw)
         CK(hipMemcpyAsync(puw, g->puw, 2 * (size_t)g->ny * sizeof(double), hipMemcpyDeviceToHost,

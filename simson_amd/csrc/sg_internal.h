@@ -10,8 +10,9 @@
 namespace sg {
 
 constexpr int kMaxRad = 16;
-constexpr int kNSlot = 7;          // u v w om_x om_z p_v p_om
-constexpr int kTauArrays = 28;     // per-column scratch arrays of the tau kernel
+constexpr int kNSlot = 7;          // host-visible slots: u v w om_x om_z p_v p_om
+constexpr int kNPhys = 5;          // slots kept at physical y on the device
+constexpr int kScratch = 16;       // per-column scratch arrays of the start-up kernels
 
 // Radix schedule and twiddle table of one FFT length.
 struct FftPlan {
@@ -22,8 +23,10 @@ struct FftPlan {
     int tabn = 0;
 };
 
-// Which planes a batched Chebyshev GEMM reads and writes.  Plane (f, z) of
-// a [slot][z][ny][ncols] array starts at ((slot*nz)+z)*ny*ncols doubles.
+// Which planes a batched Chebyshev GEMM reads and writes.
+// Physical planes ("row" layout, main storage) are [slot][z][y][x] complex;
+// Chebyshev planes ("column" layout) are [slot][z][x][k] complex, so that
+// each (kx,kz) column is contiguous for the per-column solver.
 struct GemmIO {
     const double *in;
     double *out;
@@ -31,19 +34,31 @@ struct GemmIO {
     int out_slot[8];
     int nf;          // number of fields
     int nz;          // planes per field
-    int ny;          // rows (Chebyshev length)
-    int ncols;       // doubles per row (2 * nx/2)
+    int ny;          // Chebyshev length
+    int nxh;         // complex columns per plane
+    int in_col;      // input in column layout
+    int out_col;     // output in column layout
     int skip_z;      // plane to skip (z-oddball) or -1
 };
 
-// Arguments of the per-(kx,kz)-column solver kernels.
+// Factorisation of the tau Helmholtz operator u'' - lam u for one lam per
+// column, [z][x][k]: gamma_k (bottom-up elimination multiplier) and
+// 1/pivot_k; for the lam^2 = k^2 + 2 Re/(a+b) stages also the homogeneous
+// wall-normal velocity v_h and its parity sums sum_k k^2 v_h,k.
+struct LamFactor {
+    double2 *gp = nullptr;     // (gamma, 1/pivot)
+    double *vh = nullptr;      // v_h (lam^2 stages only)
+    double2 *sh = nullptr;     // [z][x] per-parity sums (lam^2 stages only)
+};
+
+// Arguments of the per-(kx,kz)-column kernels.
 struct ColArgs {
-    double2 *U;           // main storage [7][nz][ny][nxh]
-    double2 *C;           // Chebyshev planes [5][nz][ny][nxh]
-    double *W;            // scratch [kTauArrays][ny][nz*nxh]
+    double2 *U;           // physical slots [5][nz][ny][nxh]
+    double2 *C;           // Chebyshev columns [5][nz][nxh][ny]
+    double2 *Pp;          // partial RHS columns [2][nz][nxh][ny]
+    double *W;            // start-up scratch [kScratch][ny][nz*nxh]
     double *puw;          // [2][ny]
     double *px;           // device scalar
-    double *mf;           // mean-flow scratch [8][ny]
     const double *alfa;   // [nxh]
     const double *beta;   // [nz]
     int ny, nz, nxh;
@@ -73,8 +88,14 @@ int  xrow_blocks(const NonlinArgs &a);
 void launch_cfl_reduce(const double *blk, int n, double *out, hipStream_t s);
 
 void launch_cheb_gemm(const double *A, const GemmIO &io, hipStream_t s);
-void launch_tau(const ColArgs &a, double an, double bn, double anp1, double bnp1,
-                hipStream_t s);
 void launch_prhs(const ColArgs &a, double an, hipStream_t s);
+
+// tau factorisations: k^2 once, lam^2 for one value of a+b
+void launch_factor_k2(const ColArgs &a, LamFactor kf, hipStream_t s);
+void launch_factor_lam(const ColArgs &a, double apb, LamFactor lf, LamFactor kf,
+                       hipStream_t s);
+// one linear substep, one wavefront per column
+int  launch_tau(const ColArgs &a, LamFactor lf, LamFactor kf, double an, double bn,
+                double anp1, double bnp1, hipStream_t s);
 
 }  // namespace sg

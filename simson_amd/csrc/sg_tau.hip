@@ -1,0 +1,554 @@
+// Step 3 of the bla substep (bla/linearbl.f, tau method, ibc = 0) with one
+// wavefront per (kx,kz) column.
+//
+// Lane L of the wave holds, for parity class c (k = c mod 2), the E class
+// elements M = L*E + e, i.e. Chebyshev index k = 2M + c; both classes and real
+// and imaginary parts live in registers.  Every sequential recurrence of the
+// reference becomes a wave-level scan:
+//   * dcheb (dcheb.f:24-36): b_k is a suffix sum over the opposite class,
+//     b_2M = sum_{M'>=M} 2(2M'+1) a_{2M'+1}, b_2M+1 = sum_{M'>M} 4M' a_{2M'};
+//   * the tau solves (setmacrhs.f + trid.f): with the per-column factors
+//     gamma_k, 1/pivot_k precomputed for each lam (sg_linear.hip), the
+//     bottom-up sweep rho_k = (R_k - U_k rho_{k+2})/piv_k is a suffix scan
+//     of affine maps, the forward substitution a_k = rho_k - gamma_k a_{k-2}
+//     a prefix scan of affine maps, and the boundary row a sum;
+//   * parity sums (dv/dy at the wall, the mass flux) are wave reductions.
+#include "sg_internal.h"
+#include "sg_tau.h"
+
+namespace sg {
+
+namespace {
+
+constexpr int WV = 64;
+
+__device__ __forceinline__ double shd(double x, int s) { return __shfl_down(x, s, WV); }
+__device__ __forceinline__ double shu(double x, int s) { return __shfl_up(x, s, WV); }
+
+// inclusive suffix sums over the class index of NV channels
+template <int NV, int E>
+__device__ __forceinline__ void suffix_sum(double (&x)[NV][E], int L)
+{
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+#pragma unroll
+        for (int e = E - 2; e >= 0; e--)
+            x[v][e] += x[v][e + 1];
+    double ex[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        ex[v] = shd(x[v][0], 1);
+        if (L == WV - 1)
+            ex[v] = 0.0;
+    }
+#pragma unroll
+    for (int s = 1; s < WV; s <<= 1) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            const double tv = shd(ex[v], s);
+            if (L + s < WV)
+                ex[v] += tv;
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+#pragma unroll
+        for (int e = 0; e < E; e++)
+            x[v][e] += ex[v];
+}
+
+// sum over the wave, identical on every lane (reduced, then lane 0 broadcast)
+template <int NV>
+__device__ __forceinline__ void wave_sum(double (&x)[NV])
+{
+#pragma unroll
+    for (int m = 1; m < WV; m <<= 1)
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+            x[v] += __shfl_xor(x[v], m, WV);
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+        x[v] = __shfl(x[v], 0, WV);
+}
+
+// d/dy in Chebyshev space of NC channels (dcheb.f:24-36)
+template <int NC, int E>
+__device__ __forceinline__ void dcheb_w(const double (&a)[NC][2][E], double (&b)[NC][2][E], int L)
+{
+    double s[2 * NC][E];
+    double nxt[NC];
+#pragma unroll
+    for (int q = 0; q < NC; q++) {
+        nxt[q] = shd(a[q][0][0], 1);            // next lane's first even element
+        if (L == WV - 1)
+            nxt[q] = 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int M = L * E + e;
+        const double wo = 2.0 * (double)(2 * M + 1);
+        const double we = 4.0 * (double)(M + 1);
+#pragma unroll
+        for (int q = 0; q < NC; q++) {
+            s[q][e] = wo * a[q][1][e];
+            s[NC + q][e] = we * ((e + 1 < E) ? a[q][0][e + 1] : nxt[q]);
+        }
+    }
+    suffix_sum<2 * NC, E>(s, L);
+#pragma unroll
+    for (int e = 0; e < E; e++)
+#pragma unroll
+        for (int q = 0; q < NC; q++) {
+            b[q][0][e] = s[q][e];
+            b[q][1][e] = s[NC + q][e];
+        }
+    if (L == 0)
+#pragma unroll
+        for (int q = 0; q < NC; q++)
+            b[q][0][0] *= 0.5;
+}
+
+// Tau solve of u'' - lam u = f for NC right-hand sides sharing lam (one
+// column), boundary rows bc[q][class]; gp holds (gamma, 1/pivot).
+template <int NC, int E>
+__device__ __forceinline__ void solve_w(int n, double lam, const double2 (&gp)[2][E],
+                                        const double (&f)[NC][2][E], const double (&bc)[NC][2],
+                                        double (&u)[NC][2][E], int L)
+{
+    double al[NC][2][E], be[2][E], bq[2][E];
+    double fprev[NC][2], fnext[NC][2];
+#pragma unroll
+    for (int q = 0; q < NC; q++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            fprev[q][c] = shu(f[q][c][E - 1], 1);
+            fnext[q][c] = shd(f[q][c][0], 1);
+            if (L == 0)
+                fprev[q][c] = 0.0;
+            if (L == WV - 1)
+                fnext[q][c] = 0.0;
+        }
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int M = L * E + e, k = 2 * M + c;
+            const bool row = (M >= 1) && (k < n);
+            const double ip = gp[c][e].y;
+            be[c][e] = row ? -tau_up(n, k, lam) * ip : 0.0;
+            bq[c][e] = (M == 0) ? 1.0 : (k < n ? -gp[c][e].x : 0.0);
+#pragma unroll
+            for (int q = 0; q < NC; q++) {
+                const double fm = e > 0 ? f[q][c][e - 1] : fprev[q][c];
+                const double fp = e + 1 < E ? f[q][c][e + 1] : fnext[q][c];
+                al[q][c][e] = row ? tau_rhs(n, k, fm, f[q][c][e], fp) * ip : 0.0;
+            }
+        }
+    // rho_M = al_M + be_M rho_{M+1}: local composition then suffix scan
+    double A[NC][2][E], B[2][E];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        double accB = 1.0, accA[NC];
+#pragma unroll
+        for (int q = 0; q < NC; q++)
+            accA[q] = 0.0;
+#pragma unroll
+        for (int e = E - 1; e >= 0; e--) {
+#pragma unroll
+            for (int q = 0; q < NC; q++) {
+                accA[q] = al[q][c][e] + be[c][e] * accA[q];
+                A[q][c][e] = accA[q];
+            }
+            accB = be[c][e] * accB;
+            B[c][e] = accB;
+        }
+    }
+    double X[NC][2], Y[2];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        Y[c] = shd(B[c][0], 1);
+#pragma unroll
+        for (int q = 0; q < NC; q++)
+            X[q][c] = shd(A[q][c][0], 1);
+        if (L == WV - 1) {
+            Y[c] = 1.0;
+#pragma unroll
+            for (int q = 0; q < NC; q++)
+                X[q][c] = 0.0;
+        }
+    }
+#pragma unroll
+    for (int s = 1; s < WV; s <<= 1) {
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const double Yn = shd(Y[c], s);
+            double Xn[NC];
+#pragma unroll
+            for (int q = 0; q < NC; q++)
+                Xn[q] = shd(X[q][c], s);
+            if (L + s < WV) {
+#pragma unroll
+                for (int q = 0; q < NC; q++)
+                    X[q][c] = X[q][c] + Y[c] * Xn[q];
+                Y[c] = Y[c] * Yn;
+            }
+        }
+    }
+    double rho[NC][2][E];
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int e = 0; e < E; e++)
+#pragma unroll
+            for (int q = 0; q < NC; q++)
+                rho[q][c][e] = A[q][c][e] + B[c][e] * X[q][c];
+    // P_M = rho_M + bq_M P_{M-1}, Q_M = bq_M Q_{M-1}: prefix scan
+    double Ap[NC][2][E], Bp[2][E];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        double accB = 1.0, accA[NC];
+#pragma unroll
+        for (int q = 0; q < NC; q++)
+            accA[q] = 0.0;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+#pragma unroll
+            for (int q = 0; q < NC; q++) {
+                accA[q] = rho[q][c][e] + bq[c][e] * accA[q];
+                Ap[q][c][e] = accA[q];
+            }
+            accB = bq[c][e] * accB;
+            Bp[c][e] = accB;
+        }
+    }
+    double Xp[NC][2], Yp[2];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        Yp[c] = shu(Bp[c][E - 1], 1);
+#pragma unroll
+        for (int q = 0; q < NC; q++)
+            Xp[q][c] = shu(Ap[q][c][E - 1], 1);
+        if (L == 0) {
+            Yp[c] = 1.0;
+#pragma unroll
+            for (int q = 0; q < NC; q++)
+                Xp[q][c] = 0.0;
+        }
+    }
+#pragma unroll
+    for (int s = 1; s < WV; s <<= 1) {
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const double Yo = shu(Yp[c], s);
+            double Xo[NC];
+#pragma unroll
+            for (int q = 0; q < NC; q++)
+                Xo[q] = shu(Xp[q][c], s);
+            if (L >= s) {
+#pragma unroll
+                for (int q = 0; q < NC; q++)
+                    Xp[q][c] = Xp[q][c] + Yp[c] * Xo[q];
+                Yp[c] = Yp[c] * Yo;
+            }
+        }
+    }
+    // P_M = A'_M + B'_M X', Q_M = B'_M Y' (B' the lane-local products), then
+    // the boundary row of each class: sums [q*2+c] of P and [2*NC+c] of Q
+    double sums[(NC + 1) * 2];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+#pragma unroll
+        for (int q = 0; q < NC; q++) {
+            double sp = 0.0;
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const double P = Ap[q][c][e] + Bp[c][e] * Xp[q][c];
+                Ap[q][c][e] = P;
+                sp += P;
+            }
+            sums[q * 2 + c] = sp;
+        }
+        double sq = 0.0;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const double Q = Bp[c][e] * Yp[c];
+            sq += Q;
+            Bp[c][e] = Q;
+        }
+        sums[2 * NC + c] = sq;
+    }
+    wave_sum<(NC + 1) * 2>(sums);
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int q = 0; q < NC; q++) {
+            const double a0 = (bc[q][c] - sums[q * 2 + c]) / sums[2 * NC + c];
+#pragma unroll
+            for (int e = 0; e < E; e++)
+                u[q][c][e] = Ap[q][c][e] + Bp[c][e] * a0;
+        }
+}
+
+template <int E>
+__global__ void __launch_bounds__(256) k_tau_w(ColArgs a, LamFactor lf, LamFactor kf, double an,
+                                               double bn, double anp1, double bnp1)
+{
+    const int L = threadIdx.x & (WV - 1);
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int ncol = a.nz * a.nxh;
+    if (t >= ncol)
+        return;
+    const int z = t / a.nxh, x = t - z * a.nxh;
+    if (z == a.nz / 2)                        // linearbl.f:227, z oddball
+        return;
+    const int n = a.ny;
+    const double re = a.re;
+    const double alfa = a.alfa[x], beta = a.beta[z];
+    const double k2 = alfa * alfa + beta * beta;
+    const double lam2 = k2 + 2. * re / (an + bn);
+    const bool k0 = (t == 0);
+    const double k2i = k0 ? 0. : 1. / k2;
+    const double c1 = 2. * re / (an + bn) * an;
+    const double c1n = 2. * re / (anp1 + bnp1);
+    const double c2 = 2. * re * bnp1 / (anp1 + bnp1);
+    const size_t fs = (size_t)ncol * n;
+    const double2 *Cc = a.C + (size_t)t * n;
+    const double2 *Pc = a.Pp + (size_t)t * n;
+    const double2 *glp = lf.gp + (size_t)t * n;
+    const double2 *gkp = kf.gp + (size_t)t * n;
+    const double *vhp = lf.vh + (size_t)t * n;
+
+    double H1[2][2][E], H2[2][2][E], H3[2][2][E], pv[2][2][E], pw[2][2][E], vh[2][E];
+    double2 gl[2][E], gk[2][E];
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int k = 2 * (L * E + e) + c;
+            const bool ok = k < n;
+            const double2 z2 = make_double2(0., 0.);
+            const double2 h1 = ok ? Cc[k] : z2, h2 = ok ? Cc[fs + k] : z2;
+            const double2 h3 = ok ? Cc[2 * fs + k] : z2;
+            const double2 p0 = ok ? Pc[k] : z2, p1 = ok ? Pc[fs + k] : z2;
+            H1[0][c][e] = h1.x; H1[1][c][e] = h1.y;
+            H2[0][c][e] = h2.x; H2[1][c][e] = h2.y;
+            H3[0][c][e] = h3.x; H3[1][c][e] = h3.y;
+            pv[0][c][e] = p0.x; pv[1][c][e] = p0.y;
+            pw[0][c][e] = p1.x; pw[1][c][e] = p1.y;
+            gl[c][e] = ok ? glp[k] : z2;
+            gk[c][e] = ok ? gkp[k] : z2;
+            vh[c][e] = ok ? vhp[k] : 0.;
+        }
+    // h_omega, f_omega, g = i alpha H1 + i beta H3 (linearbl.f:325-343)
+    double hw[2][2][E], fw[2][2][E], g[2][2][E], dg[2][2][E];
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            hw[0][c][e] = -beta * H1[1][c][e] + alfa * H3[1][c][e];
+            hw[1][c][e] = beta * H1[0][c][e] - alfa * H3[0][c][e];
+            fw[0][c][e] = pw[0][c][e] - c1 * hw[0][c][e];
+            fw[1][c][e] = pw[1][c][e] - c1 * hw[1][c][e];
+            g[0][c][e] = -alfa * H1[1][c][e] - beta * H3[1][c][e];
+            g[1][c][e] = alfa * H1[0][c][e] + beta * H3[0][c][e];
+        }
+    dcheb_w<2, E>(g, dg, L);
+    // h_v = -(dg/dy + k^2 H2), f_v = p_v - c1 h_v (linearbl.f:345-361)
+    double hv[2][2][E], fv[2][2][E];
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                hv[q][c][e] = -(dg[q][c][e] + k2 * H2[q][c][e]);
+                fv[q][c][e] = pv[q][c][e] - c1 * hv[q][c][e];
+                if (k0) {
+                    fv[q][c][e] = 0.;
+                    fw[q][c][e] = 0.;
+                }
+            }
+    const double bc0[2][2] = {{0., 0.}, {0., 0.}};
+    // Helmholtz solves with lam^2 (linearbl.f:625-626) and v with k^2
+    // (linearbl.f:1000-1001); the homogeneous pair phi_h, v_h is precomputed
+    double ph[2][2][E], om[2][2][E], v[2][2][E];
+    solve_w<2, E>(n, lam2, gl, fv, bc0, ph, L);
+    solve_w<2, E>(n, lam2, gl, fw, bc0, om, L);
+    solve_w<2, E>(n, k2, gk, ph, bc0, v, L);
+    // superpose v_h per parity so that dv/dy = 0 at the walls
+    // (linearbl.f:1023-1070): class-c wall slope sum_k k^2 v_k
+    {
+        double s[4] = {0., 0., 0., 0.};
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const double kk = (double)(2 * (L * E + e) + c);
+                s[0 + c] += kk * kk * v[0][c][e];
+                s[2 + c] += kk * kk * v[1][c][e];
+            }
+        wave_sum<4>(s);
+        const double2 sh = lf.sh[t];
+        const double shc[2] = {sh.x, sh.y};
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const double cr = (0. - s[c]) * (1. / shc[c]);
+            const double ci = (0. - s[2 + c]) * (1. / shc[c]);
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                v[0][c][e] = v[0][c][e] + cr * vh[c][e];
+                v[1][c][e] = v[1][c][e] + ci * vh[c][e];
+            }
+        }
+    }
+    // dv, phi = D^2 v - k^2 v, d omega (linearbl.f:1151-1166)
+    double dv[2][2][E], d2v[2][2][E], dom[2][2][E];
+    dcheb_w<2, E>(v, dv, L);
+    dcheb_w<2, E>(dv, d2v, L);
+    dcheb_w<2, E>(om, dom, L);
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int e = 0; e < E; e++)
+                ph[q][c][e] = d2v[q][c][e] - k2 * v[q][c][e];
+    // next partial right-hand sides (linearbl.f:1761-1784, tau branch)
+    double d1[2][2][E], d2[2][2][E];
+    dcheb_w<2, E>(ph, d1, L);
+    dcheb_w<2, E>(d1, d2, L);
+    const double k2m = k2 - c1n;
+    double2 *Cw = a.C + (size_t)t * n;
+    double2 *Pw = a.Pp + (size_t)t * n;
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int k = 2 * (L * E + e) + c;
+            if (k < n)
+                Pw[k] = make_double2(-d2[0][c][e] + k2m * ph[0][c][e] - c2 * hv[0][c][e],
+                                     -d2[1][c][e] + k2m * ph[1][c][e] - c2 * hv[1][c][e]);
+        }
+    dcheb_w<2, E>(dom, d1, L);
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int k = 2 * (L * E + e) + c;
+            if (k < n)
+                Pw[fs + k] = make_double2(-d1[0][c][e] + k2m * om[0][c][e] - c2 * hw[0][c][e],
+                                          -d1[1][c][e] + k2m * om[1][c][e] - c2 * hw[1][c][e]);
+        }
+    if (!k0) {
+        // u, w, om_x, om_z (linearbl.f:1190-1212) in Chebyshev space
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int k = 2 * (L * E + e) + c;
+                if (k >= n)
+                    continue;
+                const double dvr = dv[0][c][e], dvi = dv[1][c][e];
+                const double omr = om[0][c][e], omi = om[1][c][e];
+                const double dor = dom[0][c][e], doi = dom[1][c][e];
+                const double phr = ph[0][c][e], phi = ph[1][c][e];
+                Cw[k] = make_double2((-alfa * dvi + beta * omi) * k2i, (alfa * dvr - beta * omr) * k2i);
+                Cw[fs + k] = make_double2(v[0][c][e], v[1][c][e]);
+                Cw[2 * fs + k] = make_double2((-alfa * omi - beta * dvi) * k2i,
+                                              (alfa * omr + beta * dvr) * k2i);
+                Cw[3 * fs + k] = make_double2((-alfa * doi - beta * phi) * k2i,
+                                              (alfa * dor + beta * phr) * k2i);
+                Cw[4 * fs + k] = make_double2((-beta * doi + alfa * phi) * k2i,
+                                              (beta * dor - alfa * phr) * k2i);
+            }
+        return;
+    }
+    // k = 0: mean flow and constant-flux pressure gradient
+    // (linearbl.f:303-319, 1216-1417, 1734-1746; tau branch, ibc = 0)
+    double fuw[2][2][E], uw[2][2][E];
+    double px = *a.px;
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int k = 2 * (L * E + e) + c;
+            const bool ok = k < n;
+            fuw[0][c][e] = ok ? a.puw[k] - c1 * H1[0][c][e] : 0.;
+            fuw[1][c][e] = ok ? a.puw[n + k] - c1 * H3[0][c][e] : 0.;
+        }
+    if (L == 0)
+        fuw[0][0][0] += 2. * re * px;
+    const double bcu[2][2] = {{.5 * (a.u0upp + a.u0low), .5 * (a.u0upp - a.u0low)},
+                              {.5 * (a.w0upp + a.w0low), .5 * (a.w0upp - a.w0low)}};
+    solve_w<2, E>(n, lam2, gl, fuw, bcu, uw, L);
+    if (a.cflux) {
+        auto flux = [&](const double (&u)[2][2][E]) {
+            // integral over [-1,1] of sum_k u_k T_k (linearbl.f:1367-1370)
+            double m[1] = {0.};
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int k = 2 * (L * E + e);
+                if (k < n)
+                    m[0] -= u[0][0][e] * 2. / (double)(k * k - 1);
+            }
+            wave_sum<1>(m);
+            return m[0];
+        };
+        const double m1 = flux(uw);
+        if (L == 0)
+            fuw[0][0][0] = fuw[0][0][0] + 2. * re * px;
+        solve_w<2, E>(n, lam2, gl, fuw, bcu, uw, L);
+        const double m2 = flux(uw);
+        if (L == 0)
+            fuw[0][0][0] = fuw[0][0][0] - 4. * re * px;
+        px = px * (1. + (a.mflux - m1) / (m2 - m1));
+        if (L == 0)
+            fuw[0][0][0] = fuw[0][0][0] + 2. * re * px;
+        solve_w<2, E>(n, lam2, gl, fuw, bcu, uw, L);
+        if (L == 0)
+            *a.px = px;
+    }
+    double duw[2][2][E], d2uw[2][2][E];
+    dcheb_w<2, E>(uw, duw, L);
+    dcheb_w<2, E>(duw, d2uw, L);
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int k = 2 * (L * E + e) + c;
+            if (k >= n)
+                continue;
+            a.puw[k] = -d2uw[0][c][e] - c1n * uw[0][c][e] - c2 * H1[0][c][e];
+            a.puw[n + k] = -d2uw[1][c][e] - c1n * uw[1][c][e] - c2 * H3[0][c][e];
+            Cw[k] = make_double2(uw[0][c][e], 0.);
+            Cw[fs + k] = make_double2(0., 0.);
+            Cw[2 * fs + k] = make_double2(uw[1][c][e], 0.);
+            Cw[3 * fs + k] = make_double2(duw[1][c][e], 0.);
+            Cw[4 * fs + k] = make_double2(-duw[0][c][e], 0.);
+        }
+}
+
+}  // namespace
+
+int launch_tau(const ColArgs &a, LamFactor lf, LamFactor kf, double an, double bn, double anp1,
+               double bnp1, hipStream_t s)
+{
+    const int ncol = a.nz * a.nxh;
+    dim3 grid((ncol + 3) / 4);
+    const int E = (a.ny + 127) / 128;
+    switch (E) {
+    case 1:
+        hipLaunchKernelGGL(k_tau_w<1>, grid, dim3(256), 0, s, a, lf, kf, an, bn, anp1, bnp1);
+        return 0;
+    case 2:
+        hipLaunchKernelGGL(k_tau_w<2>, grid, dim3(256), 0, s, a, lf, kf, an, bn, anp1, bnp1);
+        return 0;
+    case 3:
+        hipLaunchKernelGGL(k_tau_w<3>, grid, dim3(256), 0, s, a, lf, kf, an, bn, anp1, bnp1);
+        return 0;
+    default:
+        return SIMSON_E_ARG;
+    }
+}
+
+}  // namespace sg
