@@ -59,6 +59,94 @@ __device__ __forceinline__ void dft_small(double2 *v, int R, double sign)
     }
 }
 
+// ---- compile-time radix schedule: radix-4 passes, then 2, 3, 5 ----
+constexpr int rad_at(int n, int p)
+{
+    const int order[4] = {4, 2, 3, 5};
+    for (int i = 0; i < 4; i++) {
+        const int r = order[i];
+        while (n % r == 0) {
+            if (p == 0)
+                return r;
+            --p;
+            n /= r;
+        }
+    }
+    return 0;
+}
+
+constexpr int nrad_of(int n)
+{
+    int c = 0;
+    while (rad_at(n, c) != 0)
+        c++;
+    return c;
+}
+
+constexpr bool fft_len_ok(int n)
+{
+    int m = n;
+    for (int r : {4, 2, 3, 5})
+        while (m % r == 0)
+            m /= r;
+    return m == 1;
+}
+
+template <int R, int SIGN>
+__device__ __forceinline__ void dft_small_t(double2 *v)
+{
+    dft_small(v, R, (double)SIGN);
+}
+
+// One Stockham pass of a length-N transform over nb transforms whose start
+// points are SB apart; twiddles exp(+2 pi i t/TABN) from tw (LDS).
+template <int N, int TABN, int SB, int P, int NS, int SIGN>
+__device__ __forceinline__ void fft_pass_t(const double2 *src, double2 *dst, int nb,
+                                           const double2 *tw)
+{
+    constexpr int R = rad_at(N, P);
+    constexpr int NBF = N / R;
+    constexpr int TSTEP = TABN / (NS * R);
+    for (int idx = threadIdx.x; idx < nb * NBF; idx += blockDim.x) {
+        const int b = idx / NBF;
+        const int j = idx - b * NBF;
+        const int k = j % NS;
+        const double2 *s = src + b * SB;
+        double2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            v[r] = s[j + r * NBF];
+        if constexpr (NS > 1) {
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                double2 w = tw[r * k * TSTEP];
+                if (SIGN < 0)
+                    w.y = -w.y;
+                v[r] = cmul(v[r], w);
+            }
+        }
+        dft_small_t<R, SIGN>(v);
+        double2 *d = dst + b * SB;
+        const int base = (j / NS) * NS * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            d[base + r * NS] = v[r];
+    }
+    __syncthreads();
+}
+
+// Full length-N transform (all passes); returns the buffer with the result.
+template <int N, int TABN, int SB, int SIGN, int P = 0, int NS = 1>
+__device__ __forceinline__ double2 *fft_t(double2 *src, double2 *dst, int nb, const double2 *tw)
+{
+    if constexpr (P == nrad_of(N)) {
+        return src;
+    } else {
+        fft_pass_t<N, TABN, SB, P, NS, SIGN>(src, dst, nb, tw);
+        return fft_t<N, TABN, SB, SIGN, P + 1, NS * rad_at(N, P)>(dst, src, nb, tw);
+    }
+}
+
 // nb transforms of length n stored contiguously ([b][n]) in src.  Ping-pongs
 // between src and dst; returns the buffer holding the result.  All threads of
 // the block must call it; it ends with a barrier.

@@ -55,6 +55,8 @@ CASES = [
     # nx, ny, nz, re, xl, zl, kmax, rms, cflux, retau
     (32, 33, 32, 3000.0, 4 * math.pi, 2 * math.pi, 8, 1e-3, True, 0.0),
     (16, 17, 8, 2000.0, 2 * math.pi, math.pi, 3, 1e-2, False, 100.0),
+    # FFT lengths 30 (radix 5) take the generic, runtime-sized kernels
+    (40, 25, 20, 3000.0, 2 * math.pi, math.pi, 4, 1e-2, True, 0.0),
     (64, 65, 48, 4200.0, 4 * math.pi, 4 * math.pi / 3, 10, 5e-2, True, 0.0),
     (128, 97, 128, 4200.0, 4 * math.pi, 4 * math.pi / 3, 16, 1e-1, True, 0.0),
 ]
