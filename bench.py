@@ -70,6 +70,29 @@ def alg_per_launch():
     }
 
 
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "hbm_traffic_c2_128x97x128_fp64.md")
+PROFILE_KEY = {"zsyn": "k_zsyn_t<192>", "xrow": "k_xrow_t<96>", "zana": "k_zana_t<192>",
+               "tau": "k_tau_w<1>", "cheb_fwd": "k_gemm fwd", "cheb_bwd": "k_gemm bwd"}
+
+
+def profiled_traffic(family: str):
+    """HBM bytes per launch of a kernel family from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"Machine-readable \(bytes per launch\): (.*)", text)
+    key = PROFILE_KEY.get(family)
+    if not m or not key:
+        return None
+    for item in m.group(1).split(", "):
+        k, _, v = item.rpartition("=")
+        if k == key:
+            return int(v)
+    return None
+
+
 def cpu_baseline(nsteps: int = 40, threads: int = 16):
     """Time the reference bla on the host (bounded sample of full steps)."""
     if not os.path.exists(REF_BIN):
@@ -108,6 +131,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the run, copy a device buffer of this size three times "
+                         "(a plain copy of known bytes that scales rocprofv3 --pmc "
+                         "FETCH_SIZE/WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -134,6 +161,14 @@ def main():
     kern = g.timing_read()
     g.timing(False)
     g.close()
+    if args.calib_copy_mib > 0:
+        import ctypes
+        from simson_amd import build as _b
+        cal = ctypes.CDLL(_b.CALIB_LIB)
+        cal.simson_calib_copy.restype = ctypes.c_int
+        cal.simson_calib_copy.argtypes = [ctypes.c_size_t, ctypes.c_int]
+        if cal.simson_calib_copy(args.calib_copy_mib << 20, 3) != 0:
+            print("calibration copy failed", file=sys.stderr)
 
     elapsed = t1 - t0
     steps_per_s = args.steps / elapsed
@@ -158,11 +193,12 @@ def main():
     d = kernels[dom]
     if d["bound"] == "hbm":
         roofline = {"kernel": dom, "bound": "hbm", "achieved": d["achieved_GBs"],
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["frac"], "traffic": None}
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["frac"],
+                    "traffic": profiled_traffic(dom)}
     else:
         roofline = {"kernel": dom, "bound": "mfma", "achieved": d["achieved_TFLOPs"],
                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": d["frac"],
-                    "traffic": None}
+                    "traffic": profiled_traffic(dom)}
     out = {
         "metric": METRIC,
         "value": round(steps_per_s, 3),

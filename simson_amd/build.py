@@ -4,7 +4,12 @@
 
 Every source under simson_amd/csrc is compiled for gfx950 with hipcc and
 linked into one shared library exporting the C ABI of include/simson_gpu.h.
-Packed-FP32 instructions are disabled (-packed-fp32-ops), see DESIGN.md.
+Packed-FP32 instructions are disabled (-packed-fp32-ops): some of their
+forms have given wrong results on the target machines without an error.
+
+The Fortran host (simson_amd/fortran: the ISO_C_BINDING module and the
+blagpu driver that reads bla.i and .u files) is compiled with amdflang into
+simson_amd/_lib/blagpu, linked to the step library by rpath.
 """
 from __future__ import annotations
 
@@ -48,21 +53,52 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = True) -> str:
-    os.makedirs(OUTDIR, exist_ok=True)
-    srcs = sources()
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
-    tmp = LIB + ".tmp"
+CALIB_LIB = os.path.join(OUTDIR, "libsimson_calib.so")
+
+
+def _link(objs, out):
+    tmp = out + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
     for o in objs:
         os.remove(o)
+
+
+FORTRAN = os.path.join(HERE, "fortran")
+BLAGPU = os.path.join(OUTDIR, "blagpu")
+FC = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
+
+
+def build_fortran() -> str:
+    """Compile the Fortran host against the step library (rpath $ORIGIN)."""
+    mod = os.path.join(FORTRAN, "simson_gpu_mod.f90")
+    prog = os.path.join(FORTRAN, "blagpu.f90")
+    tmp = BLAGPU + ".tmp"
+    cmd = [FC, "-O2", "-J", OUTDIR, mod, prog, "-o", tmp, "-L", OUTDIR, "-lsimson_gpu",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, BLAGPU)
+    return BLAGPU
+
+
+def build(verbose: bool = True) -> str:
+    """Compile the step library, the copy-calibration library and the
+    Fortran host."""
+    os.makedirs(OUTDIR, exist_ok=True)
+    srcs = sources()
+    calib = [os.path.join(CSRC, "calib", "sg_copy.hip")]
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs) + 1)) as ex:
+        objs = list(ex.map(_compile, srcs + calib))
+    _link(objs[: len(srcs)], LIB)
+    _link(objs[len(srcs):], CALIB_LIB)
+    build_fortran()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB}, {CALIB_LIB} and {BLAGPU}")
     return LIB
 
 

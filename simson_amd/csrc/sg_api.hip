@@ -321,6 +321,11 @@ int simson_gpu_init(const simson_params_t *p, simson_gpu_t **out)
     const int nxp = p->nx + p->nx / 2, nzp = p->nz + p->nz / 2;
     if (!valid_fft_len(nzp) || !valid_fft_len(nxp / 2))
         return SIMSON_E_ARG;
+    // limits of the kernels: one tau column per wavefront holds at most
+    // 3 x 128 Chebyshev coefficients; the generic FFT kernels keep two z
+    // lines (and twelve half x rows) in 64 KiB of LDS
+    if (p->ny > 383 || 2 * nzp * 16 > 65536 || 12 * (nxp / 2) * 16 > 65536)
+        return SIMSON_E_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return SIMSON_E_NODEV;

@@ -1,0 +1,254 @@
+! blagpu: a Fortran host for the MI355X step library, taking the inputs of
+! Simson's bla -- bla.i (version 20070716) and a .u velocity file -- and
+! writing the end field in the same .u format.
+!
+! It keeps bla's driver logic on the host: the bla.i grammar of
+! bla/rparambl.f:160-1860 (temporal channel subset), the .u reader/writer of
+! bla/rdiscbl.f:93-411 and bla/wdiscbl.f:50-136, the RK weight bookkeeping of
+! bla/bla.f:1465-1477 and 2384-2397, the CFL stop of bla/wcflbl.f:87-98 and
+! the stop criteria of bla/bla.f:2779-2833 -- and calls the library for
+! prhs (bla.f:1562), the nonlinbl plane loop (bla.f:2251-2288) and the
+! linearbl plane loop (bla.f:2433-2475) through ISO_C_BINDING.
+program blagpu
+  use iso_c_binding
+  use simson_gpu_mod
+  implicit none
+
+  integer, parameter :: dp = kind(1.d0)
+  real(dp), parameter :: pi = 3.14159265358979323846_dp
+  character(len=256) :: line, namnin, namnut
+  character(len=256) :: nmsave(100)
+  real(dp) :: tsave(101)
+  integer :: version, maxit, nst, ibc, icfl, iamp, iext, ixys, msave
+  integer :: mwave, npl, isfd, imhd, loctyp, wbci, i, it, isave, st
+  real(dp) :: tmax, cpumax, wallmax, dt, cflmaxin, xl, rot, retau, cdev
+  logical :: write_inter, varsiz, cflux, pert, cim, gall, suction, spat
+  logical :: sgs, tripf, longli, fileurms, pou, stopf
+  ! field and header
+  integer :: nx, nyp, nzc, nfzsym, fltype
+  real(dp) :: re, xlf, zl, t, xs, dstar
+  real(dp), allocatable :: ur(:,:,:,:), ui(:,:,:,:), urx(:)
+  ! time stepping (rkcoeff.f:21-32)
+  real(dp) :: anrk(3), bnrk(3), dtn, dtnp1, an, bn, anp1, bnp1, cflp, cfl, px
+  type(simson_params_t) :: p
+  type(c_ptr) :: g
+  integer(8) :: c0, c1, crate
+
+  anrk = [8._dp/15._dp, 5._dp/12._dp, 3._dp/4._dp]
+  bnrk = [0._dp, -17._dp/60._dp, -5._dp/12._dp]
+
+  ! ---------------- bla.i (rparambl.f) ----------------
+  open(10, file='bla.i', status='old')
+  call nxt(line); read(line, *) version
+  if (version /= 20070716) call stopnow(345431)
+  call nxt(line); namnin = word(line)
+  call nxt(line); namnut = word(line)
+  call nxt(line); read(line, *) tmax;     tmax = max(tmax, 0._dp)
+  call nxt(line); read(line, *) maxit;    maxit = max(maxit, 0)
+  call nxt(line); read(line, *) cpumax
+  call nxt(line); read(line, *) wallmax
+  call nxt(line); read(line, *) write_inter
+  call nxt(line); read(line, *) dt
+  call nxt(line); read(line, *) nst
+  call nxt(line); read(line, *) cflmaxin
+  call nxt(line); read(line, *) xl
+  call nxt(line); read(line, *) varsiz
+  call nxt(line); read(line, *) rot
+  call nxt(line); read(line, *) cflux
+  retau = 0._dp
+  if (.not. cflux) then
+     call nxt(line); read(line, *) retau
+  end if
+  call nxt(line); read(line, *) pert
+  call nxt(line); read(line, *) ibc
+  call nxt(line); read(line, *) cim
+  call nxt(line); read(line, *) gall
+  call nxt(line); read(line, *) suction
+  call nxt(line); read(line, *) spat
+  call nxt(line); read(line, *) cdev
+  call nxt(line); read(line, *) sgs
+  call nxt(line); read(line, *) isfd
+  call nxt(line); read(line, *) imhd
+  call nxt(line); read(line, *) loctyp
+  call nxt(line); read(line, *) tripf
+  call nxt(line); read(line, *) wbci
+  call nxt(line); read(line, *) icfl
+  call nxt(line); read(line, *) iamp
+  if (iamp > 0) then
+     call nxt(line)
+     call nxt(line); read(line, *) fileurms
+  end if
+  call nxt(line); read(line, *) longli
+  call nxt(line); read(line, *) iext
+  if (iext > 0) call nxt(line)
+  call nxt(line); read(line, *) ixys
+  call nxt(line); read(line, *) msave
+  if (msave < 0 .or. msave > 99) call stopnow(13)
+  do i = 1, msave
+     call nxt(line); read(line, *) tsave(i)
+     call nxt(line); nmsave(i) = word(line)
+  end do
+  tsave(msave + 1) = 1.e30_dp
+  call nxt(line); read(line, *) mwave
+  call nxt(line); read(line, *) npl
+  close(10)
+  if (pert .or. cim .or. suction .or. spat .or. sgs .or. isfd /= 0 .or. imhd /= 0 .or. &
+       loctyp /= 0 .or. tripf .or. wbci /= 0 .or. ixys > 0 .or. ibc /= 0 .or. nst /= 3 &
+       .or. mwave > 0 .or. npl > 0 &
+       .or. dt <= 0._dp) then
+     write(*,*) 'blagpu: bla.i requests a feature outside the channel step library'
+     call stopnow(13)
+  end if
+  ! rparambl.f:1860-1866, 2029-2031
+  if (icfl > 0 .and. icfl < nst) icfl = nst
+  if (icfl > 0) icfl = min(icfl / nst * nst, 3 * nst)
+  maxit = maxit / nst * nst
+
+  ! ---------------- .u file (rdiscbl.f) ----------------
+  open(12, file=trim(namnin), status='old', form='unformatted')
+  read(12) re, pou, xlf, zl, t, xs
+  read(12) nx, nyp, nzc, nfzsym
+  read(12) fltype, dstar
+  if (fltype /= 1 .or. nfzsym /= 0) call stopnow(13)
+  allocate(ur(nx/2, nyp, nzc, 7), ui(nx/2, nyp, nzc, 7), urx(nx))
+  ur = 0._dp
+  ui = 0._dp
+  do i = 1, 3
+     do st = 1, nzc
+        do it = 1, nyp
+           read(12) urx
+           ur(:, it, st, i) = urx(1:nx:2)
+           ui(:, it, st, i) = urx(2:nx:2)
+        end do
+     end do
+  end do
+  close(12)
+
+  ! ---------------- device state ----------------
+  p%nx = nx; p%ny = nyp; p%nz = nzc
+  p%precision = SIMSON_FP64; p%ibc = 0
+  p%cflux = merge(1, 0, cflux)
+  p%re = re; p%xl = xl; p%zl = zl; p%rot = rot; p%retau = retau
+  call chk(simson_gpu_init(p, g))
+  if (cflux) then
+     px = -2._dp / re                             ! bla.f:1629-1630
+  else
+     px = -(retau / re)**2                        ! bla.f:1645
+  end if
+  call chk(simson_gpu_upload(g, ur, ui, 3_c_int, c_null_ptr, px))
+
+  ! ---------------- time stepping (bla.f:1465-1477, 1790-2885) ----------------
+  dtn = dt
+  an = dtn * anrk(1)
+  dtnp1 = dtn
+  anp1 = an
+  bnp1 = 0._dp
+  call chk(simson_gpu_prhs(g, anp1))
+  isave = 1
+  it = 1
+  cfl = 0._dp
+  call system_clock(c0, crate)
+  do
+     if (icfl > 0 .and. mod(it - 1, icfl) == 0) then
+        call chk(simson_gpu_nonlin(g, 1_c_int, cflp))
+        cfl = (cflp * pi + 2._dp * abs(rot)) * dtn   ! wcflbl.f:87
+        write(*, '(a,f22.16,a,f22.16)') 'CFL*dt ', cfl, ' dt ', dtn / dstar
+        if (cfl > 10._dp) then
+           write(*,*) 'CFL*dt too high. Stop'
+           call stopnow(324234)
+        end if
+     else
+        call chk(simson_gpu_nonlin(g, 0_c_int, cflp))
+     end if
+     an = anp1
+     bn = bnp1
+     anp1 = dtnp1 * anrk(mod(it, nst) + 1)
+     bnp1 = dtnp1 * bnrk(mod(it, nst) + 1)
+     call chk(simson_gpu_linear(g, an, bn, anp1, bnp1, c_null_ptr))
+     it = it + 1
+     if (mod(it - 1, nst) == 0) then
+        t = t + dtn
+        if (t + .5_dp * dtn > tsave(isave)) then
+           call save_field(trim(nmsave(isave)))
+           isave = isave + 1
+        end if
+        inquire(file='stop.now', exist=stopf)
+        if (stopf) write(*,*) 'STOP DUE TO stop.now FILE'
+        if (maxit > 0 .and. it - 1 >= maxit) then
+           write(*,*) 'STOP DUE TO ITERATIONS'
+           stopf = .true.
+        end if
+        if (tmax > 0._dp .and. t + .5_dp * dtn > tmax) then
+           write(*,*) 'STOP DUE TO INTEGRATION TIME'
+           stopf = .true.
+        end if
+        if (stopf) exit
+     end if
+  end do
+  call system_clock(c1)
+  call save_field(trim(namnut))
+  write(*,*) 'Average time step    : ', real(c1 - c0, dp) / real(crate, dp) / &
+       real((it - 1) / nst, dp)
+  call simson_gpu_finalize(g)
+  write(*,*) 'blagpu terminated normally.'
+
+contains
+
+  subroutine chk(code)
+    integer(c_int), intent(in) :: code
+    if (code /= 0) then
+       write(*,*) 'simson_gpu call failed with status ', code
+       call stopnow(int(code))
+    end if
+  end subroutine chk
+
+  ! stopnow.f:23-32: print the code and stop
+  subroutine stopnow(code)
+    integer, intent(in) :: code
+    write(*,*) 'stopnow: ', code
+    stop 1
+  end subroutine stopnow
+
+  ! next line that is not a comment (comment.f)
+  subroutine nxt(l)
+    character(len=*), intent(out) :: l
+    do
+       read(10, '(a)') l
+       if (len_trim(l) == 0) cycle
+       if (l(1:1) /= '#') exit
+    end do
+  end subroutine nxt
+
+  function word(l) result(w)
+    character(len=*), intent(in) :: l
+    character(len=256) :: w
+    integer :: k
+    w = adjustl(l)
+    k = index(trim(w), ' ')
+    if (k > 0) w = w(1:k-1)
+  end function word
+
+  ! wdiscbl.f:50-136 (channel: no shift, no extra header records)
+  subroutine save_field(name)
+    character(len=*), intent(in) :: name
+    integer :: c, z, y
+    real(dp) :: pxo
+    call chk(simson_gpu_download(g, ur, ui, 3_c_int, c_null_ptr, pxo))
+    open(11, file=name, form='unformatted', status='replace')
+    write(11) re, .false., xl, zl, t, 0._dp
+    write(11) nx, nyp, nzc, nfzsym
+    write(11) fltype, dstar
+    do c = 1, 3
+       do z = 1, nzc
+          do y = 1, nyp
+             urx(1:nx:2) = ur(:, y, z, c)
+             urx(2:nx:2) = ui(:, y, z, c)
+             write(11) urx
+          end do
+       end do
+    end do
+    close(11)
+    write(*,*) '** Write velocity field ** ', trim(name), t / dstar
+  end subroutine save_field
+
+end program blagpu

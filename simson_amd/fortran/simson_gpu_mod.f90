@@ -1,0 +1,75 @@
+! ISO_C_BINDING interface to the MI355X Simson step library
+! (include/simson_gpu.h).  Arrays are passed in the reference main-storage
+! layout ur(memnx,memny,memnz,memnxyz) of bla/bla.f:67.
+module simson_gpu_mod
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: SIMSON_FP64 = 0
+
+  type, bind(c) :: simson_params_t
+     integer(c_int) :: nx, ny, nz
+     integer(c_int) :: precision
+     integer(c_int) :: ibc
+     integer(c_int) :: cflux
+     real(c_double) :: re
+     real(c_double) :: xl, zl
+     real(c_double) :: rot
+     real(c_double) :: retau
+  end type simson_params_t
+
+  interface
+     integer(c_int) function simson_gpu_init(p, g) bind(c, name='simson_gpu_init')
+       import :: c_int, c_ptr, simson_params_t
+       type(simson_params_t), intent(in) :: p
+       type(c_ptr), intent(out) :: g
+     end function simson_gpu_init
+
+     integer(c_int) function simson_gpu_upload(g, ur, ui, nslots, puw, px) &
+          bind(c, name='simson_gpu_upload')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: g
+       real(c_double), intent(in) :: ur(*), ui(*)
+       integer(c_int), value :: nslots
+       type(c_ptr), value :: puw
+       real(c_double), value :: px
+     end function simson_gpu_upload
+
+     integer(c_int) function simson_gpu_prhs(g, an) bind(c, name='simson_gpu_prhs')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: g
+       real(c_double), value :: an
+     end function simson_gpu_prhs
+
+     integer(c_int) function simson_gpu_nonlin(g, do_cfl, cfl_out) &
+          bind(c, name='simson_gpu_nonlin')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: g
+       integer(c_int), value :: do_cfl
+       real(c_double), intent(out) :: cfl_out
+     end function simson_gpu_nonlin
+
+     integer(c_int) function simson_gpu_linear(g, an, bn, anp1, bnp1, px_out) &
+          bind(c, name='simson_gpu_linear')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: g
+       real(c_double), value :: an, bn, anp1, bnp1
+       type(c_ptr), value :: px_out
+     end function simson_gpu_linear
+
+     integer(c_int) function simson_gpu_download(g, ur, ui, nslots, puw, px) &
+          bind(c, name='simson_gpu_download')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: g
+       real(c_double), intent(out) :: ur(*), ui(*)
+       integer(c_int), value :: nslots
+       type(c_ptr), value :: puw
+       real(c_double), intent(out) :: px
+     end function simson_gpu_download
+
+     subroutine simson_gpu_finalize(g) bind(c, name='simson_gpu_finalize')
+       import :: c_ptr
+       type(c_ptr), value :: g
+     end subroutine simson_gpu_finalize
+  end interface
+end module simson_gpu_mod

@@ -41,6 +41,9 @@ def test_init_rejects_unsupported_settings():
         gpu.SimsonParams(28, 33, 32, 0, 0, 1, 3000.0, 12.5, 6.28, 0.0, 0.0),   # nx%4
         gpu.SimsonParams(32, 33, 32, 7, 0, 1, 3000.0, 12.5, 6.28, 0.0, 0.0),   # precision
         gpu.SimsonParams(32, 33, 28, 0, 0, 1, 3000.0, 12.5, 6.28, 0.0, 0.0),   # nzp=42 (7)
+        gpu.SimsonParams(32, 385, 32, 0, 0, 1, 3000.0, 12.5, 6.28, 0.0, 0.0),  # ny > 383
+        gpu.SimsonParams(32, 33, 1440, 0, 0, 1, 3000.0, 12.5, 6.28, 0.0, 0.0),  # nzp=2160
+        gpu.SimsonParams(480, 33, 32, 0, 0, 1, 3000.0, 12.5, 6.28, 0.0, 0.0),  # nxp/2=360
     ]
     for p in bad:
         h = C.c_void_p()

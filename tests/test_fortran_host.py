@@ -1,0 +1,105 @@
+"""The Fortran host (simson_amd/_lib/blagpu) run like the reference bla.
+
+blagpu reads bla.i and the .u start file, drives the RK3 substeps through
+the ISO_C_BINDING interface and writes .u files.  On the GPU it is run on
+the golden 32x33x32 case and its t1.u / t10.u / end.u and CFL log lines are
+compared with what the reference bla wrote for the same inputs; on the CPU
+only its input checks run (they stop before any device call).
+"""
+import math
+import os
+import re
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from simson_amd import blai, fields, ufile
+
+BLAGPU = os.path.join(ROOT, "simson_amd", "_lib", "blagpu")
+
+
+def _need_binary():
+    if not os.path.exists(BLAGPU):
+        pytest.fail("blagpu is not built (python -m simson_amd.build)")
+
+
+def _c1_dir(**over):
+    work = tempfile.mkdtemp(prefix="blagpu.")
+    f = fields.channel_field(32, 33, 32, 4 * math.pi, 2 * math.pi, 3000.0,
+                             kmax=8, rms=1e-3, seed=1)
+    ufile.write_u(os.path.join(work, "init.u"), f)
+    kw = dict(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=300, dt=0.01, nst=3,
+              xl=4 * math.pi, cflux=True, icfl=3, tsave=[0.01, 0.1],
+              nmsave=["t1.u", "t10.u"])
+    kw.update(over)
+    with open(os.path.join(work, "bla.i"), "w") as fh:
+        fh.write(blai.format_channel(blai.BlaParams(**kw)))
+    return work
+
+
+def _run(work, timeout=120):
+    return subprocess.run([BLAGPU], cwd=work, capture_output=True, text=True,
+                          timeout=timeout)
+
+
+def test_rejects_wrong_version():
+    _need_binary()
+    work = _c1_dir()
+    try:
+        path = os.path.join(work, "bla.i")
+        text = open(path).read().replace("20070716", "20070715", 1)
+        open(path, "w").write(text)
+        r = _run(work)
+        assert r.returncode != 0
+        assert "345431" in r.stdout
+    finally:
+        shutil.rmtree(work)
+
+
+def test_rejects_features_outside_the_library():
+    _need_binary()
+    work = _c1_dir()
+    try:
+        path = os.path.join(work, "bla.i")
+        # switch on the SGS model line (the step library has no LES closure)
+        lines = open(path).read().splitlines()
+        k = next(i for i, l in enumerate(lines) if "(sgs)" in l)
+        lines[k] = ".true.  SGS model (sgs)"
+        open(path, "w").write("\n".join(lines) + "\n")
+        r = _run(work)
+        assert r.returncode != 0
+        assert "outside the channel step library" in r.stdout
+    finally:
+        shutil.rmtree(work)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.gpu
+def test_fortran_host_matches_reference_bla(golden_c1):
+    _need_binary()
+    work = _c1_dir()
+    try:
+        r = _run(work)
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert "blagpu terminated normally" in r.stdout
+        for name, key in (("t1.u", "u1"), ("t10.u", "u10"), ("end.u", "u100")):
+            uf = ufile.read_u(os.path.join(work, name))
+            assert (uf.nx, uf.nyp, uf.nz) == (32, 33, 32)
+            ref = golden_c1[key]
+            for c in range(3):
+                err = _rel(uf.u[c], ref[c])
+                assert err <= 1e-12, (name, c, err)
+        end = ufile.read_u(os.path.join(work, "end.u"))
+        assert abs(end.t - 1.0) < 1e-12
+        cfl = [float(m) for m in re.findall(r"CFL\*dt\s+([0-9.Ee+-]+)", r.stdout)]
+        assert len(cfl) == len(golden_c1["cfl"])
+        np.testing.assert_allclose(cfl, golden_c1["cfl"], rtol=1e-12)
+    finally:
+        shutil.rmtree(work)
