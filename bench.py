@@ -93,7 +93,7 @@ def profiled_traffic(family: str):
     return None
 
 
-def cpu_baseline(nsteps: int = 40, threads: int = 16):
+def cpu_baseline(nsteps: int = 300, threads: int = 16):
     """Time the reference bla on the host (bounded sample of full steps)."""
     if not os.path.exists(REF_BIN):
         return None
@@ -130,7 +130,7 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy a device buffer of this size three times "
                          "(a plain copy of known bytes that scales rocprofv3 --pmc "

@@ -13,17 +13,17 @@
 //     of affine maps, the forward substitution a_k = rho_k - gamma_k a_{k-2}
 //     a prefix scan of affine maps, and the boundary row a sum;
 //   * parity sums (dv/dy at the wall, the mass flux) are wave reductions.
+// The scans run on DPP lane moves within 16-lane rows and row broadcasts or
+// lane reads across rows (sg_wave.h), so no step waits on an LDS permute.
 #include "sg_internal.h"
 #include "sg_tau.h"
+#include "sg_wave.h"
 
 namespace sg {
 
 namespace {
 
-constexpr int WV = 64;
-
-__device__ __forceinline__ double shd(double x, int s) { return __shfl_down(x, s, WV); }
-__device__ __forceinline__ double shu(double x, int s) { return __shfl_up(x, s, WV); }
+constexpr int WV = wv::kWave;
 
 // inclusive suffix sums over the class index of NV channels
 template <int NV, int E>
@@ -36,20 +36,10 @@ __device__ __forceinline__ void suffix_sum(double (&x)[NV][E], int L)
             x[v][e] += x[v][e + 1];
     double ex[NV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) {
-        ex[v] = shd(x[v][0], 1);
-        if (L == WV - 1)
-            ex[v] = 0.0;
-    }
-#pragma unroll
-    for (int s = 1; s < WV; s <<= 1) {
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-            const double tv = shd(ex[v], s);
-            if (L + s < WV)
-                ex[v] += tv;
-        }
-    }
+    for (int v = 0; v < NV; v++)
+        ex[v] = wv::from_next(x[v][0], 0.0);
+    double one = 1.0;
+    wv::scan_suffix<NV, false>(ex, one, L);
 #pragma unroll
     for (int v = 0; v < NV; v++)
 #pragma unroll
@@ -57,18 +47,11 @@ __device__ __forceinline__ void suffix_sum(double (&x)[NV][E], int L)
             x[v][e] += ex[v];
 }
 
-// sum over the wave, identical on every lane (reduced, then lane 0 broadcast)
+// sum over the wave, identical on every lane
 template <int NV>
 __device__ __forceinline__ void wave_sum(double (&x)[NV])
 {
-#pragma unroll
-    for (int m = 1; m < WV; m <<= 1)
-#pragma unroll
-        for (int v = 0; v < NV; v++)
-            x[v] += __shfl_xor(x[v], m, WV);
-#pragma unroll
-    for (int v = 0; v < NV; v++)
-        x[v] = __shfl(x[v], 0, WV);
+    wv::sum_all<NV>(x);
 }
 
 // d/dy in Chebyshev space of NC channels (dcheb.f:24-36)
@@ -78,11 +61,8 @@ __device__ __forceinline__ void dcheb_w(const double (&a)[NC][2][E], double (&b)
     double s[2 * NC][E];
     double nxt[NC];
 #pragma unroll
-    for (int q = 0; q < NC; q++) {
-        nxt[q] = shd(a[q][0][0], 1);            // next lane's first even element
-        if (L == WV - 1)
-            nxt[q] = 0.0;
-    }
+    for (int q = 0; q < NC; q++)
+        nxt[q] = wv::from_next(a[q][0][0], 0.0);   // next lane's first even element
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const int M = L * E + e;
@@ -121,12 +101,8 @@ __device__ __forceinline__ void solve_w(int n, double lam, const double2 (&gp)[2
     for (int q = 0; q < NC; q++)
 #pragma unroll
         for (int c = 0; c < 2; c++) {
-            fprev[q][c] = shu(f[q][c][E - 1], 1);
-            fnext[q][c] = shd(f[q][c][0], 1);
-            if (L == 0)
-                fprev[q][c] = 0.0;
-            if (L == WV - 1)
-                fnext[q][c] = 0.0;
+            fprev[q][c] = wv::from_prev(f[q][c][E - 1], 0.0);
+            fnext[q][c] = wv::from_next(f[q][c][0], 0.0);
         }
 #pragma unroll
     for (int c = 0; c < 2; c++)
@@ -166,33 +142,21 @@ __device__ __forceinline__ void solve_w(int n, double lam, const double2 (&gp)[2
     double X[NC][2], Y[2];
 #pragma unroll
     for (int c = 0; c < 2; c++) {
-        Y[c] = shd(B[c][0], 1);
+        Y[c] = wv::from_next(B[c][0], 1.0);
 #pragma unroll
         for (int q = 0; q < NC; q++)
-            X[q][c] = shd(A[q][c][0], 1);
-        if (L == WV - 1) {
-            Y[c] = 1.0;
-#pragma unroll
-            for (int q = 0; q < NC; q++)
-                X[q][c] = 0.0;
-        }
+            X[q][c] = wv::from_next(A[q][c][0], 0.0);
     }
 #pragma unroll
-    for (int s = 1; s < WV; s <<= 1) {
+    for (int c = 0; c < 2; c++) {
+        double Xc[NC];
 #pragma unroll
-        for (int c = 0; c < 2; c++) {
-            const double Yn = shd(Y[c], s);
-            double Xn[NC];
+        for (int q = 0; q < NC; q++)
+            Xc[q] = X[q][c];
+        wv::scan_suffix<NC, true>(Xc, Y[c], L);
 #pragma unroll
-            for (int q = 0; q < NC; q++)
-                Xn[q] = shd(X[q][c], s);
-            if (L + s < WV) {
-#pragma unroll
-                for (int q = 0; q < NC; q++)
-                    X[q][c] = X[q][c] + Y[c] * Xn[q];
-                Y[c] = Y[c] * Yn;
-            }
-        }
+        for (int q = 0; q < NC; q++)
+            X[q][c] = Xc[q];
     }
     double rho[NC][2][E];
 #pragma unroll
@@ -224,33 +188,15 @@ __device__ __forceinline__ void solve_w(int n, double lam, const double2 (&gp)[2
     double Xp[NC][2], Yp[2];
 #pragma unroll
     for (int c = 0; c < 2; c++) {
-        Yp[c] = shu(Bp[c][E - 1], 1);
+        Yp[c] = wv::from_prev(Bp[c][E - 1], 1.0);
+        double Xc[NC];
 #pragma unroll
         for (int q = 0; q < NC; q++)
-            Xp[q][c] = shu(Ap[q][c][E - 1], 1);
-        if (L == 0) {
-            Yp[c] = 1.0;
+            Xc[q] = wv::from_prev(Ap[q][c][E - 1], 0.0);
+        wv::scan_prefix<NC, true>(Xc, Yp[c]);
 #pragma unroll
-            for (int q = 0; q < NC; q++)
-                Xp[q][c] = 0.0;
-        }
-    }
-#pragma unroll
-    for (int s = 1; s < WV; s <<= 1) {
-#pragma unroll
-        for (int c = 0; c < 2; c++) {
-            const double Yo = shu(Yp[c], s);
-            double Xo[NC];
-#pragma unroll
-            for (int q = 0; q < NC; q++)
-                Xo[q] = shu(Xp[q][c], s);
-            if (L >= s) {
-#pragma unroll
-                for (int q = 0; q < NC; q++)
-                    Xp[q][c] = Xp[q][c] + Yp[c] * Xo[q];
-                Yp[c] = Yp[c] * Yo;
-            }
-        }
+        for (int q = 0; q < NC; q++)
+            Xp[q][c] = Xc[q];
     }
     // P_M = A'_M + B'_M X', Q_M = B'_M Y' (B' the lane-local products), then
     // the boundary row of each class: sums [q*2+c] of P and [2*NC+c] of Q
