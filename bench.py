@@ -43,18 +43,20 @@ FP64_PEAK_TFLOPS = 78.6
 REF_BIN = os.path.join(HERE, "oracle", "_ref", "bla_128x97x128_omp16")
 
 
-def alg_per_launch():
+def alg_per_launch(mixed: bool):
     """Algorithmic HBM bytes (or flops) of one launch of each kernel family.
 
     F = nz*ny*nx/2 complex values per field, Fp = ny*nzp*nx/2 per padded
-    z-physical field, 16 bytes per FP64 complex value; the z-oddball plane
-    is skipped by step 3 (nz-1 planes).
+    z-physical field; a stored complex value is 16 bytes in FP64 and 8 bytes
+    in the FP32 copies of the mixed form (step 2 and the transforms), while
+    the partial right-hand sides p_v, p_om stay FP64; the z-oddball plane is
+    skipped by step 3 (nz-1 planes).
     """
     nxh, nzp = NX // 2, NZ + NZ // 2
     F = NZ * NY * nxh
     Fp = NY * nzp * nxh
     F3 = (NZ - 1) * NY * nxh
-    b = 16
+    b = 8 if mixed else 16
     return {
         # read u,v,w,om_x,om_z; write six z-synthesised fields
         "zsyn": ("hbm", (5 * F + 6 * Fp) * b),
@@ -62,33 +64,32 @@ def alg_per_launch():
         "xrow": ("hbm", (6 * Fp + 3 * Fp) * b),
         # read three products, write H into three storage slots
         "zana": ("hbm", (3 * Fp + 3 * F) * b),
-        # read H (3 fields) and p_v, p_om; write u,v,w,om_x,om_z and p_v, p_om
-        "tau": ("hbm", (5 * F3 + 7 * F3) * b),
+        # read H (3 fields), write u,v,w,om_x,om_z; read and write p_v, p_om (FP64)
+        "tau": ("hbm", (3 * F3 + 5 * F3) * b + 4 * F3 * 16),
         # dense ny x ny transform of 3 (fwd) / 5 (bwd) complex fields
         "cheb_fwd": ("flops", 2.0 * NY * NY * 2 * nxh * (NZ - 1) * 3),
         "cheb_bwd": ("flops", 2.0 * NY * NY * 2 * nxh * (NZ - 1) * 5),
     }
 
 
-TRAFFIC_FILE = os.path.join(HERE, "profiles", "hbm_traffic_c2_128x97x128_fp64.md")
-PROFILE_KEY = {"zsyn": "k_zsyn_t<192>", "xrow": "k_xrow_t<96>", "zana": "k_zana_t<192>",
-               "tau": "k_tau_w<1>", "cheb_fwd": "k_gemm fwd", "cheb_bwd": "k_gemm bwd"}
+def traffic_file(mixed: bool) -> str:
+    form = "mixed" if mixed else "fp64"
+    return os.path.join(HERE, "profiles", f"hbm_traffic_c2_128x97x128_{form}.md")
 
 
-def profiled_traffic(family: str):
+def profiled_traffic(family: str, mixed: bool):
     """HBM bytes per launch of a kernel family from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
+    FETCH_SIZE/WRITE_SIZE summary of that form (copy-calibrated), or None."""
     try:
-        text = open(TRAFFIC_FILE).read()
+        text = open(traffic_file(mixed)).read()
     except OSError:
         return None
     m = re.search(r"Machine-readable \(bytes per launch\): (.*)", text)
-    key = PROFILE_KEY.get(family)
-    if not m or not key:
+    if not m:
         return None
     for item in m.group(1).split(", "):
         k, _, v = item.rpartition("=")
-        if k == key:
+        if k.strip() == family:
             return int(v)
     return None
 
@@ -124,6 +125,71 @@ def cpu_baseline(nsteps: int = 300, threads: int = 16):
         shutil.rmtree(work, ignore_errors=True)
 
 
+FP32_PEAK_TFLOPS = 157.3
+
+
+def time_form(precision: int, steps: int, warmup: int):
+    """Run one form of the step on the bench workload; returns steps/s,
+    seconds, per-family (total ms, launches) and the last CFL*dt."""
+    from simson_amd import fields
+    from simson_amd.gpu import BlaGPU
+
+    f = fields.channel_field(NX, NY, NZ, XL, ZL, RE, kmax=16, rms=0.1, seed=2)
+    g = BlaGPU(NX, NY, NZ, RE, XL, ZL, cflux=True, precision=precision)
+    try:
+        g.upload_velocity(f.u, -2.0 / RE)
+        g.run(warmup, DT, icfl=3, first=True)
+        g.sync()
+        g.timing(True)
+        g.timing_read()                     # drop anything recorded so far
+        t0 = time.perf_counter()
+        cfl = g.run(steps, DT, icfl=3, first=False)
+        g.sync()
+        t1 = time.perf_counter()
+        kern = g.timing_read()
+        g.timing(False)
+    finally:
+        g.close()
+    return steps / (t1 - t0), t1 - t0, kern, cfl
+
+
+def kernel_table(kern: dict, mixed: bool):
+    alg = alg_per_launch(mixed)
+    flop_peak = FP32_PEAK_TFLOPS if mixed else FP64_PEAK_TFLOPS
+    total = max(1e-12, sum(v[0] for v in kern.values()))
+    kernels = {}
+    for name, (ms, cnt) in kern.items():
+        if cnt == 0:
+            continue
+        avg_us = 1e3 * ms / cnt
+        entry = {"launches": cnt, "avg_us": round(avg_us, 3), "share": round(ms / total, 4)}
+        if name in alg:
+            kind, amount = alg[name]
+            if kind == "hbm":
+                gbs = amount / (avg_us * 1e-6) / 1e9
+                entry.update(bound="hbm", alg_bytes=int(amount), achieved_GBs=round(gbs, 1),
+                             frac=round(gbs / HBM_PEAK_GBS, 4))
+            else:
+                tf = amount / (avg_us * 1e-6) / 1e12
+                entry.update(bound="mfma", alg_flops=int(amount), achieved_TFLOPs=round(tf, 3),
+                             frac=round(tf / flop_peak, 4))
+        kernels[name] = entry
+    return kernels
+
+
+def roofline_of(kernels: dict, kern: dict, mixed: bool):
+    alg = alg_per_launch(mixed)
+    dom = max((k for k in kernels if k in alg), key=lambda k: kern[k][0])
+    d = kernels[dom]
+    if d["bound"] == "hbm":
+        return {"kernel": dom, "bound": "hbm", "achieved": d["achieved_GBs"],
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["frac"],
+                "traffic": profiled_traffic(dom, mixed)}
+    return {"kernel": dom, "bound": "mfma", "achieved": d["achieved_TFLOPs"],
+            "peak": FP32_PEAK_TFLOPS if mixed else FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": d["frac"], "traffic": profiled_traffic(dom, mixed)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +197,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--form", choices=["both", "mixed", "fp64"], default="both",
+                    help="which forms of the step to time (the headline value is the "
+                         "mixed form; 'fp64' reports the FP64 step as the value)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy a device buffer of this size three times "
                          "(a plain copy of known bytes that scales rocprofv3 --pmc "
@@ -144,23 +213,13 @@ def main():
     # the CPU leg runs first, before this process touches the GPU
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_steps)
 
-    from simson_amd import fields
-    from simson_amd.gpu import BlaGPU
+    from simson_amd.gpu import SIMSON_FP64, SIMSON_MIXED
 
-    f = fields.channel_field(NX, NY, NZ, XL, ZL, RE, kmax=16, rms=0.1, seed=2)
-    g = BlaGPU(NX, NY, NZ, RE, XL, ZL, cflux=True)
-    g.upload_velocity(f.u, -2.0 / RE)
-    g.run(args.warmup, DT, icfl=3, first=True)
-    g.sync()
-    g.timing(True)
-    g.timing_read()                     # drop anything recorded so far
-    t0 = time.perf_counter()
-    cfl = g.run(args.steps, DT, icfl=3, first=False)
-    g.sync()
-    t1 = time.perf_counter()
-    kern = g.timing_read()
-    g.timing(False)
-    g.close()
+    res = {}
+    if args.form in ("both", "fp64"):
+        res["fp64"] = time_form(SIMSON_FP64, args.steps, args.warmup)
+    if args.form in ("both", "mixed"):
+        res["mixed"] = time_form(SIMSON_MIXED, args.steps, args.warmup)
     if args.calib_copy_mib > 0:
         import ctypes
         from simson_amd import build as _b
@@ -170,38 +229,18 @@ def main():
         if cal.simson_calib_copy(args.calib_copy_mib << 20, 3) != 0:
             print("calibration copy failed", file=sys.stderr)
 
-    elapsed = t1 - t0
-    steps_per_s = args.steps / elapsed
-    alg = alg_per_launch()
-    kernels = {}
-    for name, (ms, cnt) in kern.items():
-        avg_us = 1e3 * ms / cnt
-        entry = {"launches": cnt, "avg_us": round(avg_us, 3),
-                 "share": round(ms / max(1e-12, sum(v[0] for v in kern.values())), 4)}
-        if name in alg:
-            kind, amount = alg[name]
-            if kind == "hbm":
-                gbs = amount / (avg_us * 1e-6) / 1e9
-                entry.update(bound="hbm", alg_bytes=int(amount), achieved_GBs=round(gbs, 1),
-                             frac=round(gbs / HBM_PEAK_GBS, 4))
-            else:
-                tf = amount / (avg_us * 1e-6) / 1e12
-                entry.update(bound="fp64", alg_flops=int(amount), achieved_TFLOPs=round(tf, 3),
-                             frac=round(tf / FP64_PEAK_TFLOPS, 4))
-        kernels[name] = entry
-    dom = max((k for k in kernels if k in alg), key=lambda k: kern[k][0])
-    d = kernels[dom]
-    if d["bound"] == "hbm":
-        roofline = {"kernel": dom, "bound": "hbm", "achieved": d["achieved_GBs"],
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["frac"],
-                    "traffic": profiled_traffic(dom)}
-    else:
-        roofline = {"kernel": dom, "bound": "mfma", "achieved": d["achieved_TFLOPs"],
-                    "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": d["frac"],
-                    "traffic": profiled_traffic(dom)}
+    head = "mixed" if "mixed" in res else "fp64"
+    mixed = head == "mixed"
+    sps, elapsed, kern, cfl = res[head]
+    kernels = kernel_table(kern, mixed)
+    forms = {}
+    for name, (fs, fel, fk, fc) in res.items():
+        forms[name] = {"steps_per_s": round(fs, 3), "ms_per_step": round(1e3 * fel / args.steps, 4),
+                       "grid_points_steps_per_s": round(fs * NX * NY * NZ, 1),
+                       "cfl_dt": fc, "kernels": kernel_table(fk, name == "mixed")}
     out = {
         "metric": METRIC,
-        "value": round(steps_per_s, 3),
+        "value": round(sps, 3),
         "unit": "time-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -210,16 +249,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic",
         "config": {"workload": "Re_tau~180 plane channel 128x97x128, RK3-CN dt=0.005, "
-                               "constant flux, FP64 step",
+                               "constant flux, " + ("FP32-mixed step" if mixed else "FP64 step"),
                    "nx": NX, "ny": NY, "nz": NZ, "re_cl": RE, "lx": "4pi", "lz": "4pi/3",
-                   "dt": DT, "icfl": 3, "form": "fp64", "parallelism": "single GPU"},
-        "grid_points_steps_per_s": round(steps_per_s * NX * NY * NZ, 1),
+                   "dt": DT, "icfl": 3, "form": head, "parallelism": "single GPU"},
+        "grid_points_steps_per_s": round(sps * NX * NY * NZ, 1),
+        "mixed_vs_fp64": (round(res["mixed"][0] / res["fp64"][0], 3)
+                          if "mixed" in res and "fp64" in res else None),
         "cfl_dt": cfl,
-        "roofline": roofline,
-        "kernels": kernels,
+        "roofline": roofline_of(kernels, kern, mixed),
+        "forms": forms,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

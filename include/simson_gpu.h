@@ -34,10 +34,19 @@ extern "C" {
 #define SIMSON_E_NODEV (-3)     /* no HIP device */
 
 #define SIMSON_FP64 0           /* all arithmetic in FP64 */
+/* Mixed precision: the x-z FFTs, the nonlinear products and the y
+ * Chebyshev transforms run in FP32 (transforms on FP32 matrix cores with
+ * FP32 accumulation) on FP32 copies of the fields; the Chebyshev-tau solves,
+ * the kx = kz = 0 mean flow, the pressure-gradient / flux update and the
+ * RK3-CN partial right-hand sides p_v, p_omega, puw stay FP64.  Uploads,
+ * prhs and downloads are FP64 (downloaded velocities carry FP32 rounding).
+ * Requires nzp = 3nz/2 and nxp/2 = 3nx/4 among the specialised FFT lengths
+ * and ny <= 127. */
+#define SIMSON_MIXED 1
 
 typedef struct simson_params {
     int nx, ny, nz;      /* par.f nx, ny (= nyp, nfyd = 0), nz; nfxd = nfzd = 1 */
-    int precision;       /* SIMSON_FP64 */
+    int precision;       /* SIMSON_FP64 or SIMSON_MIXED */
     int ibc;             /* bla.i ibc; only 0 (no-slip walls) */
     int cflux;           /* bla.i cflux (constant mass flux) */
     double re;           /* Reynolds number from the .u header */

@@ -45,6 +45,12 @@ struct simson_gpu {
     long clock = 0;
     double *Tf = nullptr, *Tp = nullptr, *Tb = nullptr;
     double *alfa = nullptr, *beta = nullptr, *deta = nullptr;
+    // mixed form: FP32 copies of U, P, C and of the tables; U is refreshed
+    // from U32 before a download (u64_stale) and U32 from U after an upload
+    // or prhs (u32_stale)
+    bool mixed = false, u32_stale = false, u64_stale = false;
+    float2 *U32 = nullptr, *P32 = nullptr, *C32 = nullptr, *twz32 = nullptr, *twx32 = nullptr;
+    float *Tf32 = nullptr, *Tb32 = nullptr;
     FftPlan pz, pm;
     double mflux = 0., u0upp = 0., u0low = 0., w0upp = 0., w0low = 0.;
     bool timing = false;
@@ -134,7 +140,8 @@ void free_all(simson_gpu_t *g)
                     g->cfl_blk, g->cfl, g->Tf, g->Tp, g->Tb, g->alfa, g->beta, g->deta,
                     g->kfac.gp, g->lam[0].gp, g->lam[0].vh, g->lam[0].sh,
                     g->lam[1].gp, g->lam[1].vh, g->lam[1].sh,
-                    g->lam[2].gp, g->lam[2].vh, g->lam[2].sh};
+                    g->lam[2].gp, g->lam[2].vh, g->lam[2].sh, g->U32, g->P32, g->C32,
+                    g->twz32, g->twx32, g->Tf32, g->Tb32};
     for (void *q : ptrs)
         if (q)
             (void)hipFree(q);
@@ -183,6 +190,7 @@ NonlinArgs nonlin_args(simson_gpu_t *g, int do_cfl)
     a.xl = g->p.xl;
     a.zl = g->p.zl;
     a.do_cfl = do_cfl;
+    a.split_mean = 0;
     return a;
 }
 
@@ -191,6 +199,7 @@ ColArgs col_args(simson_gpu_t *g)
     ColArgs a;
     a.U = g->U;
     a.C = g->C;
+    a.C32 = g->C32;
     a.Pp = g->Pp;
     a.W = g->W;
     a.puw = g->puw;
@@ -254,14 +263,76 @@ int lam_slot(simson_gpu_t *g, double apb)
     return v;
 }
 
+Step2Args<float2> nonlin_args32(simson_gpu_t *g, int do_cfl)
+{
+    const NonlinArgs d = nonlin_args(g, do_cfl);
+    Step2Args<float2> a;
+    a.U = g->U32;
+    a.Uw = g->U32;
+    a.P = g->P32;
+    a.alfa = d.alfa;
+    a.beta = d.beta;
+    a.deta = d.deta;
+    a.cfl_blk = d.cfl_blk;
+    a.nx = d.nx;
+    a.ny = d.ny;
+    a.nz = d.nz;
+    a.nxh = d.nxh;
+    a.nxp = d.nxp;
+    a.nzp = d.nzp;
+    a.rot = d.rot;
+    a.xl = d.xl;
+    a.zl = d.zl;
+    a.do_cfl = d.do_cfl;
+    a.split_mean = 1;
+    return a;
+}
+
+size_t field_size(const simson_gpu_t *g) { return (size_t)g->nz * g->ny * g->nxh; }
+
+// mixed form: FP32 state from the FP64 one after an upload or prhs
+void refresh_u32(simson_gpu_t *g)
+{
+    if (g->mixed && g->u32_stale) {
+        launch_narrow(g->U, g->U32, kNPhys * field_size(g), g->stream);
+        g->u32_stale = false;
+    }
+}
+
 int do_nonlin(simson_gpu_t *g, int do_cfl)
 {
+    if (g->mixed) {
+        refresh_u32(g);
+        Step2Args<float2> a = nonlin_args32(g, do_cfl);
+        timed(g, F_ZSYN, [&] { launch_zsyn4(a, g->twz32, g->stream); });
+        timed(g, F_XROW, [&] { launch_xrow4(a, g->twx32, g->stream); });
+        if (do_cfl)
+            timed(g, F_CFL, [&] {
+                launch_cfl_reduce(g->cfl_blk, xrow4_blocks(a), g->cfl, g->stream);
+            });
+        timed(g, F_ZANA, [&] { launch_zana4(a, g->twz32, g->stream); });
+        g->u64_stale = true;
+        CK(hipGetLastError());
+        return 0;
+    }
     NonlinArgs a = nonlin_args(g, do_cfl);
-    timed(g, F_ZSYN, [&] { launch_zsyn(a, g->pz, g->stream); });
-    timed(g, F_XROW, [&] { launch_xrow(a, g->pm, g->twx, g->stream); });
+    timed(g, F_ZSYN, [&] {
+        if (!launch_zsyn4(a, (const double2 *)g->twz, g->stream))
+            launch_zsyn(a, g->pz, g->stream);
+    });
+    bool four = false;
+    timed(g, F_XROW, [&] {
+        four = launch_xrow4(a, (const double2 *)g->twx, g->stream);
+        if (!four)
+            launch_xrow(a, g->pm, g->twx, g->stream);
+    });
+    const int nblk = four ? xrow4_blocks(a) : xrow_blocks(a);
     if (do_cfl)
-        timed(g, F_CFL, [&] { launch_cfl_reduce(g->cfl_blk, xrow_blocks(a), g->cfl, g->stream); });
-    timed(g, F_ZANA, [&] { launch_zana(a, g->pz, g->stream); });
+        timed(g, F_CFL, [&] { launch_cfl_reduce(g->cfl_blk, nblk, g->cfl, g->stream); });
+    timed(g, F_ZANA, [&] {
+        if (!launch_zana4(a, (const double2 *)g->twz, g->stream))
+            launch_zana(a, g->pz, g->stream);
+    });
     CK(hipGetLastError());
     return 0;
 }
@@ -270,31 +341,51 @@ int do_linear(simson_gpu_t *g, double an, double bn, double anp1, double bnp1)
 {
     const int odd = g->nz / 2;
     const int slot = lam_slot(g, an + bn);
+    if (g->mixed)
+        refresh_u32(g);
     GemmIO fw = gemm_io(g, g->U, 0, g->C, 1, {0, 1, 2}, {0, 1, 2}, odd);
-    timed(g, F_CHF, [&] { launch_cheb_gemm(g->Tf, fw, g->stream); });
+    timed(g, F_CHF, [&] {
+        if (g->mixed)
+            launch_cheb_mfma_f32(g->Tf32, (const float *)g->U32, (float *)g->C32, fw, g->stream);
+        else
+            launch_cheb_gemm(g->Tf, fw, g->stream);
+    });
     ColArgs ca = col_args(g);
     int st = 0;
     timed(g, F_TAU, [&] {
-        st = launch_tau(ca, g->lam[slot], g->kfac, an, bn, anp1, bnp1, g->stream);
+        st = launch_tau(ca, g->lam[slot], g->kfac, an, bn, anp1, bnp1, g->mixed, g->stream);
     });
     if (st) {
         g->err = "unsupported ny for the column solver";
         return st;
     }
     GemmIO bw = gemm_io(g, g->C, 1, g->U, 0, {0, 1, 2, 3, 4}, {0, 1, 2, 3, 4}, odd);
-    timed(g, F_CHB, [&] { launch_cheb_gemm(g->Tb, bw, g->stream); });
+    timed(g, F_CHB, [&] {
+        if (g->mixed)
+            launch_cheb_mfma_f32(g->Tb32, (const float *)g->C32, (float *)g->U32, bw, g->stream);
+        else
+            launch_cheb_gemm(g->Tb, bw, g->stream);
+    });
+    if (g->mixed)
+        g->u64_stale = true;
     CK(hipGetLastError());
     return 0;
 }
 
+// The start-up partial RHS runs in FP64 in both forms, on the FP64 fields.
 int do_prhs(simson_gpu_t *g, double an)
 {
+    if (g->mixed && g->u64_stale) {
+        launch_widen(g->U32, g->U, kNPhys * field_size(g), g->stream);
+        g->u64_stale = false;
+    }
     GemmIO fw = gemm_io(g, g->U, 0, g->C, 1, {0, 1, 2}, {0, 1, 2}, -1);
     timed(g, F_PCHF, [&] { launch_cheb_gemm(g->Tp, fw, g->stream); });
     ColArgs ca = col_args(g);
     timed(g, F_PRHS, [&] { launch_prhs(ca, an, g->stream); });
     GemmIO bw = gemm_io(g, g->C, 1, g->U, 0, {3, 4}, {3, 4}, -1);
     timed(g, F_PCHB, [&] { launch_cheb_gemm(g->Tb, bw, g->stream); });
+    g->u32_stale = g->mixed;
     CK(hipGetLastError());
     return 0;
 }
@@ -314,7 +405,8 @@ int simson_gpu_init(const simson_params_t *p, simson_gpu_t **out)
     if (!p || !out)
         return SIMSON_E_ARG;
     *out = nullptr;
-    if (p->precision != SIMSON_FP64 || p->ibc != 0 || p->nx < 4 || p->nz < 4 ||
+    if ((p->precision != SIMSON_FP64 && p->precision != SIMSON_MIXED) || p->ibc != 0 ||
+        p->nx < 4 || p->nz < 4 ||
         p->nx % 4 != 0 || p->nz % 2 != 0 || p->ny < 9 || p->ny % 2 == 0 ||
         p->re <= 0 || p->xl <= 0 || p->zl <= 0)
         return SIMSON_E_ARG;
@@ -325,6 +417,10 @@ int simson_gpu_init(const simson_params_t *p, simson_gpu_t **out)
     // 3 x 128 Chebyshev coefficients; the generic FFT kernels keep two z
     // lines (and twelve half x rows) in 64 KiB of LDS
     if (p->ny > 383 || 2 * nzp * 16 > 65536 || 12 * (nxp / 2) * 16 > 65536)
+        return SIMSON_E_ARG;
+    // the FP32 path exists only as the specialised kernels
+    if (p->precision == SIMSON_MIXED &&
+        (!has_zfft4(nzp) || !has_xrow4(nxp / 2) || cheb_mfma_ks(p->ny) == 0))
         return SIMSON_E_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
@@ -415,6 +511,28 @@ int simson_gpu_init(const simson_params_t *p, simson_gpu_t **out)
         h2d(g, g->Tp, Tp.data(), Tp.size()) || h2d(g, g->Tb, Tb.data(), Tb.size()) ||
         h2d(g, g->twz, twz.data(), twz.size()) || h2d(g, g->twx, twx.data(), twx.size()))
         return fail(SIMSON_E_HIP);
+    // FP32 state and tables of the mixed form (rounded from the FP64 ones)
+    std::vector<float2> twz32(nzp), twx32(nxp);
+    std::vector<float> Tf32((size_t)ny * ny), Tb32((size_t)ny * ny);
+    g->mixed = p->precision == SIMSON_MIXED;
+    if (g->mixed) {
+        for (int i = 0; i < nzp; i++)
+            twz32[i] = make_float2((float)twz[i].x, (float)twz[i].y);
+        for (int i = 0; i < nxp; i++)
+            twx32[i] = make_float2((float)twx[i].x, (float)twx[i].y);
+        for (size_t i = 0; i < Tf32.size(); i++) {
+            Tf32[i] = (float)Tf[i];
+            Tb32[i] = (float)Tb[i];
+        }
+        if (dalloc(g, &g->U32, kNPhys * F) || dalloc(g, &g->P32, (size_t)6 * ny * nzp * nxh) ||
+            dalloc(g, &g->C32, 5 * F) || dalloc(g, &g->twz32, (size_t)nzp) ||
+            dalloc(g, &g->twx32, (size_t)nxp) || dalloc(g, &g->Tf32, Tf32.size()) ||
+            dalloc(g, &g->Tb32, Tb32.size()) ||
+            h2d(g, g->twz32, twz32.data(), twz32.size()) ||
+            h2d(g, g->twx32, twx32.data(), twx32.size()) ||
+            h2d(g, g->Tf32, Tf32.data(), Tf32.size()) || h2d(g, g->Tb32, Tb32.data(), Tb32.size()))
+            return fail(SIMSON_E_HIP);
+    }
     // pressure gradient set-up (bla.f:1622-1662)
     double px;
     if (p->cflux) {
@@ -481,6 +599,8 @@ int simson_gpu_upload(simson_gpu_t *g, const double *ur, const double *ui, int n
     g->u0low = v[1];
     g->w0upp = v[2];
     g->w0low = v[3];
+    g->u32_stale = g->mixed;
+    g->u64_stale = false;
     CK(hipStreamSynchronize(g->stream));
     return 0;
 }
@@ -595,6 +715,11 @@ int simson_gpu_download(simson_gpu_t *g, double *ur, double *ui, int nslots, dou
     if (!g || nslots < 0 || nslots > kNSlot || (nslots > 0 && (!ur || !ui)))
         return SIMSON_E_ARG;
     const size_t F = (size_t)g->nz * g->ny * g->nxh;
+    if (g->mixed && g->u64_stale) {
+        launch_widen(g->U32, g->U, kNPhys * F, g->stream);
+        g->u64_stale = false;
+        CK(hipGetLastError());
+    }
     if (nslots > 0) {
         const int nphys = nslots < kNPhys ? nslots : kNPhys;
         std::vector<double2> h(nphys * F);

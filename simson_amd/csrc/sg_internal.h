@@ -55,6 +55,7 @@ struct LamFactor {
 struct ColArgs {
     double2 *U;           // physical slots [5][nz][ny][nxh]
     double2 *C;           // Chebyshev columns [5][nz][nxh][ny]
+    float2 *C32;          // the same in FP32 (mixed form)
     double2 *Pp;          // partial RHS columns [2][nz][nxh][ny]
     double *W;            // start-up scratch [kScratch][ny][nz*nxh]
     double *puw;          // [2][ny]
@@ -69,25 +70,56 @@ struct ColArgs {
 };
 
 // Step-2 launch arguments.
-struct NonlinArgs {
-    const double2 *U;
-    double2 *Uw;
-    double2 *P;           // [6][ny][nzp][nxh]
+// V is double2 (FP64 step) or float2 (FP32 step 2 of the mixed form).
+template <typename V>
+struct Step2Args {
+    const V *U;           // physical slots [5][nz][ny][nxh]
+    V *Uw;                // H_1..3 into slots 0-2 of the same layout
+    V *P;                 // [6][ny][nzp][nxh]
     const double *alfa, *beta, *deta;
     double *cfl_blk;
     int nx, ny, nz, nxh, nxp, nzp;
     double rot;
     double xl, zl;
     int do_cfl;
+    // split_mean: the kx = kz = 0 mode (the mean profiles U, W, Om_x, Om_z
+    // of each y plane) stays out of the transformed fields and enters the
+    // products as exact per-plane coefficients, so that FP32 fields carry
+    // the fluctuations at full FP32 relative precision (mixed form)
+    int split_mean;
 };
+typedef Step2Args<double2> NonlinArgs;
 
 void launch_zsyn(const NonlinArgs &a, const FftPlan &pz, hipStream_t s);
 void launch_xrow(const NonlinArgs &a, const FftPlan &pm, const double2 *wx, hipStream_t s);
 void launch_zana(const NonlinArgs &a, const FftPlan &pz, hipStream_t s);
 int  xrow_blocks(const NonlinArgs &a);
+// four-step x-row kernel (register DFTs, in-place LDS passes); false when
+// nxp/2 has no specialisation.  Blocks written to cfl_blk: xrow4_blocks.
+template <typename V>
+bool launch_xrow4(const Step2Args<V> &a, const V *wx, hipStream_t s);
+template <typename V>
+int xrow4_blocks(const Step2Args<V> &a);
+// four-step z kernels; false when nzp has no specialisation
+template <typename V>
+bool launch_zsyn4(const Step2Args<V> &a, const V *tw, hipStream_t s);
+template <typename V>
+bool launch_zana4(const Step2Args<V> &a, const V *tw, hipStream_t s);
 void launch_cfl_reduce(const double *blk, int n, double *out, hipStream_t s);
 
 void launch_cheb_gemm(const double *A, const GemmIO &io, hipStream_t s);
+// MFMA form of the transform for ny <= 128; false when ny is outside it
+bool launch_cheb_mfma(const double *A, const GemmIO &io, hipStream_t s);
+// FP32 form (mixed precision): in/out replace io.in/io.out
+bool launch_cheb_mfma_f32(const float *A, const float *in, float *out, const GemmIO &io,
+                          hipStream_t s);
+int  cheb_mfma_ks(int ny);
+// nzp / nxp/2 with a four-step step-2 kernel (needed by the FP32 path)
+bool has_zfft4(int nzp);
+bool has_xrow4(int m);
+// FP64 <-> FP32 copies of n complex values
+void launch_narrow(const double2 *src, float2 *dst, size_t n, hipStream_t s);
+void launch_widen(const float2 *src, double2 *dst, size_t n, hipStream_t s);
 void launch_prhs(const ColArgs &a, double an, hipStream_t s);
 
 // tau factorisations: k^2 once, lam^2 for one value of a+b
@@ -95,7 +127,8 @@ void launch_factor_k2(const ColArgs &a, LamFactor kf, hipStream_t s);
 void launch_factor_lam(const ColArgs &a, double apb, LamFactor lf, LamFactor kf,
                        hipStream_t s);
 // one linear substep, one wavefront per column
+// (fp32_cols: the Chebyshev columns are C32, else C)
 int  launch_tau(const ColArgs &a, LamFactor lf, LamFactor kf, double an, double bn,
-                double anp1, double bnp1, hipStream_t s);
+                double anp1, double bnp1, bool fp32_cols, hipStream_t s);
 
 }  // namespace sg

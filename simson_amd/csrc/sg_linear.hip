@@ -294,6 +294,8 @@ __global__ void __launch_bounds__(256) k_prhs(ColArgs a, double an)
 
 void launch_cheb_gemm(const double *A, const GemmIO &io, hipStream_t s)
 {
+    if (launch_cheb_mfma(A, io, s))
+        return;
     dim3 grid((2 * io.nxh + GBN - 1) / GBN, (io.ny + GBM - 1) / GBM, io.nf * io.nz);
     hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, s, A, io);
 }

@@ -235,8 +235,30 @@ __device__ __forceinline__ void solve_w(int n, double lam, const double2 (&gp)[2
         }
 }
 
-template <int E>
-__global__ void __launch_bounds__(256) k_tau_w(ColArgs a, LamFactor lf, LamFactor kf, double an,
+// Chebyshev-column storage: FP64 (C) or FP32 (C32, the mixed form, whose
+// transforms run in FP32); the solve itself is FP64 either way.
+template <typename CV>
+__device__ __forceinline__ CV *cols(const ColArgs &a);
+template <>
+__device__ __forceinline__ double2 *cols<double2>(const ColArgs &a)
+{
+    return a.C;
+}
+template <>
+__device__ __forceinline__ float2 *cols<float2>(const ColArgs &a)
+{
+    return a.C32;
+}
+__device__ __forceinline__ double2 widen(double2 v) { return v; }
+__device__ __forceinline__ double2 widen(float2 v) { return make_double2(v.x, v.y); }
+__device__ __forceinline__ void put(double2 *p, double x, double y) { *p = make_double2(x, y); }
+__device__ __forceinline__ void put(float2 *p, double x, double y)
+{
+    *p = make_float2((float)x, (float)y);
+}
+
+template <int E, typename CV>
+__global__ void __launch_bounds__(256, E == 1 ? 3 : 1) k_tau_w(ColArgs a, LamFactor lf, LamFactor kf, double an,
                                                double bn, double anp1, double bnp1)
 {
     const int L = threadIdx.x & (WV - 1);
@@ -258,7 +280,7 @@ __global__ void __launch_bounds__(256) k_tau_w(ColArgs a, LamFactor lf, LamFacto
     const double c1n = 2. * re / (anp1 + bnp1);
     const double c2 = 2. * re * bnp1 / (anp1 + bnp1);
     const size_t fs = (size_t)ncol * n;
-    const double2 *Cc = a.C + (size_t)t * n;
+    const CV *Cc = cols<CV>(a) + (size_t)t * n;
     const double2 *Pc = a.Pp + (size_t)t * n;
     const double2 *glp = lf.gp + (size_t)t * n;
     const double2 *gkp = kf.gp + (size_t)t * n;
@@ -273,8 +295,8 @@ __global__ void __launch_bounds__(256) k_tau_w(ColArgs a, LamFactor lf, LamFacto
             const int k = 2 * (L * E + e) + c;
             const bool ok = k < n;
             const double2 z2 = make_double2(0., 0.);
-            const double2 h1 = ok ? Cc[k] : z2, h2 = ok ? Cc[fs + k] : z2;
-            const double2 h3 = ok ? Cc[2 * fs + k] : z2;
+            const double2 h1 = ok ? widen(Cc[k]) : z2, h2 = ok ? widen(Cc[fs + k]) : z2;
+            const double2 h3 = ok ? widen(Cc[2 * fs + k]) : z2;
             const double2 p0 = ok ? Pc[k] : z2, p1 = ok ? Pc[fs + k] : z2;
             H1[0][c][e] = h1.x; H1[1][c][e] = h1.y;
             H2[0][c][e] = h2.x; H2[1][c][e] = h2.y;
@@ -364,7 +386,7 @@ __global__ void __launch_bounds__(256) k_tau_w(ColArgs a, LamFactor lf, LamFacto
     dcheb_w<2, E>(ph, d1, L);
     dcheb_w<2, E>(d1, d2, L);
     const double k2m = k2 - c1n;
-    double2 *Cw = a.C + (size_t)t * n;
+    CV *Cw = cols<CV>(a) + (size_t)t * n;
     double2 *Pw = a.Pp + (size_t)t * n;
 #pragma unroll
     for (int c = 0; c < 2; c++)
@@ -398,14 +420,14 @@ __global__ void __launch_bounds__(256) k_tau_w(ColArgs a, LamFactor lf, LamFacto
                 const double omr = om[0][c][e], omi = om[1][c][e];
                 const double dor = dom[0][c][e], doi = dom[1][c][e];
                 const double phr = ph[0][c][e], phi = ph[1][c][e];
-                Cw[k] = make_double2((-alfa * dvi + beta * omi) * k2i, (alfa * dvr - beta * omr) * k2i);
-                Cw[fs + k] = make_double2(v[0][c][e], v[1][c][e]);
-                Cw[2 * fs + k] = make_double2((-alfa * omi - beta * dvi) * k2i,
-                                              (alfa * omr + beta * dvr) * k2i);
-                Cw[3 * fs + k] = make_double2((-alfa * doi - beta * phi) * k2i,
-                                              (alfa * dor + beta * phr) * k2i);
-                Cw[4 * fs + k] = make_double2((-beta * doi + alfa * phi) * k2i,
-                                              (beta * dor - alfa * phr) * k2i);
+                put(Cw + k, (-alfa * dvi + beta * omi) * k2i, (alfa * dvr - beta * omr) * k2i);
+                put(Cw + fs + k, v[0][c][e], v[1][c][e]);
+                put(Cw + 2 * fs + k, (-alfa * omi - beta * dvi) * k2i,
+                    (alfa * omr + beta * dvr) * k2i);
+                put(Cw + 3 * fs + k, (-alfa * doi - beta * phi) * k2i,
+                    (alfa * dor + beta * phr) * k2i);
+                put(Cw + 4 * fs + k, (-beta * doi + alfa * phi) * k2i,
+                    (beta * dor - alfa * phr) * k2i);
             }
         return;
     }
@@ -466,35 +488,43 @@ __global__ void __launch_bounds__(256) k_tau_w(ColArgs a, LamFactor lf, LamFacto
                 continue;
             a.puw[k] = -d2uw[0][c][e] - c1n * uw[0][c][e] - c2 * H1[0][c][e];
             a.puw[n + k] = -d2uw[1][c][e] - c1n * uw[1][c][e] - c2 * H3[0][c][e];
-            Cw[k] = make_double2(uw[0][c][e], 0.);
-            Cw[fs + k] = make_double2(0., 0.);
-            Cw[2 * fs + k] = make_double2(uw[1][c][e], 0.);
-            Cw[3 * fs + k] = make_double2(duw[1][c][e], 0.);
-            Cw[4 * fs + k] = make_double2(-duw[0][c][e], 0.);
+            put(Cw + k, uw[0][c][e], 0.);
+            put(Cw + fs + k, 0., 0.);
+            put(Cw + 2 * fs + k, uw[1][c][e], 0.);
+            put(Cw + 3 * fs + k, duw[1][c][e], 0.);
+            put(Cw + 4 * fs + k, -duw[0][c][e], 0.);
         }
 }
 
 }  // namespace
 
-int launch_tau(const ColArgs &a, LamFactor lf, LamFactor kf, double an, double bn, double anp1,
-               double bnp1, hipStream_t s)
+template <typename CV>
+int launch_tau_t(const ColArgs &a, LamFactor lf, LamFactor kf, double an, double bn,
+                 double anp1, double bnp1, hipStream_t s)
 {
     const int ncol = a.nz * a.nxh;
     dim3 grid((ncol + 3) / 4);
     const int E = (a.ny + 127) / 128;
     switch (E) {
     case 1:
-        hipLaunchKernelGGL(k_tau_w<1>, grid, dim3(256), 0, s, a, lf, kf, an, bn, anp1, bnp1);
+        hipLaunchKernelGGL((k_tau_w<1, CV>), grid, dim3(256), 0, s, a, lf, kf, an, bn, anp1, bnp1);
         return 0;
     case 2:
-        hipLaunchKernelGGL(k_tau_w<2>, grid, dim3(256), 0, s, a, lf, kf, an, bn, anp1, bnp1);
+        hipLaunchKernelGGL((k_tau_w<2, CV>), grid, dim3(256), 0, s, a, lf, kf, an, bn, anp1, bnp1);
         return 0;
     case 3:
-        hipLaunchKernelGGL(k_tau_w<3>, grid, dim3(256), 0, s, a, lf, kf, an, bn, anp1, bnp1);
+        hipLaunchKernelGGL((k_tau_w<3, CV>), grid, dim3(256), 0, s, a, lf, kf, an, bn, anp1, bnp1);
         return 0;
     default:
         return SIMSON_E_ARG;
     }
+}
+
+int launch_tau(const ColArgs &a, LamFactor lf, LamFactor kf, double an, double bn, double anp1,
+               double bnp1, bool fp32_cols, hipStream_t s)
+{
+    return fp32_cols ? launch_tau_t<float2>(a, lf, kf, an, bn, anp1, bnp1, s)
+                     : launch_tau_t<double2>(a, lf, kf, an, bn, anp1, bnp1, s);
 }
 
 }  // namespace sg

@@ -1,0 +1,322 @@
+// x-row pass of step 2 (bla/nonlinbl.f) as a four-step FFT in LDS.
+//
+// Per (y, zp) row and field the half spectrum X_k, k < nx/2, is packed into
+// M = nxp/2 complex values Z (the real transform of length nxp as one
+// complex transform of length M, physical point pairs (x_2n, x_2n+1) in
+// (re, im)), transformed, multiplied pointwise (advection.f:52-72, with the
+// CFL partial of boxcfl.f) and transformed back (fft2db.f:45, fft2df.f:32).
+//
+// With M = M1 M2 every length-M transform is two in-place LDS passes of
+// register DFTs:
+//   inverse  pass 1  for each n1 < M1: M2-point DFT of the stride-M1 column
+//                    n1 + M1 n2, times exp(+2 pi i n1 k2 / M), stored back
+//                    at n1 + M1 k2;
+//            pass 2  for each k2 < M2: M1-point DFT of the contiguous block
+//                    M1 k2 + n1; position M1 k2 + k1 then holds the physical
+//                    pair n = k2 + M2 k1 (a digit-reversed order);
+//   forward  pass A  for each k2: M1-point DFT of block k2 (the same
+//                    digit-reversed positions), times exp(-2 pi i k2 c / M);
+//            pass B  for each c < M1: M2-point DFT of column c, which leaves
+//                    the spectrum in natural order.
+// Pass 2, the products and pass A touch only one block per task and run
+// fused in registers; each pass reads and writes the same LDS slots, so one
+// buffer serves all passes.  Rows are padded by one slot per M1 block.
+#include "sg_cplx.h"
+#include "sg_internal.h"
+
+namespace sg {
+
+namespace {
+
+template <int M>
+struct XPlan {
+    static constexpr int M1 = (M % 4 == 0) ? ((M > 96 && M % 8 == 0) ? 8 : 4)
+                                           : ((M % 2 == 0) ? 2 : 3);
+    static constexpr int M2 = M / M1;
+    static constexpr int SROW = M + M / M1;
+    static constexpr int TABN = 2 * M;
+};
+
+template <typename V, int M>
+struct XBlock {
+    static constexpr int budget = 52 * 1024;
+    static constexpr int per_row = 6 * XPlan<M>::SROW * (int)sizeof(V);
+    static constexpr int raw = (budget - XPlan<M>::TABN * (int)sizeof(V)) / per_row;
+    static constexpr int RW = raw < 1 ? 1 : (raw > 8 ? 8 : raw);
+    static constexpr int tasks1 = 6 * RW * XPlan<M>::M1;
+    static constexpr int tasks2 = RW * XPlan<M>::M2;
+    static constexpr int tmax = tasks1 > tasks2 ? tasks1 : tasks2;
+    static constexpr int NT = tmax >= 256 ? 256 : ((tmax + 63) / 64) * 64;
+    static constexpr size_t lds = (size_t)(XPlan<M>::TABN + 6 * RW * XPlan<M>::SROW) * sizeof(V);
+};
+
+template <typename V, int M>
+__global__ void __launch_bounds__((XBlock<V, M>::NT)) k_xrow4(Step2Args<V> a, const V *__restrict__ twg)
+{
+    typedef typename ScalarOf<V>::T T;
+    constexpr int M1 = XPlan<M>::M1, M2 = XPlan<M>::M2, SROW = XPlan<M>::SROW;
+    constexpr int TABN = XPlan<M>::TABN;
+    constexpr int RW = XBlock<V, M>::RW, NT = XBlock<V, M>::NT;
+    constexpr int S1 = M1 + 1;                   // padded block stride
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    V *tw = reinterpret_cast<V *>(lds_raw);      // exp(+2 pi i t / (2M))
+    V *A = tw + TABN;                            // [6][RW][SROW]
+    const int nxh = a.nxh, nzp = a.nzp, ny = a.ny;
+    const int zp0 = blockIdx.x * RW, y = blockIdx.y;
+    const int nr = min(RW, nzp - zp0);
+    const int tid = threadIdx.x;
+    for (int i = tid; i < TABN; i += NT)
+        tw[i] = twg[i];
+    __syncthreads();
+
+    // half spectrum X_k (k < nx/2, zero above) -> packed Z_k, k < M
+    for (int idx = tid; idx < 6 * RW * M; idx += NT) {
+        const int f = idx / (RW * M);
+        const int rem = idx - f * RW * M;
+        const int r = rem / M, k = rem - r * M;
+        V Z = vmk<V>(0, 0);
+        if (r < nr) {
+            const V *row = a.P + (((size_t)f * ny + y) * nzp + zp0 + r) * nxh;
+            V Xk = k < nxh ? row[k] : vmk<V>(0, 0);
+            if (k == 0)
+                Xk.y = 0;                        // only Re X_0 enters vrfftb
+            const int km = M - k;                // X_M (k = 0) is beyond nx/2
+            V Xm = (k > 0 && km < nxh) ? row[km] : vmk<V>(0, 0);
+            Xm.y = -Xm.y;
+            const V s = vadd(Xk, Xm), d = vsub(Xk, Xm);
+            Z = vadd(s, vmul_is(vmul(tw[k], d), (T)1));
+        }
+        A[(f * RW + r) * SROW + k + k / M1] = Z;
+    }
+    __syncthreads();
+
+    // inverse pass 1: six fields
+    for (int t = tid; t < 6 * RW * M1; t += NT) {
+        const int fr = t / M1, n1 = t - fr * M1;
+        V *row = A + fr * SROW + n1;
+        V v[M2];
+#pragma unroll
+        for (int n2 = 0; n2 < M2; n2++)
+            v[n2] = row[S1 * n2];
+        reg_dft<V, M2, +1, TABN>(v, tw);
+#pragma unroll
+        for (int k2 = 1; k2 < M2; k2++)
+            if (n1 > 0)
+                v[k2] = vmul(v[k2], tw[2 * n1 * k2]);
+#pragma unroll
+        for (int k2 = 0; k2 < M2; k2++)
+            row[S1 * k2] = v[k2];
+    }
+    __syncthreads();
+
+    // inverse pass 2, products, forward pass A (one block per task)
+    T cflmax = 0;
+    T dyinv = (T)(1.0 / a.deta[y]);
+    if (a.ny >= 17)
+        dyinv = fmin(dyinv, (T)((double)(a.ny - 1) * (double)(a.ny - 1) / 10. / 3.14));
+    const T dxi = (T)((double)(a.nx - 2) / a.xl);
+    const T dzi = (T)((double)(a.nz - 2) / a.zl);
+    const T r2 = (T)(2.0 * a.rot);
+    // mean profiles of this plane when the fields hold only fluctuations
+    // (storage slots 0 u, 2 w, 3 om_x, 4 om_z at kx = kz = 0; V = Om_y = 0)
+    T Um = 0, Wm = 0, Oxm = 0, Ozm = 0;
+    if (a.split_mean) {
+        const size_t F = (size_t)a.nz * ny * nxh, yo = (size_t)y * nxh;
+        Um = a.U[yo].x;
+        Wm = a.U[2 * F + yo].x;
+        Oxm = a.U[3 * F + yo].x;
+        Ozm = a.U[4 * F + yo].x;
+    }
+    for (int t = tid; t < RW * M2; t += NT) {
+        const int r = t / M2, k2 = t - r * M2;
+        V *blk = A + r * SROW + S1 * k2;         // field f at blk + f * RW * SROW
+        constexpr int FS = RW * SROW;
+        V u[M1], v[M1], w[M1], o[M1], h1[M1], h2[M1], h3[M1];
+#pragma unroll
+        for (int i = 0; i < M1; i++) {
+            u[i] = blk[i];
+            v[i] = blk[FS + i];
+            w[i] = blk[2 * FS + i];
+        }
+        reg_dft<V, M1, +1, TABN>(u, tw);
+        reg_dft<V, M1, +1, TABN>(v, tw);
+        reg_dft<V, M1, +1, TABN>(w, tw);
+        if (a.do_cfl && r < nr) {
+#pragma unroll
+            for (int i = 0; i < M1; i++) {
+                const T ce = fabs(Um + u[i].x) * dxi + fabs(v[i].x) * dyinv +
+                             fabs(Wm + w[i].x) * dzi;
+                const T co = fabs(Um + u[i].y) * dxi + fabs(v[i].y) * dyinv +
+                             fabs(Wm + w[i].y) * dzi;
+                cflmax = fmax(cflmax, fmax(ce, co));
+            }
+        }
+        // H = u x (om + 2 Omega e_z) (advection.f:52-72).  With the mean
+        // split off, u = Um + u' etc.; the mean x mean part (nonzero only in
+        // the kx = kz = 0 mode of H_2, which the v equation does not use
+        // there) is left out and every remaining term is O(fluctuation):
+        //   H1 = v (om_z + Ozm + 2Om) - (Wm + w) om_y
+        //   H2 = (Wm + w) om_x + w Oxm - (Um + u) om_z - u (Ozm + 2Om)
+        //   H3 = (Um + u) om_y - v (Oxm + om_x)
+        // (all means zero: the plain rotational form).
+        const T zr = Ozm + r2;
+        // om_x
+#pragma unroll
+        for (int i = 0; i < M1; i++)
+            o[i] = blk[3 * FS + i];
+        reg_dft<V, M1, +1, TABN>(o, tw);
+#pragma unroll
+        for (int i = 0; i < M1; i++) {
+            h2[i] = vmk<V>((Wm + w[i].x) * o[i].x + w[i].x * Oxm,
+                           (Wm + w[i].y) * o[i].y + w[i].y * Oxm);
+            h3[i] = vmk<V>(-v[i].x * (Oxm + o[i].x), -v[i].y * (Oxm + o[i].y));
+        }
+        // om_y
+#pragma unroll
+        for (int i = 0; i < M1; i++)
+            o[i] = blk[4 * FS + i];
+        reg_dft<V, M1, +1, TABN>(o, tw);
+#pragma unroll
+        for (int i = 0; i < M1; i++) {
+            h1[i] = vmk<V>(-(Wm + w[i].x) * o[i].x, -(Wm + w[i].y) * o[i].y);
+            h3[i] = vmk<V>(h3[i].x + (Um + u[i].x) * o[i].x, h3[i].y + (Um + u[i].y) * o[i].y);
+        }
+        // om_z + 2 Omega
+#pragma unroll
+        for (int i = 0; i < M1; i++)
+            o[i] = blk[5 * FS + i];
+        reg_dft<V, M1, +1, TABN>(o, tw);
+#pragma unroll
+        for (int i = 0; i < M1; i++) {
+            const T zx = o[i].x + zr, zy = o[i].y + zr;
+            h1[i] = vmk<V>(v[i].x * zx + h1[i].x, v[i].y * zy + h1[i].y);
+            h2[i] = vmk<V>(h2[i].x - (Um * o[i].x + u[i].x * zx),
+                           h2[i].y - (Um * o[i].y + u[i].y * zy));
+        }
+        // forward pass A
+        reg_dft<V, M1, -1, TABN>(h1, tw);
+        reg_dft<V, M1, -1, TABN>(h2, tw);
+        reg_dft<V, M1, -1, TABN>(h3, tw);
+#pragma unroll
+        for (int c = 0; c < M1; c++) {
+            if (c > 0 && k2 > 0) {
+                V wt = tw[2 * k2 * c];
+                wt.y = -wt.y;
+                h1[c] = vmul(h1[c], wt);
+                h2[c] = vmul(h2[c], wt);
+                h3[c] = vmul(h3[c], wt);
+            }
+            blk[c] = h1[c];
+            blk[FS + c] = h2[c];
+            blk[2 * FS + c] = h3[c];
+        }
+    }
+    __syncthreads();
+
+    // forward pass B: three fields, natural order
+    for (int t = tid; t < 3 * RW * M1; t += NT) {
+        const int hr = t / M1, c = t - hr * M1;
+        V *row = A + hr * SROW + c;
+        V v[M2];
+#pragma unroll
+        for (int n2 = 0; n2 < M2; n2++)
+            v[n2] = row[S1 * n2];
+        reg_dft<V, M2, -1, TABN>(v, tw);
+#pragma unroll
+        for (int d = 0; d < M2; d++)
+            row[S1 * d] = v[d];
+    }
+    __syncthreads();
+
+    // packed spectrum -> X_k = E_k + exp(-2 pi i k/nxp) O_k for k < nx/2
+    for (int idx = tid; idx < 3 * RW * nxh; idx += NT) {
+        const int h = idx / (RW * nxh);
+        const int rem = idx - h * RW * nxh;
+        const int r = rem / nxh, k = rem - r * nxh;
+        if (r < nr) {
+            const V *row = A + (h * RW + r) * SROW;
+            const int km = k == 0 ? 0 : M - k;
+            const V Zk = row[k + k / M1];
+            const V Zm = vconj(row[km + km / M1]);
+            const V E = vmk<V>((T)0.5 * (Zk.x + Zm.x), (T)0.5 * (Zk.y + Zm.y));
+            const V dd = vsub(Zk, Zm);
+            const V O = vmk<V>((T)0.5 * dd.y, (T)-0.5 * dd.x);
+            const V X = vadd(E, vmul(vconj(tw[k]), O));
+            a.P[(((size_t)h * ny + y) * nzp + zp0 + r) * nxh + k] = X;
+        }
+    }
+    if (a.do_cfl) {
+        double m = (double)cflmax;
+        for (int off = 32; off > 0; off >>= 1)
+            m = fmax(m, __shfl_xor(m, off));
+        __shared__ double wmax[NT / 64];
+        const int lane = tid & 63, wv = tid >> 6;
+        if (lane == 0)
+            wmax[wv] = m;
+        __syncthreads();
+        if (tid == 0) {
+            double b = 0.;
+            for (int i = 0; i < NT / 64; i++)
+                b = fmax(b, wmax[i]);
+            a.cfl_blk[blockIdx.y * gridDim.x + blockIdx.x] = b;
+        }
+    }
+}
+
+template <typename V, int M>
+void xrow4(const Step2Args<V> &a, const V *tw, hipStream_t s)
+{
+    using B = XBlock<V, M>;
+    dim3 grid((a.nzp + B::RW - 1) / B::RW, a.ny);
+    hipLaunchKernelGGL((k_xrow4<V, M>), grid, dim3(B::NT), B::lds, s, a, tw);
+}
+
+}  // namespace
+
+#define SG_X4LENS(X) X(12) X(24) X(48) X(96) X(192) X(288)
+
+bool has_xrow4(int m)
+{
+#define SG_CASE(N) \
+    if (m == N)    \
+        return true;
+    SG_X4LENS(SG_CASE)
+#undef SG_CASE
+    return false;
+}
+
+template <typename V>
+bool launch_xrow4(const Step2Args<V> &a, const V *wx, hipStream_t s)
+{
+    switch (a.nxp / 2) {
+#define SG_CASE(N)                \
+    case N:                       \
+        xrow4<V, N>(a, wx, s);    \
+        return true;
+        SG_X4LENS(SG_CASE)
+#undef SG_CASE
+    default:
+        return false;
+    }
+}
+
+template <typename V>
+int xrow4_blocks(const Step2Args<V> &a)
+{
+    switch (a.nxp / 2) {
+#define SG_CASE(N)                                                         \
+    case N:                                                                \
+        return ((a.nzp + XBlock<V, N>::RW - 1) / XBlock<V, N>::RW) * a.ny;
+        SG_X4LENS(SG_CASE)
+#undef SG_CASE
+    default:
+        return 0;
+    }
+}
+
+template bool launch_xrow4<double2>(const Step2Args<double2> &, const double2 *, hipStream_t);
+template bool launch_xrow4<float2>(const Step2Args<float2> &, const float2 *, hipStream_t);
+template int xrow4_blocks<double2>(const Step2Args<double2> &);
+template int xrow4_blocks<float2>(const Step2Args<float2> &);
+
+}  // namespace sg

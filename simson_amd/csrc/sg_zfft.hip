@@ -1,0 +1,254 @@
+// z passes of step 2 (bla/nonlinbl.f) as four-step FFTs in LDS.
+//
+//   k_zsyn4  gather u, v, w, om_x, om_z of a y plane, form
+//            om_y = i beta u - i alpha w, pad to nzp modes with the zeroed
+//            dealiasing band (getxz.f:41-83), inverse complex z transform
+//            (fft2db.f:33); with split_mean the kx = kz = 0 mode is left
+//            out (the x-row kernel adds the mean profiles itself);
+//   k_zana4  forward complex z transform (fft2df.f:37) and truncation back
+//            to nz modes, z oddball row included (putxz.f:34-54).
+//
+// A block owns B x columns of one (field, y) plane; each column is one
+// length-N (N = nzp = N1 N2) transform done as two in-place LDS passes of
+// register DFTs, as in the x-row kernel:
+//   inverse: pass 1 (for n1 < N1, N2-point DFT of the stride-N1 column,
+//            twiddle exp(+2 pi i n1 k2 / N)), pass 2 (for k2 < N2, N1-point
+//            DFT of block k2); position N1 k2 + k1 then holds zp = k2 + N2 k1;
+//   forward: the input is loaded into that digit-reversed order, pass A
+//            (N1-point DFT of block a, twiddle exp(-2 pi i a c / N)) and
+//            pass B (N2-point DFT of column c) leave natural order.
+// Columns are padded by one slot per N1 block.
+#include "sg_cplx.h"
+#include "sg_internal.h"
+
+namespace sg {
+
+namespace {
+
+// N = N1 N2 with N1 the smallest divisor >= sqrt(N) (contiguous blocks)
+constexpr int z_n1(int n)
+{
+    int best = n;
+    for (int d = n; d >= 1; d--)
+        if (n % d == 0 && d * d >= n)
+            best = d;
+    return best;
+}
+
+template <typename V, int N>
+struct ZPlan {
+    static constexpr int N1 = z_n1(N);
+    static constexpr int N2 = N / N1;
+    static constexpr int SCOL = N + N / N1;
+    static constexpr int budget = 40 * 1024;
+    static constexpr int per_col = SCOL * (int)sizeof(V);
+    static constexpr int raw = (budget - N * (int)sizeof(V)) / per_col;
+    static constexpr int B = raw >= 16 ? 16 : (raw >= 8 ? 8 : (raw >= 4 ? 4 : (raw >= 2 ? 2 : 1)));
+    static constexpr int tasks = B * (N1 > N2 ? N1 : N2);
+    static constexpr int NT = tasks >= 256 ? 256 : ((tasks + 63) / 64) * 64;
+    static constexpr size_t lds = (size_t)(N + B * SCOL) * sizeof(V);
+};
+
+__device__ __forceinline__ size_t uidx(int slot, int z, int y, int x, int nz, int ny, int nxh)
+{
+    return (((size_t)slot * nz + z) * ny + y) * nxh + x;
+}
+
+// digit-reversed position of zp (zp = k2 + N2 k1 -> N1 k2 + k1), padded
+template <int N1, int N2>
+__device__ __forceinline__ int dr_off(int zp)
+{
+    const int k2 = zp % N2, k1 = zp / N2;
+    return (N1 + 1) * k2 + k1;
+}
+
+template <typename V, int N>
+__global__ void __launch_bounds__((ZPlan<V, N>::NT)) k_zsyn4(Step2Args<V> a, const V *__restrict__ twg)
+{
+    typedef typename ScalarOf<V>::T T;
+    using P = ZPlan<V, N>;
+    constexpr int N1 = P::N1, N2 = P::N2, SCOL = P::SCOL, B = P::B, NT = P::NT;
+    constexpr int S1 = N1 + 1;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    V *tw = reinterpret_cast<V *>(lds_raw);     // exp(+2 pi i t / N)
+    V *A = tw + N;                              // [B][SCOL]
+    const int nz = a.nz, nxh = a.nxh, ny = a.ny, tid = threadIdx.x;
+    const int f = blockIdx.y, y = blockIdx.z, x0 = blockIdx.x * B;
+    const int nb = min(B, nxh - x0);
+    for (int i = tid; i < N; i += NT)
+        tw[i] = twg[i];
+    for (int idx = tid; idx < N * B; idx += NT) {
+        const int zp = idx / B, b = idx - zp * B;
+        V v = vmk<V>(0, 0);
+        // storage row of padded mode zp; rows nz/2 .. N-nz/2 stay zero
+        const int z = zp < nz / 2 ? zp : (zp > N - nz / 2 ? zp - (N - nz) : -1);
+        // with split_mean the kx = kz = 0 mode enters the products directly
+        if (b < nb && z >= 0 && !(a.split_mean && z == 0 && x0 + b == 0)) {
+            const int x = x0 + b;
+            if (f == 4) {
+                const V u = a.U[uidx(0, z, y, x, nz, ny, nxh)];
+                const V w = a.U[uidx(2, z, y, x, nz, ny, nxh)];
+                const T be = (T)a.beta[z], al = (T)a.alfa[x];
+                v = vmk<V>(-be * u.y + al * w.y, be * u.x - al * w.x);
+            } else {
+                v = a.U[uidx(f < 4 ? f : 4, z, y, x, nz, ny, nxh)];
+            }
+        }
+        A[b * SCOL + zp + zp / N1] = v;
+    }
+    __syncthreads();
+    for (int t = tid; t < B * N1; t += NT) {
+        const int b = t / N1, n1 = t - b * N1;
+        V *col = A + b * SCOL + n1;
+        V v[N2];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; n2++)
+            v[n2] = col[S1 * n2];
+        reg_dft<V, N2, +1, N>(v, tw);
+#pragma unroll
+        for (int k2 = 1; k2 < N2; k2++)
+            if (n1 > 0)
+                v[k2] = vmul(v[k2], tw[n1 * k2]);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; k2++)
+            col[S1 * k2] = v[k2];
+    }
+    __syncthreads();
+    for (int t = tid; t < B * N2; t += NT) {
+        const int b = t / N2, k2 = t - b * N2;
+        V *blk = A + b * SCOL + S1 * k2;
+        V v[N1];
+#pragma unroll
+        for (int i = 0; i < N1; i++)
+            v[i] = blk[i];
+        reg_dft<V, N1, +1, N>(v, tw);
+#pragma unroll
+        for (int i = 0; i < N1; i++)
+            blk[i] = v[i];
+    }
+    __syncthreads();
+    V *Pp = a.P + (((size_t)f * ny + y) * N) * nxh + x0;
+    for (int idx = tid; idx < N * B; idx += NT) {
+        const int zp = idx / B, b = idx - zp * B;
+        if (b < nb)
+            Pp[(size_t)zp * nxh + b] = A[b * SCOL + dr_off<N1, N2>(zp)];
+    }
+}
+
+template <typename V, int N>
+__global__ void __launch_bounds__((ZPlan<V, N>::NT)) k_zana4(Step2Args<V> a, const V *__restrict__ twg)
+{
+    using P = ZPlan<V, N>;
+    constexpr int N1 = P::N1, N2 = P::N2, SCOL = P::SCOL, B = P::B, NT = P::NT;
+    constexpr int S1 = N1 + 1;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    V *tw = reinterpret_cast<V *>(lds_raw);
+    V *A = tw + N;
+    const int nz = a.nz, nxh = a.nxh, ny = a.ny, tid = threadIdx.x;
+    const int h = blockIdx.y, y = blockIdx.z, x0 = blockIdx.x * B;
+    const int nb = min(B, nxh - x0);
+    for (int i = tid; i < N; i += NT)
+        tw[i] = twg[i];
+    const V *Pp = a.P + (((size_t)h * ny + y) * N) * nxh + x0;
+    for (int idx = tid; idx < N * B; idx += NT) {
+        const int zp = idx / B, b = idx - zp * B;
+        A[b * SCOL + dr_off<N1, N2>(zp)] = b < nb ? Pp[(size_t)zp * nxh + b] : vmk<V>(0, 0);
+    }
+    __syncthreads();
+    for (int t = tid; t < B * N2; t += NT) {
+        const int b = t / N2, k2 = t - b * N2;
+        V *blk = A + b * SCOL + S1 * k2;
+        V v[N1];
+#pragma unroll
+        for (int i = 0; i < N1; i++)
+            v[i] = blk[i];
+        reg_dft<V, N1, -1, N>(v, tw);
+#pragma unroll
+        for (int c = 0; c < N1; c++) {
+            if (c > 0 && k2 > 0) {
+                V wt = tw[k2 * c];
+                wt.y = -wt.y;
+                v[c] = vmul(v[c], wt);
+            }
+            blk[c] = v[c];
+        }
+    }
+    __syncthreads();
+    for (int t = tid; t < B * N1; t += NT) {
+        const int b = t / N1, c = t - b * N1;
+        V *col = A + b * SCOL + c;
+        V v[N2];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; n2++)
+            v[n2] = col[S1 * n2];
+        reg_dft<V, N2, -1, N>(v, tw);
+#pragma unroll
+        for (int d = 0; d < N2; d++)
+            col[S1 * d] = v[d];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < nz * B; idx += NT) {
+        const int z = idx / B, b = idx - z * B;
+        if (b < nb) {
+            const int zp = z < nz / 2 ? z : z + N - nz;
+            a.Uw[uidx(h, z, y, x0 + b, nz, ny, nxh)] = A[b * SCOL + zp + zp / N1];
+        }
+    }
+}
+
+}  // namespace
+
+#define SG_Z4LENS(X) X(12) X(24) X(48) X(72) X(96) X(144) X(192) X(288) X(384) X(576)
+
+bool has_zfft4(int nzp)
+{
+#define SG_CASE(N)  \
+    if (nzp == N)   \
+        return true;
+    SG_Z4LENS(SG_CASE)
+#undef SG_CASE
+    return false;
+}
+
+template <typename V>
+bool launch_zsyn4(const Step2Args<V> &a, const V *tw, hipStream_t s)
+{
+    switch (a.nzp) {
+#define SG_CASE(N)                                                                       \
+    case N: {                                                                            \
+        using P = ZPlan<V, N>;                                                           \
+        dim3 grid((a.nxh + P::B - 1) / P::B, 6, a.ny);                                   \
+        hipLaunchKernelGGL((k_zsyn4<V, N>), grid, dim3(P::NT), P::lds, s, a, tw);        \
+        return true;                                                                     \
+    }
+        SG_Z4LENS(SG_CASE)
+#undef SG_CASE
+    default:
+        return false;
+    }
+}
+
+template <typename V>
+bool launch_zana4(const Step2Args<V> &a, const V *tw, hipStream_t s)
+{
+    switch (a.nzp) {
+#define SG_CASE(N)                                                                       \
+    case N: {                                                                            \
+        using P = ZPlan<V, N>;                                                           \
+        dim3 grid((a.nxh + P::B - 1) / P::B, 3, a.ny);                                   \
+        hipLaunchKernelGGL((k_zana4<V, N>), grid, dim3(P::NT), P::lds, s, a, tw);        \
+        return true;                                                                     \
+    }
+        SG_Z4LENS(SG_CASE)
+#undef SG_CASE
+    default:
+        return false;
+    }
+}
+
+template bool launch_zsyn4<double2>(const Step2Args<double2> &, const double2 *, hipStream_t);
+template bool launch_zsyn4<float2>(const Step2Args<float2> &, const float2 *, hipStream_t);
+template bool launch_zana4<double2>(const Step2Args<double2> &, const double2 *, hipStream_t);
+template bool launch_zana4<float2>(const Step2Args<float2> &, const float2 *, hipStream_t);
+
+}  // namespace sg

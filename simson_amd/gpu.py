@@ -19,6 +19,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "_lib", "libsimson_gpu.so")
 
 SIMSON_FP64 = 0
+SIMSON_MIXED = 1
 NSLOT = 7
 
 # Every entry point declared in include/simson_gpu.h.
@@ -89,9 +90,13 @@ class BlaGPU:
     """Device-resident bla state for one channel configuration."""
 
     def __init__(self, nx: int, ny: int, nz: int, re: float, xl: float, zl: float,
-                 rot: float = 0.0, cflux: bool = True, retau: float = 0.0):
+                 rot: float = 0.0, cflux: bool = True, retau: float = 0.0,
+                 precision: int = SIMSON_FP64):
+        """precision: SIMSON_FP64, or SIMSON_MIXED (FP32 FFTs, products and
+        Chebyshev transforms; FP64 tau solves, mean flow and partial RHS)."""
         self.nx, self.ny, self.nz = nx, ny, nz
-        p = SimsonParams(nx, ny, nz, SIMSON_FP64, 0, int(cflux), re, xl, zl, rot, retau)
+        self.precision = precision
+        p = SimsonParams(nx, ny, nz, precision, 0, int(cflux), re, xl, zl, rot, retau)
         h = C.c_void_p()
         self._check(lib().simson_gpu_init(C.byref(p), C.byref(h)))
         self.h = h
