@@ -41,7 +41,10 @@ def rel(a, b):
 
 def profiles(u):
     """Mean U(y) and rms u', v', w' profiles from Fourier coefficients
-    u[comp, z, y, kx] (Parseval; kx > 0 modes count twice)."""
+    u[comp, z, y, kx] (Parseval; kx > 0 modes count twice).  The z-oddball
+    plane kz = -nz/2 holds the unnormalised nonlinear term, not velocity
+    (both codes zero it before every physical-space use), and is left out."""
+    u = np.delete(u, u.shape[1] // 2, axis=1)
     mean = u[:, 0, :, 0].real
     w = np.ones(u.shape[3])
     w[1:] = 2.0
