@@ -106,6 +106,14 @@ int simson_gpu_run(simson_gpu_t *g, int nsteps, double dt, int icfl, int first,
 int simson_gpu_download(simson_gpu_t *g, double *ur, double *ui, int nslots,
                         double *puw, double *px);
 
+/* Plane statistics of the amplitude file from the current velocity state:
+ * the per-y sums of boxamp.f for every y plane (the call in nonlinbl.f:430,
+ * which bla makes at the first substep of each iamp-th step), into amp, a
+ * C array [20][ny] whose row i is amp(:, i+1) of boxamp.f; rows 13, 19 and 20
+ * are zero.  The integration over y and the amp.stat line are the host's
+ * (wampbl.f).  Synchronises. */
+int simson_gpu_amp(simson_gpu_t *g, double *amp);
+
 /* Wait for all queued work. */
 int simson_gpu_sync(simson_gpu_t *g);
 

@@ -37,6 +37,7 @@ struct simson_gpu {
     double2 *twz = nullptr, *twx = nullptr;
     double *W = nullptr, *puw = nullptr, *px = nullptr;
     double *cfl_blk = nullptr, *cfl = nullptr;
+    double *amp = nullptr;                // [20][ny] plane statistics
     // tau factorisations: k^2 once, lam^2 for up to three values of a+b
     LamFactor kfac;
     LamFactor lam[3];
@@ -137,7 +138,7 @@ bool make_radices(int n, FftPlan &pl)
 void free_all(simson_gpu_t *g)
 {
     void *ptrs[] = {g->U, g->P, g->C, g->Pp, g->twz, g->twx, g->W, g->puw, g->px,
-                    g->cfl_blk, g->cfl, g->Tf, g->Tp, g->Tb, g->alfa, g->beta, g->deta,
+                    g->cfl_blk, g->cfl, g->amp, g->Tf, g->Tp, g->Tb, g->alfa, g->beta, g->deta,
                     g->kfac.gp, g->lam[0].gp, g->lam[0].vh, g->lam[0].sh,
                     g->lam[1].gp, g->lam[1].vh, g->lam[1].sh,
                     g->lam[2].gp, g->lam[2].vh, g->lam[2].sh, g->U32, g->P32, g->C32,
@@ -457,7 +458,8 @@ int simson_gpu_init(const simson_params_t *p, simson_gpu_t **out)
         dalloc(g, &g->lam[1].sh, ncol) || dalloc(g, &g->lam[2].sh, ncol) ||
         dalloc(g, &g->puw, 2 * (size_t)ny) || dalloc(g, &g->px, 1) ||
         dalloc(g, &g->cfl_blk, (size_t)xblk) ||
-        dalloc(g, &g->cfl, 2) || dalloc(g, &g->Tf, (size_t)ny * ny) ||
+        dalloc(g, &g->cfl, 2) || dalloc(g, &g->amp, 20 * (size_t)ny) ||
+        dalloc(g, &g->Tf, (size_t)ny * ny) ||
         dalloc(g, &g->Tp, (size_t)ny * ny) || dalloc(g, &g->Tb, (size_t)ny * ny) ||
         dalloc(g, &g->alfa, (size_t)nxh) || dalloc(g, &g->beta, (size_t)nz) ||
         dalloc(g, &g->deta, (size_t)ny) || dalloc(g, &g->twz, (size_t)nzp) ||
@@ -753,6 +755,22 @@ int simson_gpu_download(simson_gpu_t *g, double *ur, double *ui, int nslots, dou
                           g->stream));
     if (px)
         CK(hipMemcpyAsync(px, g->px, sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    CK(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int simson_gpu_amp(simson_gpu_t *g, double *amp)
+{
+    if (!g || !amp)
+        return SIMSON_E_ARG;
+    if (g->mixed && g->u64_stale) {
+        launch_widen(g->U32, g->U, kNPhys * field_size(g), g->stream);
+        g->u64_stale = false;
+    }
+    launch_amp(g->U, g->alfa, g->beta, g->nz, g->ny, g->nxh, g->amp, g->stream);
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(amp, g->amp, 20 * (size_t)g->ny * sizeof(double), hipMemcpyDeviceToHost,
+                      g->stream));
     CK(hipStreamSynchronize(g->stream));
     return 0;
 }

@@ -63,7 +63,9 @@ __global__ void __launch_bounds__(256) k_cheb_mfma(const T *__restrict__ A, cons
 {
     typedef typename Mfma<T>::v4 v4;
     __shared__ T Bs[lds_rows<T, KS>() * kLdb];
-    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    // w is uniform over the wave; readfirstlane lets the block tests below
+    // be scalar branches
+    const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63;
     const int ny = io.ny, nc = 2 * io.nxh, nxh = io.nxh;
     const int li = l & 15, lk = l >> 4;
 
@@ -100,13 +102,18 @@ __global__ void __launch_bounds__(256) k_cheb_mfma(const T *__restrict__ A, cons
                 Bs[k * kLdb + n] = (k < ny && n0 + n < nc) ? X[(size_t)k * nc + n0 + n] : (T)0;
             }
         } else {
-            // the tile's columns x0..x0+15 are contiguous: [x][k][re,im]
+            // the tile's columns x0..x0+15 are contiguous: [x][k][re,im];
+            // thread tid walks the element offsets rem = tid, tid + 256 ...
+            // of every column (no division by the runtime length)
             const int len = 2 * ny;
             const T *Xt = X + (size_t)x0 * len;
-            for (int e = tid; e < (kNB / 2) * len; e += 256) {
-                const int xl = e / len, rem = e - xl * len;
+            const int xn = min(kNB / 2, nxh - x0);
+            for (int rem = tid; rem < len; rem += 256) {
                 const int k = rem >> 1, r = rem & 1;
-                Bs[k * kLdb + 2 * xl + r] = (x0 + xl < nxh) ? Xt[e] : (T)0;
+                T *dst = Bs + k * kLdb + r;
+#pragma unroll 4
+                for (int xl = 0; xl < kNB / 2; xl++)
+                    dst[2 * xl] = xl < xn ? Xt[(size_t)xl * len + rem] : (T)0;
             }
             for (int e = tid; e < (4 * KS - ny) * kNB; e += 256) {
                 const int k = ny + e / kNB, n = e % kNB;
@@ -121,16 +128,24 @@ __global__ void __launch_bounds__(256) k_cheb_mfma(const T *__restrict__ A, cons
 #pragma unroll
             for (int nb = 0; nb < 2; nb++)
                 acc[j][nb] = v4{0, 0, 0, 0};
+        if (act[1]) {
 #pragma unroll
-        for (int s = 0; s < KS; s++) {
-            const T *brow = Bs + (4 * s + lk) * kLdb + li;
-            const T b0 = brow[0], b1 = brow[16];
+            for (int s = 0; s < KS; s++) {
+                const T *brow = Bs + (4 * s + lk) * kLdb + li;
+                const T b0 = brow[0], b1 = brow[16];
 #pragma unroll
-            for (int j = 0; j < 2; j++) {
-                if (!act[j])
-                    continue;
-                acc[j][0] = Mfma<T>::op(a[j][s], b0, acc[j][0]);
-                acc[j][1] = Mfma<T>::op(a[j][s], b1, acc[j][1]);
+                for (int j = 0; j < 2; j++) {
+                    acc[j][0] = Mfma<T>::op(a[j][s], b0, acc[j][0]);
+                    acc[j][1] = Mfma<T>::op(a[j][s], b1, acc[j][1]);
+                }
+            }
+        } else if (act[0]) {
+#pragma unroll
+            for (int s = 0; s < KS; s++) {
+                const T *brow = Bs + (4 * s + lk) * kLdb + li;
+                const T b0 = brow[0], b1 = brow[16];
+                acc[0][0] = Mfma<T>::op(a[0][s], b0, acc[0][0]);
+                acc[0][1] = Mfma<T>::op(a[0][s], b1, acc[0][1]);
             }
         }
 
@@ -167,11 +182,14 @@ __global__ void __launch_bounds__(256) k_cheb_mfma(const T *__restrict__ A, cons
         __syncthreads();
         const int len = 2 * ny;
         T *Yt = Y + (size_t)x0 * len;
-        for (int e = tid; e < (kNB / 2) * len; e += 256) {
-            const int xl = e / len, rem = e - xl * len;
+        const int xn = min(kNB / 2, nxh - x0);
+        for (int rem = tid; rem < len; rem += 256) {
             const int k = rem >> 1, r = rem & 1;
-            if (x0 + xl < nxh)
-                Yt[e] = Bs[k * kLdb + 2 * xl + r];
+            const T *src = Bs + k * kLdb + r;
+#pragma unroll 4
+            for (int xl = 0; xl < kNB / 2; xl++)
+                if (xl < xn)
+                    Yt[(size_t)xl * len + rem] = src[2 * xl];
         }
     }
 }

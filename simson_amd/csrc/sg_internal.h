@@ -106,6 +106,9 @@ bool launch_zsyn4(const Step2Args<V> &a, const V *tw, hipStream_t s);
 template <typename V>
 bool launch_zana4(const Step2Args<V> &a, const V *tw, hipStream_t s);
 void launch_cfl_reduce(const double *blk, int n, double *out, hipStream_t s);
+// per-y amplitude sums of boxamp.f into amp[20][ny]
+void launch_amp(const double2 *U, const double *alfa, const double *beta, int nz, int ny, int nxh,
+                double *amp, hipStream_t s);
 
 void launch_cheb_gemm(const double *A, const GemmIO &io, hipStream_t s);
 // MFMA form of the transform for ny <= 128; false when ny is outside it

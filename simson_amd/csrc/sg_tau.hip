@@ -15,6 +15,8 @@
 //   * parity sums (dv/dy at the wall, the mass flux) are wave reductions.
 // The scans run on DPP lane moves within 16-lane rows and row broadcasts or
 // lane reads across rows (sg_wave.h), so no step waits on an LDS permute.
+#include <cstdlib>
+
 #include "sg_internal.h"
 #include "sg_tau.h"
 #include "sg_wave.h"
@@ -26,8 +28,8 @@ namespace {
 constexpr int WV = wv::kWave;
 
 // inclusive suffix sums over the class index of NV channels
-template <int NV, int E>
-__device__ __forceinline__ void suffix_sum(double (&x)[NV][E], int L)
+template <int SW, int NV, int E>
+__device__ __forceinline__ void suffix_sum(double (&x)[NV][E], int l)
 {
 #pragma unroll
     for (int v = 0; v < NV; v++)
@@ -37,9 +39,9 @@ __device__ __forceinline__ void suffix_sum(double (&x)[NV][E], int L)
     double ex[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++)
-        ex[v] = wv::from_next(x[v][0], 0.0);
+        ex[v] = wv::from_next<SW>(x[v][0], 0.0, l);
     double one = 1.0;
-    wv::scan_suffix<NV, false>(ex, one, L);
+    wv::scan_suffix<SW, NV, false>(ex, one, l);
 #pragma unroll
     for (int v = 0; v < NV; v++)
 #pragma unroll
@@ -47,22 +49,23 @@ __device__ __forceinline__ void suffix_sum(double (&x)[NV][E], int L)
             x[v][e] += ex[v];
 }
 
-// sum over the wave, identical on every lane
-template <int NV>
-__device__ __forceinline__ void wave_sum(double (&x)[NV])
+// sum over the segment, identical on every lane of it
+template <int SW, int NV>
+__device__ __forceinline__ void wave_sum(double (&x)[NV], int l)
 {
-    wv::sum_all<NV>(x);
+    wv::sum_all<SW, NV>(x, l);
 }
 
 // d/dy in Chebyshev space of NC channels (dcheb.f:24-36)
-template <int NC, int E>
-__device__ __forceinline__ void dcheb_w(const double (&a)[NC][2][E], double (&b)[NC][2][E], int L)
+template <int SW, int NC, int E>
+__device__ __forceinline__ void dcheb_w(const double (&a)[NC][2][E], double (&b)[NC][2][E], int l)
 {
+    const int L = l & (SW - 1);
     double s[2 * NC][E];
     double nxt[NC];
 #pragma unroll
     for (int q = 0; q < NC; q++)
-        nxt[q] = wv::from_next(a[q][0][0], 0.0);   // next lane's first even element
+        nxt[q] = wv::from_next<SW>(a[q][0][0], 0.0, l);   // next lane's first even element
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const int M = L * E + e;
@@ -74,7 +77,7 @@ __device__ __forceinline__ void dcheb_w(const double (&a)[NC][2][E], double (&b)
             s[NC + q][e] = we * ((e + 1 < E) ? a[q][0][e + 1] : nxt[q]);
         }
     }
-    suffix_sum<2 * NC, E>(s, L);
+    suffix_sum<SW, 2 * NC, E>(s, l);
 #pragma unroll
     for (int e = 0; e < E; e++)
 #pragma unroll
@@ -88,21 +91,45 @@ __device__ __forceinline__ void dcheb_w(const double (&a)[NC][2][E], double (&b)
             b[q][0][0] *= 0.5;
 }
 
+// Row weights of the tau system for the elements of this lane (they depend
+// on k only): wm = c_{k-2}/(4k(k-1)), w0 = [k<=n-3]/(2(k^2-1)),
+// wp = [k<=n-5]/(4k(k+1)), so that R_k = wm f_{k-2} - w0 f_k + wp f_{k+2}
+// and U_k = -lam wp (sg_tau.h); zero outside the rows k = 2..n-1.
+template <int E>
+struct TauW {
+    double wm[2][E], w0[2][E], wp[2][E];
+    __device__ __forceinline__ TauW(int n, int L)
+    {
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int M = L * E + e, k = 2 * M + c;
+                const double kk = (double)k;
+                const bool row = (M >= 1) && (k < n);
+                wm[c][e] = row ? ((k == 2) ? 2.0 : 1.0) / (4.0 * kk * (kk - 1.0)) : 0.0;
+                w0[c][e] = (row && k <= n - 3) ? 1.0 / (2.0 * (kk * kk - 1.0)) : 0.0;
+                wp[c][e] = (row && k <= n - 5) ? 1.0 / (4.0 * kk * (kk + 1.0)) : 0.0;
+            }
+    }
+};
+
 // Tau solve of u'' - lam u = f for NC right-hand sides sharing lam (one
 // column), boundary rows bc[q][class]; gp holds (gamma, 1/pivot).
-template <int NC, int E>
+template <int SW, int NC, int E>
 __device__ __forceinline__ void solve_w(int n, double lam, const double2 (&gp)[2][E],
-                                        const double (&f)[NC][2][E], const double (&bc)[NC][2],
-                                        double (&u)[NC][2][E], int L)
+                                        const TauW<E> &tw, const double (&f)[NC][2][E],
+                                        const double (&bc)[NC][2], double (&u)[NC][2][E], int l)
 {
+    const int L = l & (SW - 1);
     double al[NC][2][E], be[2][E], bq[2][E];
     double fprev[NC][2], fnext[NC][2];
 #pragma unroll
     for (int q = 0; q < NC; q++)
 #pragma unroll
         for (int c = 0; c < 2; c++) {
-            fprev[q][c] = wv::from_prev(f[q][c][E - 1], 0.0);
-            fnext[q][c] = wv::from_next(f[q][c][0], 0.0);
+            fprev[q][c] = wv::from_prev<SW>(f[q][c][E - 1], 0.0, l);
+            fnext[q][c] = wv::from_next<SW>(f[q][c][0], 0.0, l);
         }
 #pragma unroll
     for (int c = 0; c < 2; c++)
@@ -111,13 +138,15 @@ __device__ __forceinline__ void solve_w(int n, double lam, const double2 (&gp)[2
             const int M = L * E + e, k = 2 * M + c;
             const bool row = (M >= 1) && (k < n);
             const double ip = gp[c][e].y;
-            be[c][e] = row ? -tau_up(n, k, lam) * ip : 0.0;
+            // -U_k / piv_k and R_k / piv_k with the row weights of TauW
+            be[c][e] = row ? lam * tw.wp[c][e] * ip : 0.0;
             bq[c][e] = (M == 0) ? 1.0 : (k < n ? -gp[c][e].x : 0.0);
 #pragma unroll
             for (int q = 0; q < NC; q++) {
                 const double fm = e > 0 ? f[q][c][e - 1] : fprev[q][c];
                 const double fp = e + 1 < E ? f[q][c][e + 1] : fnext[q][c];
-                al[q][c][e] = row ? tau_rhs(n, k, fm, f[q][c][e], fp) * ip : 0.0;
+                const double R = tw.wm[c][e] * fm - tw.w0[c][e] * f[q][c][e] + tw.wp[c][e] * fp;
+                al[q][c][e] = row ? R * ip : 0.0;
             }
         }
     // rho_M = al_M + be_M rho_{M+1}: local composition then suffix scan
@@ -142,10 +171,10 @@ __device__ __forceinline__ void solve_w(int n, double lam, const double2 (&gp)[2
     double X[NC][2], Y[2];
 #pragma unroll
     for (int c = 0; c < 2; c++) {
-        Y[c] = wv::from_next(B[c][0], 1.0);
+        Y[c] = wv::from_next<SW>(B[c][0], 1.0, l);
 #pragma unroll
         for (int q = 0; q < NC; q++)
-            X[q][c] = wv::from_next(A[q][c][0], 0.0);
+            X[q][c] = wv::from_next<SW>(A[q][c][0], 0.0, l);
     }
 #pragma unroll
     for (int c = 0; c < 2; c++) {
@@ -153,7 +182,7 @@ __device__ __forceinline__ void solve_w(int n, double lam, const double2 (&gp)[2
 #pragma unroll
         for (int q = 0; q < NC; q++)
             Xc[q] = X[q][c];
-        wv::scan_suffix<NC, true>(Xc, Y[c], L);
+        wv::scan_suffix<SW, NC, true>(Xc, Y[c], l);
 #pragma unroll
         for (int q = 0; q < NC; q++)
             X[q][c] = Xc[q];
@@ -188,12 +217,12 @@ __device__ __forceinline__ void solve_w(int n, double lam, const double2 (&gp)[2
     double Xp[NC][2], Yp[2];
 #pragma unroll
     for (int c = 0; c < 2; c++) {
-        Yp[c] = wv::from_prev(Bp[c][E - 1], 1.0);
+        Yp[c] = wv::from_prev<SW>(Bp[c][E - 1], 1.0, l);
         double Xc[NC];
 #pragma unroll
         for (int q = 0; q < NC; q++)
-            Xc[q] = wv::from_prev(Ap[q][c][E - 1], 0.0);
-        wv::scan_prefix<NC, true>(Xc, Yp[c]);
+            Xc[q] = wv::from_prev<SW>(Ap[q][c][E - 1], 0.0, l);
+        wv::scan_prefix<SW, NC, true>(Xc, Yp[c]);
 #pragma unroll
         for (int q = 0; q < NC; q++)
             Xp[q][c] = Xc[q];
@@ -223,7 +252,7 @@ __device__ __forceinline__ void solve_w(int n, double lam, const double2 (&gp)[2
         }
         sums[2 * NC + c] = sq;
     }
-    wave_sum<(NC + 1) * 2>(sums);
+    wave_sum<SW, (NC + 1) * 2>(sums, l);
 #pragma unroll
     for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -257,19 +286,23 @@ __device__ __forceinline__ void put(float2 *p, double x, double y)
     *p = make_float2((float)x, (float)y);
 }
 
-template <int E, typename CV>
-__global__ void __launch_bounds__(256, E == 1 ? 3 : 1) k_tau_w(ColArgs a, LamFactor lf, LamFactor kf, double an,
-                                               double bn, double anp1, double bnp1)
+// One column per SW-lane segment (64 / SW columns per wave).  A segment
+// without a column of its own (past the end, or the z oddball plane,
+// linearbl.f:227) computes on column 1 and stores nothing, so that every
+// lane of the wave stays active through the lane moves.
+template <int SW, int E, typename CV>
+__global__ void __launch_bounds__(256, (SW == 64 && E == 1) ? 3 : 1)
+    k_tau_w(ColArgs a, LamFactor lf, LamFactor kf, double an, double bn, double anp1, double bnp1)
 {
-    const int L = threadIdx.x & (WV - 1);
-    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int l = threadIdx.x & (WV - 1);
+    const int L = l & (SW - 1);
+    const int tw_ = (blockIdx.x * 4 + (threadIdx.x >> 6)) * (WV / SW) + l / SW;
     const int ncol = a.nz * a.nxh;
-    if (t >= ncol)
-        return;
+    const bool own = tw_ < ncol && tw_ / a.nxh != a.nz / 2;
+    const int t = own ? tw_ : 1;
     const int z = t / a.nxh, x = t - z * a.nxh;
-    if (z == a.nz / 2)                        // linearbl.f:227, z oddball
-        return;
     const int n = a.ny;
+    const TauW<E> tw(n, L);
     const double re = a.re;
     const double alfa = a.alfa[x], beta = a.beta[z];
     const double k2 = alfa * alfa + beta * beta;
@@ -320,7 +353,7 @@ __global__ void __launch_bounds__(256, E == 1 ? 3 : 1) k_tau_w(ColArgs a, LamFac
             g[0][c][e] = -alfa * H1[1][c][e] - beta * H3[1][c][e];
             g[1][c][e] = alfa * H1[0][c][e] + beta * H3[0][c][e];
         }
-    dcheb_w<2, E>(g, dg, L);
+    dcheb_w<SW, 2, E>(g, dg, l);
     // h_v = -(dg/dy + k^2 H2), f_v = p_v - c1 h_v (linearbl.f:345-361)
     double hv[2][2][E], fv[2][2][E];
 #pragma unroll
@@ -340,9 +373,9 @@ __global__ void __launch_bounds__(256, E == 1 ? 3 : 1) k_tau_w(ColArgs a, LamFac
     // Helmholtz solves with lam^2 (linearbl.f:625-626) and v with k^2
     // (linearbl.f:1000-1001); the homogeneous pair phi_h, v_h is precomputed
     double ph[2][2][E], om[2][2][E], v[2][2][E];
-    solve_w<2, E>(n, lam2, gl, fv, bc0, ph, L);
-    solve_w<2, E>(n, lam2, gl, fw, bc0, om, L);
-    solve_w<2, E>(n, k2, gk, ph, bc0, v, L);
+    solve_w<SW, 2, E>(n, lam2, gl, tw, fv, bc0, ph, l);
+    solve_w<SW, 2, E>(n, lam2, gl, tw, fw, bc0, om, l);
+    solve_w<SW, 2, E>(n, k2, gk, tw, ph, bc0, v, l);
     // superpose v_h per parity so that dv/dy = 0 at the walls
     // (linearbl.f:1023-1070): class-c wall slope sum_k k^2 v_k
     {
@@ -355,7 +388,7 @@ __global__ void __launch_bounds__(256, E == 1 ? 3 : 1) k_tau_w(ColArgs a, LamFac
                 s[0 + c] += kk * kk * v[0][c][e];
                 s[2 + c] += kk * kk * v[1][c][e];
             }
-        wave_sum<4>(s);
+        wave_sum<SW, 4>(s, l);
         const double2 sh = lf.sh[t];
         const double shc[2] = {sh.x, sh.y};
 #pragma unroll
@@ -371,9 +404,9 @@ __global__ void __launch_bounds__(256, E == 1 ? 3 : 1) k_tau_w(ColArgs a, LamFac
     }
     // dv, phi = D^2 v - k^2 v, d omega (linearbl.f:1151-1166)
     double dv[2][2][E], d2v[2][2][E], dom[2][2][E];
-    dcheb_w<2, E>(v, dv, L);
-    dcheb_w<2, E>(dv, d2v, L);
-    dcheb_w<2, E>(om, dom, L);
+    dcheb_w<SW, 2, E>(v, dv, l);
+    dcheb_w<SW, 2, E>(dv, d2v, l);
+    dcheb_w<SW, 2, E>(om, dom, l);
 #pragma unroll
     for (int q = 0; q < 2; q++)
 #pragma unroll
@@ -383,8 +416,8 @@ __global__ void __launch_bounds__(256, E == 1 ? 3 : 1) k_tau_w(ColArgs a, LamFac
                 ph[q][c][e] = d2v[q][c][e] - k2 * v[q][c][e];
     // next partial right-hand sides (linearbl.f:1761-1784, tau branch)
     double d1[2][2][E], d2[2][2][E];
-    dcheb_w<2, E>(ph, d1, L);
-    dcheb_w<2, E>(d1, d2, L);
+    dcheb_w<SW, 2, E>(ph, d1, l);
+    dcheb_w<SW, 2, E>(d1, d2, l);
     const double k2m = k2 - c1n;
     CV *Cw = cols<CV>(a) + (size_t)t * n;
     double2 *Pw = a.Pp + (size_t)t * n;
@@ -393,17 +426,17 @@ __global__ void __launch_bounds__(256, E == 1 ? 3 : 1) k_tau_w(ColArgs a, LamFac
 #pragma unroll
         for (int e = 0; e < E; e++) {
             const int k = 2 * (L * E + e) + c;
-            if (k < n)
+            if (own && k < n)
                 Pw[k] = make_double2(-d2[0][c][e] + k2m * ph[0][c][e] - c2 * hv[0][c][e],
                                      -d2[1][c][e] + k2m * ph[1][c][e] - c2 * hv[1][c][e]);
         }
-    dcheb_w<2, E>(dom, d1, L);
+    dcheb_w<SW, 2, E>(dom, d1, l);
 #pragma unroll
     for (int c = 0; c < 2; c++)
 #pragma unroll
         for (int e = 0; e < E; e++) {
             const int k = 2 * (L * E + e) + c;
-            if (k < n)
+            if (own && k < n)
                 Pw[fs + k] = make_double2(-d1[0][c][e] + k2m * om[0][c][e] - c2 * hw[0][c][e],
                                           -d1[1][c][e] + k2m * om[1][c][e] - c2 * hw[1][c][e]);
         }
@@ -414,7 +447,7 @@ __global__ void __launch_bounds__(256, E == 1 ? 3 : 1) k_tau_w(ColArgs a, LamFac
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 const int k = 2 * (L * E + e) + c;
-                if (k >= n)
+                if (!own || k >= n)
                     continue;
                 const double dvr = dv[0][c][e], dvi = dv[1][c][e];
                 const double omr = om[0][c][e], omi = om[1][c][e];
@@ -448,7 +481,7 @@ __global__ void __launch_bounds__(256, E == 1 ? 3 : 1) k_tau_w(ColArgs a, LamFac
         fuw[0][0][0] += 2. * re * px;
     const double bcu[2][2] = {{.5 * (a.u0upp + a.u0low), .5 * (a.u0upp - a.u0low)},
                               {.5 * (a.w0upp + a.w0low), .5 * (a.w0upp - a.w0low)}};
-    solve_w<2, E>(n, lam2, gl, fuw, bcu, uw, L);
+    solve_w<SW, 2, E>(n, lam2, gl, tw, fuw, bcu, uw, l);
     if (a.cflux) {
         auto flux = [&](const double (&u)[2][2][E]) {
             // integral over [-1,1] of sum_k u_k T_k (linearbl.f:1367-1370)
@@ -459,26 +492,26 @@ __global__ void __launch_bounds__(256, E == 1 ? 3 : 1) k_tau_w(ColArgs a, LamFac
                 if (k < n)
                     m[0] -= u[0][0][e] * 2. / (double)(k * k - 1);
             }
-            wave_sum<1>(m);
+            wave_sum<SW, 1>(m, l);
             return m[0];
         };
         const double m1 = flux(uw);
         if (L == 0)
             fuw[0][0][0] = fuw[0][0][0] + 2. * re * px;
-        solve_w<2, E>(n, lam2, gl, fuw, bcu, uw, L);
+        solve_w<SW, 2, E>(n, lam2, gl, tw, fuw, bcu, uw, l);
         const double m2 = flux(uw);
         if (L == 0)
             fuw[0][0][0] = fuw[0][0][0] - 4. * re * px;
         px = px * (1. + (a.mflux - m1) / (m2 - m1));
         if (L == 0)
             fuw[0][0][0] = fuw[0][0][0] + 2. * re * px;
-        solve_w<2, E>(n, lam2, gl, fuw, bcu, uw, L);
+        solve_w<SW, 2, E>(n, lam2, gl, tw, fuw, bcu, uw, l);
         if (L == 0)
             *a.px = px;
     }
     double duw[2][2][E], d2uw[2][2][E];
-    dcheb_w<2, E>(uw, duw, L);
-    dcheb_w<2, E>(duw, d2uw, L);
+    dcheb_w<SW, 2, E>(uw, duw, l);
+    dcheb_w<SW, 2, E>(duw, d2uw, l);
 #pragma unroll
     for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -498,25 +531,52 @@ __global__ void __launch_bounds__(256, E == 1 ? 3 : 1) k_tau_w(ColArgs a, LamFac
 
 }  // namespace
 
+template <int SW, int E, typename CV>
+void launch_seg(const ColArgs &a, LamFactor lf, LamFactor kf, double an, double bn,
+                double anp1, double bnp1, hipStream_t s)
+{
+    const int ncol = a.nz * a.nxh;
+    constexpr int per_block = 4 * (WV / SW);
+    dim3 grid((ncol + per_block - 1) / per_block);
+    hipLaunchKernelGGL((k_tau_w<SW, E, CV>), grid, dim3(256), 0, s, a, lf, kf, an, bn, anp1, bnp1);
+}
+
+// Segment width: the narrowest of 64 or 32 lanes whose elements per lane
+// and parity class stay at most 2, unless SIMSON_TAU_SEG (64 or 32) says
+// otherwise.
+int tau_segment(int ny)
+{
+    static int forced = -1;
+    if (forced < 0) {
+        const char *e = getenv("SIMSON_TAU_SEG");
+        forced = e ? atoi(e) : 0;
+    }
+    const int ce = (ny + 1) / 2;              // elements of the even class
+    if (forced == 64 || forced == 32)
+        return (ce + forced - 1) / forced <= 3 ? forced : 64;
+    return (ce + 31) / 32 <= 2 ? 32 : 64;
+}
+
 template <typename CV>
 int launch_tau_t(const ColArgs &a, LamFactor lf, LamFactor kf, double an, double bn,
                  double anp1, double bnp1, hipStream_t s)
 {
-    const int ncol = a.nz * a.nxh;
-    dim3 grid((ncol + 3) / 4);
-    const int E = (a.ny + 127) / 128;
+    const int ce = (a.ny + 1) / 2;
+    const int SW = tau_segment(a.ny);
+    const int E = (ce + SW - 1) / SW;
+    if (SW == 32) {
+        switch (E) {
+        case 1: launch_seg<32, 1, CV>(a, lf, kf, an, bn, anp1, bnp1, s); return 0;
+        case 2: launch_seg<32, 2, CV>(a, lf, kf, an, bn, anp1, bnp1, s); return 0;
+        case 3: launch_seg<32, 3, CV>(a, lf, kf, an, bn, anp1, bnp1, s); return 0;
+        default: return SIMSON_E_ARG;
+        }
+    }
     switch (E) {
-    case 1:
-        hipLaunchKernelGGL((k_tau_w<1, CV>), grid, dim3(256), 0, s, a, lf, kf, an, bn, anp1, bnp1);
-        return 0;
-    case 2:
-        hipLaunchKernelGGL((k_tau_w<2, CV>), grid, dim3(256), 0, s, a, lf, kf, an, bn, anp1, bnp1);
-        return 0;
-    case 3:
-        hipLaunchKernelGGL((k_tau_w<3, CV>), grid, dim3(256), 0, s, a, lf, kf, an, bn, anp1, bnp1);
-        return 0;
-    default:
-        return SIMSON_E_ARG;
+    case 1: launch_seg<64, 1, CV>(a, lf, kf, an, bn, anp1, bnp1, s); return 0;
+    case 2: launch_seg<64, 2, CV>(a, lf, kf, an, bn, anp1, bnp1, s); return 0;
+    case 3: launch_seg<64, 3, CV>(a, lf, kf, an, bn, anp1, bnp1, s); return 0;
+    default: return SIMSON_E_ARG;
     }
 }
 

@@ -1,6 +1,10 @@
-// Wavefront-level data movement for 64-lane waves on gfx950, built on DPP
+// Lane-segment data movement for 64-lane waves on gfx950, built on DPP
 // (data-parallel primitives: a VALU operand modifier that reads another
 // lane's register in the same instruction) instead of LDS permutes.
+//
+// A wave is split into segments of SW = 16, 32 or 64 lanes (one column of
+// the column solver per segment); every scan, shift and sum below stays
+// inside its segment.
 //
 // DPP controls used (GFX9 encoding):
 //   row_shr:s  0x110+s  lane i reads lane i-s within its 16-lane row
@@ -9,9 +13,9 @@
 //   wave_shr1  0x138    lane i reads lane i-1 across the wave
 //   row_bcast15 0x142   lane 15 of row r is read by row r+1
 //   row_bcast31 0x143   lane 31 is read by rows 2 and 3
-// A lane whose source does not exist, or whose row is masked off, keeps
-// the 'old' operand (bound_ctrl off), which the helpers set to the identity
-// of the combining operation.
+// With bound_ctrl set a lane whose source does not exist reads 0; without
+// it, such a lane (or one in a row masked off) keeps the 'old' operand,
+// which the helpers set to the identity of the combining operation.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,34 +26,67 @@ namespace wv {
 
 constexpr int kWave = 64;
 
+__device__ __forceinline__ long long bits(double x) { return __double_as_longlong(x); }
+__device__ __forceinline__ double dbl(int lo, int hi)
+{
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// lanes without a source keep 'old'
 template <int CTRL, int ROWMASK = 0xf>
 __device__ __forceinline__ double dpp(double old, double x)
 {
-    const long long o = __double_as_longlong(old), v = __double_as_longlong(x);
+    const long long o = bits(old), v = bits(x);
     const int lo = __builtin_amdgcn_update_dpp((int)o, (int)v, CTRL, ROWMASK, 0xf, false);
     const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(v >> 32), CTRL, ROWMASK,
                                                0xf, false);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    return dbl(lo, hi);
+}
+
+// lanes without a source read 0
+template <int CTRL>
+__device__ __forceinline__ double dpp0(double x)
+{
+    const long long v = bits(x);
+    const int lo = __builtin_amdgcn_mov_dpp((int)v, CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(v >> 32), CTRL, 0xf, 0xf, true);
+    return dbl(lo, hi);
 }
 
 __device__ __forceinline__ double readlane(double x, int lane)
 {
-    const long long v = __double_as_longlong(x);
+    const long long v = bits(x);
     const int lo = __builtin_amdgcn_readlane((int)v, lane);
     const int hi = __builtin_amdgcn_readlane((int)(v >> 32), lane);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    return dbl(lo, hi);
 }
 
-// value of lane i+1 (lane 63 gets fill)
-__device__ __forceinline__ double from_next(double x, double fill) { return dpp<0x130>(fill, x); }
-// value of lane i-1 (lane 0 gets fill)
-__device__ __forceinline__ double from_prev(double x, double fill) { return dpp<0x138>(fill, x); }
+// value of segment lane i+1 (the last lane of the segment gets fill);
+// l is the lane index in the wave
+template <int SW>
+__device__ __forceinline__ double from_next(double x, double fill, int l)
+{
+    double r = (fill == 0.0) ? dpp0<0x130>(x) : dpp<0x130>(fill, x);
+    if (SW < kWave && (l & (SW - 1)) == SW - 1)
+        r = fill;
+    return r;
+}
 
-// Inclusive scans of affine maps r -> X[q] + Y r over the lanes.  Each lane
-// holds the map of its segment; after the call lane i holds the composition
-// of lanes 0..i (prefix: earlier lanes applied first, i.e.
-// X_i <- X_i + Y_i X_{i-1}) or of lanes i..63 (suffix: X_i <- X_i + Y_i X_{i+1}).
-// With HASY false, Y is 1 and the scan is a plain sum.
+// value of segment lane i-1 (the first lane of the segment gets fill)
+template <int SW>
+__device__ __forceinline__ double from_prev(double x, double fill, int l)
+{
+    double r = (fill == 0.0) ? dpp0<0x138>(x) : dpp<0x138>(fill, x);
+    if (SW < kWave && (l & (SW - 1)) == 0)
+        r = fill;
+    return r;
+}
+
+// Inclusive scans of affine maps r -> X[q] + Y r over the lanes of each
+// segment.  Each lane holds the map of its part; afterwards lane i holds the
+// composition of segment lanes up to i (prefix: X_i <- X_i + Y_i X_{i-1}) or
+// from i on (suffix: X_i <- X_i + Y_i X_{i+1}).  With HASY false, Y is 1 and
+// the scan is a plain sum.
 template <int NX, bool HASY>
 __device__ __forceinline__ void combine(double (&X)[NX], double &Y, const double (&Xo)[NX],
                                         double Yo)
@@ -61,73 +98,104 @@ __device__ __forceinline__ void combine(double (&X)[NX], double &Y, const double
         Y = Y * Yo;
 }
 
-template <int S, int NX, bool HASY, bool SUFFIX>
-__device__ __forceinline__ void row_step(double (&X)[NX], double &Y)
+template <int CTRL, int NX, bool HASY>
+__device__ __forceinline__ void dpp_step(double (&X)[NX], double &Y)
 {
-    constexpr int ctrl = SUFFIX ? 0x100 + S : 0x110 + S;
     double Xo[NX];
 #pragma unroll
     for (int q = 0; q < NX; q++)
-        Xo[q] = dpp<ctrl>(0.0, X[q]);
-    const double Yo = HASY ? dpp<ctrl>(1.0, Y) : 1.0;
+        Xo[q] = dpp0<CTRL>(X[q]);
+    const double Yo = HASY ? dpp<CTRL>(1.0, Y) : 1.0;
     combine<NX, HASY>(X, Y, Xo, Yo);
 }
 
-template <int NX, bool HASY>
-__device__ __forceinline__ void scan_prefix(double (&X)[NX], double &Y)
+template <int CTRL, int ROWMASK, int NX, bool HASY>
+__device__ __forceinline__ void bcast_step(double (&X)[NX], double &Y)
 {
-    row_step<1, NX, HASY, false>(X, Y);
-    row_step<2, NX, HASY, false>(X, Y);
-    row_step<4, NX, HASY, false>(X, Y);
-    row_step<8, NX, HASY, false>(X, Y);
-    {
-        double Xo[NX];
-#pragma unroll
-        for (int q = 0; q < NX; q++)
-            Xo[q] = dpp<0x142, 0xa>(0.0, X[q]);
-        const double Yo = HASY ? dpp<0x142, 0xa>(1.0, Y) : 1.0;
-        combine<NX, HASY>(X, Y, Xo, Yo);
-    }
-    {
-        double Xo[NX];
-#pragma unroll
-        for (int q = 0; q < NX; q++)
-            Xo[q] = dpp<0x143, 0xc>(0.0, X[q]);
-        const double Yo = HASY ? dpp<0x143, 0xc>(1.0, Y) : 1.0;
-        combine<NX, HASY>(X, Y, Xo, Yo);
-    }
-}
-
-template <int NX, bool HASY>
-__device__ __forceinline__ void scan_suffix(double (&X)[NX], double &Y, int lane)
-{
-    row_step<1, NX, HASY, true>(X, Y);
-    row_step<2, NX, HASY, true>(X, Y);
-    row_step<4, NX, HASY, true>(X, Y);
-    row_step<8, NX, HASY, true>(X, Y);
-    // rows 2, 1, 0 in turn take the total of the rows above them, which
-    // the head lane (16 r) of the next row holds
-#pragma unroll
-    for (int r = 2; r >= 0; r--) {
-        double Xo[NX];
-#pragma unroll
-        for (int q = 0; q < NX; q++)
-            Xo[q] = readlane(X[q], 16 * (r + 1));
-        const double Yo = HASY ? readlane(Y, 16 * (r + 1)) : 1.0;
-        if ((lane >> 4) == r)
-            combine<NX, HASY>(X, Y, Xo, Yo);
-    }
-}
-
-// sum over the wave of NX channels, the same value on every lane
-template <int NX>
-__device__ __forceinline__ void sum_all(double (&X)[NX])
-{
-    double one = 1.0;
-    scan_prefix<NX, false>(X, one);
+    double Xo[NX];
 #pragma unroll
     for (int q = 0; q < NX; q++)
-        X[q] = readlane(X[q], kWave - 1);
+        Xo[q] = dpp<CTRL, ROWMASK>(0.0, X[q]);
+    const double Yo = HASY ? dpp<CTRL, ROWMASK>(1.0, Y) : 1.0;
+    combine<NX, HASY>(X, Y, Xo, Yo);
+}
+
+template <int SW, int NX, bool HASY>
+__device__ __forceinline__ void scan_prefix(double (&X)[NX], double &Y)
+{
+    dpp_step<0x111, NX, HASY>(X, Y);
+    dpp_step<0x112, NX, HASY>(X, Y);
+    dpp_step<0x114, NX, HASY>(X, Y);
+    dpp_step<0x118, NX, HASY>(X, Y);
+    if (SW >= 32)
+        bcast_step<0x142, 0xa, NX, HASY>(X, Y);   // rows 1, 3 take rows 0, 2
+    if (SW >= 64)
+        bcast_step<0x143, 0xc, NX, HASY>(X, Y);   // rows 2, 3 take rows 0-1
+}
+
+template <int SW, int NX, bool HASY>
+__device__ __forceinline__ void scan_suffix(double (&X)[NX], double &Y, int l)
+{
+    dpp_step<0x101, NX, HASY>(X, Y);
+    dpp_step<0x102, NX, HASY>(X, Y);
+    dpp_step<0x104, NX, HASY>(X, Y);
+    dpp_step<0x108, NX, HASY>(X, Y);
+    const int row = l >> 4;
+    if (SW == 32) {
+        // rows 0 and 2 take the totals of rows 1 and 3 (head lanes 16, 48)
+        double Xa[NX], Xb[NX];
+#pragma unroll
+        for (int q = 0; q < NX; q++) {
+            Xa[q] = readlane(X[q], 16);
+            Xb[q] = readlane(X[q], 48);
+        }
+        const double Ya = HASY ? readlane(Y, 16) : 1.0, Yb = HASY ? readlane(Y, 48) : 1.0;
+        if ((row & 1) == 0) {
+            double Xo[NX];
+#pragma unroll
+            for (int q = 0; q < NX; q++)
+                Xo[q] = row == 0 ? Xa[q] : Xb[q];
+            combine<NX, HASY>(X, Y, Xo, row == 0 ? Ya : Yb);
+        }
+    } else if (SW == 64) {
+        // rows 2, 1, 0 in turn take the total of the rows above them
+#pragma unroll
+        for (int r = 2; r >= 0; r--) {
+            double Xo[NX];
+#pragma unroll
+            for (int q = 0; q < NX; q++)
+                Xo[q] = readlane(X[q], 16 * (r + 1));
+            const double Yo = HASY ? readlane(Y, 16 * (r + 1)) : 1.0;
+            if (row == r)
+                combine<NX, HASY>(X, Y, Xo, Yo);
+        }
+    }
+}
+
+// sum over each segment of NX channels, the same value on every lane of
+// the segment
+template <int SW, int NX>
+__device__ __forceinline__ void sum_all(double (&X)[NX], int l)
+{
+    double one = 1.0;
+    dpp_step<0x111, NX, false>(X, one);
+    dpp_step<0x112, NX, false>(X, one);
+    dpp_step<0x114, NX, false>(X, one);
+    dpp_step<0x118, NX, false>(X, one);
+    // lane 16 r + 15 holds the total of row r
+    const int row = l >> 4;
+#pragma unroll
+    for (int q = 0; q < NX; q++) {
+        const double r0 = readlane(X[q], 15), r1 = readlane(X[q], 31);
+        const double r2 = readlane(X[q], 47), r3 = readlane(X[q], 63);
+        if (SW == 64) {
+            X[q] = (r0 + r1) + (r2 + r3);
+        } else if (SW == 32) {
+            X[q] = row < 2 ? r0 + r1 : r2 + r3;
+        } else {
+            X[q] = row == 0 ? r0 : (row == 1 ? r1 : (row == 2 ? r2 : r3));
+        }
+    }
 }
 
 }  // namespace wv

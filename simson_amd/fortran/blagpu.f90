@@ -6,9 +6,11 @@
 ! bla/rparambl.f:160-1860 (temporal channel subset), the .u reader/writer of
 ! bla/rdiscbl.f:93-411 and bla/wdiscbl.f:50-136, the RK weight bookkeeping of
 ! bla/bla.f:1465-1477 and 2384-2397, the CFL stop of bla/wcflbl.f:87-98 and
-! the stop criteria of bla/bla.f:2779-2833 -- and calls the library for
-! prhs (bla.f:1562), the nonlinbl plane loop (bla.f:2251-2288) and the
-! linearbl plane loop (bla.f:2433-2475) through ISO_C_BINDING.
+! the stop criteria of bla/bla.f:2779-2833 and the amp.stat writer of
+! bla/wampbl.f (channel) -- and calls the library for prhs (bla.f:1562),
+! the nonlinbl plane loop (bla.f:2251-2288), its plane statistics
+! (boxamp.f, nonlinbl.f:430) and the linearbl plane loop (bla.f:2433-2475)
+! through ISO_C_BINDING.
 program blagpu
   use iso_c_binding
   use simson_gpu_mod
@@ -16,7 +18,7 @@ program blagpu
 
   integer, parameter :: dp = kind(1.d0)
   real(dp), parameter :: pi = 3.14159265358979323846_dp
-  character(len=256) :: line, namnin, namnut
+  character(len=256) :: line, namnin, namnut, namamp
   character(len=256) :: nmsave(100)
   real(dp) :: tsave(101)
   integer :: version, maxit, nst, ibc, icfl, iamp, iext, ixys, msave
@@ -30,6 +32,7 @@ program blagpu
   real(dp), allocatable :: ur(:,:,:,:), ui(:,:,:,:), urx(:)
   ! time stepping (rkcoeff.f:21-32)
   real(dp) :: anrk(3), bnrk(3), dtn, dtnp1, an, bn, anp1, bnp1, cflp, cfl, px
+  real(dp), allocatable :: amp(:,:), wint(:)
   type(simson_params_t) :: p
   type(c_ptr) :: g
   integer(8) :: c0, c1, crate
@@ -75,7 +78,7 @@ program blagpu
   call nxt(line); read(line, *) icfl
   call nxt(line); read(line, *) iamp
   if (iamp > 0) then
-     call nxt(line)
+     call nxt(line); namamp = word(line)
      call nxt(line); read(line, *) fileurms
   end if
   call nxt(line); read(line, *) longli
@@ -136,6 +139,19 @@ program blagpu
      px = -(retau / re)**2                        ! bla.f:1645
   end if
   call chk(simson_gpu_upload(g, ur, ui, 3_c_int, c_null_ptr, px))
+  ! Clenshaw-Curtis weights (preprbl.f:78-87) for the amplitude file
+  allocate(amp(nyp, 20), wint(nyp))
+  do it = 1, nyp
+     wint(it) = -.5_dp
+     do i = 1, (nyp + 1) / 2 - 2
+        wint(it) = wint(it) + cos(pi * real(2 * i * (it - 1), dp) / real(nyp - 1, dp)) / &
+             real((2 * i)**2 - 1, dp)
+     end do
+     wint(it) = wint(it) + .5_dp * cos(pi * real(it - 1, dp)) / real((nyp - 1)**2 - 1, dp)
+     wint(it) = -4._dp / real(nyp - 1, dp) * wint(it)
+     if (it == 1 .or. it == nyp) wint(it) = wint(it) * .5_dp
+  end do
+  if (iamp > 0) open(15, file=trim(namamp), status='replace')
 
   ! ---------------- time stepping (bla.f:1465-1477, 1790-2885) ----------------
   dtn = dt
@@ -149,6 +165,13 @@ program blagpu
   cfl = 0._dp
   call system_clock(c0, crate)
   do
+     ! plane statistics at the start of every iamp-th step (nonlinbl.f:430)
+     ! and the amp.stat line (bla.f:2326-2331)
+     if (iamp > 0 .and. mod(it - 1 - nst, iamp) == 0) then
+        call chk(simson_gpu_amp(g, amp))
+        call chk(simson_gpu_download(g, ur, ui, 0_c_int, c_null_ptr, px))
+        call write_amp(t, px)
+     end if
      if (icfl > 0 .and. mod(it - 1, icfl) == 0) then
         call chk(simson_gpu_nonlin(g, 1_c_int, cflp))
         cfl = (cflp * pi + 2._dp * abs(rot)) * dtn   ! wcflbl.f:87
@@ -186,6 +209,7 @@ program blagpu
      end if
   end do
   call system_clock(c1)
+  if (iamp > 0) close(15)
   call save_field(trim(namnut))
   write(*,*) 'Average time step    : ', real(c1 - c0, dp) / real(crate, dp) / &
        real((it - 1) / nst, dp)
@@ -227,6 +251,47 @@ contains
     k = index(trim(w), ' ')
     if (k > 0) w = w(1:k-1)
   end function word
+
+  ! wampbl.f, channel flow (fltype 1): y integrals of the plane sums and one
+  ! line of amp.stat.  amp(y, i) holds boxamp.f's amp(y, i).
+  subroutine write_amp(tt, pxx)
+    real(dp), intent(in) :: tt, pxx
+    real(dp) :: sm(20), e0, hplus, tau1, tau2, wb, delta(3), gy
+    integer :: y, j
+    amp(:, 8) = -amp(:, 8)
+    amp(:, 13) = amp(:, 9)
+    sm = 0._dp
+    do j = 1, 10
+       sm(j) = sum(amp(:, j) * wint)
+    end do
+    sm(13) = sum(amp(:, 13) * wint)
+    delta = 0._dp
+    do y = 1, nyp
+       gy = (1._dp + cos(pi * real(y - 1, dp) / real(nyp - 1, dp))) / dstar
+       if (delta(1) == 0._dp .and. amp(y, 13) <= .99_dp) then
+          if (amp(y, 13) == .99_dp .or. y == 1) then
+             delta(1) = gy
+          else
+             delta(1) = gy + (.99_dp - amp(y, 13)) * &
+                  ((1._dp + cos(pi * real(y - 2, dp) / real(nyp - 1, dp))) / dstar - gy) / &
+                  (amp(y - 1, 13) - amp(y, 13))
+          end if
+       end if
+       delta(2) = delta(2) + (1._dp - amp(y, 13)) * wint(y)
+       delta(3) = delta(3) + amp(y, 13) * (1._dp - amp(y, 13)) * wint(y)
+    end do
+    delta(2:3) = delta(2:3) / dstar
+    do j = 1, 7
+       sm(j) = sqrt(sm(j) * .5_dp)
+    end do
+    wb = sm(10) * .5_dp
+    e0 = sqrt((sm(9) + sm(10)) * .5_dp)
+    tau1 = amp(1, 12)
+    tau2 = amp(nyp, 12)
+    hplus = sqrt((abs(tau1) + abs(tau2)) * re / 2._dp)
+    write(15, '(19(e25.16e3,1x))') tt / dstar, (sm(j), j = 1, 3), (sm(j) * dstar, j = 4, 6), &
+         sm(7), sm(8), e0, hplus, pxx, sm(13) * re / 2._dp, tau1, tau2, (delta(j), j = 1, 3), wb
+  end subroutine write_amp
 
   ! wdiscbl.f:50-136 (channel: no shift, no extra header records)
   subroutine save_field(name)

@@ -67,6 +67,12 @@ module simson_gpu_mod
        real(c_double), intent(out) :: px
      end function simson_gpu_download
 
+     integer(c_int) function simson_gpu_amp(g, amp) bind(c, name='simson_gpu_amp')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: g
+       real(c_double), intent(out) :: amp(*)
+     end function simson_gpu_amp
+
      subroutine simson_gpu_finalize(g) bind(c, name='simson_gpu_finalize')
        import :: c_ptr
        type(c_ptr), value :: g

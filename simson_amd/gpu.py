@@ -26,7 +26,7 @@ NSLOT = 7
 EXPORTS = (
     "simson_gpu_init", "simson_gpu_upload", "simson_gpu_prhs", "simson_gpu_nonlin",
     "simson_gpu_linear", "simson_gpu_substep",
-    "simson_gpu_run", "simson_gpu_download", "simson_gpu_sync",
+    "simson_gpu_run", "simson_gpu_download", "simson_gpu_amp", "simson_gpu_sync",
     "simson_gpu_timing_enable", "simson_gpu_timing_read", "simson_gpu_last_error",
     "simson_gpu_finalize",
 )
@@ -71,6 +71,7 @@ def lib():
                                      C.POINTER(C.c_double)]
         L.simson_gpu_download.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                           C.POINTER(C.c_double)]
+        L.simson_gpu_amp.argtypes = [vp, _dp]
         L.simson_gpu_sync.argtypes = [vp]
         L.simson_gpu_timing_enable.argtypes = [vp, C.c_int]
         L.simson_gpu_timing_read.argtypes = [vp, C.POINTER(C.c_char_p),
@@ -189,6 +190,12 @@ class BlaGPU:
     # --- timing ------------------------------------------------------------
     def timing(self, on: bool):
         self._check(lib().simson_gpu_timing_enable(self.h, int(on)))
+
+    def amp(self) -> np.ndarray:
+        """Per-y plane statistics of boxamp.f, [20][ny] (row i = amp(:, i+1))."""
+        a = np.zeros((20, self.ny))
+        self._check(lib().simson_gpu_amp(self.h, a))
+        return a
 
     def timing_read(self) -> dict:
         cap = 32

@@ -14,7 +14,8 @@ Runs in the build container only (it needs /root/reference and amdflang):
    evaluated every substep;
 4. stores the reference outputs in c1_32x33x32.npz: the three fields
    (complex128 [comp, z, y, kx], exactly as read from bla's .u files), the
-   px trajectory, the CFL*dt log lines and the sha256 of the input file.
+   whole amp.stat table (19 columns per full step, wampbl.f:276-278; column
+   12 is px), the CFL*dt log lines and the sha256 of the input file.
 
 The fixture holds data produced by running the reference, not its source.
 """
@@ -73,7 +74,7 @@ def main():
         u100 = ufile.read_u(os.path.join(work, "end.u")).u
         np.savez_compressed(os.path.join(HERE, "c1_32x33x32.npz"),
                             u1=u1, u10=u10, u100=u100,
-                            amp_t=amp[:, 0], amp_px=amp[:, 11],
+                            amp_t=amp[:, 0], amp_px=amp[:, 11], amp=amp,
                             cfl=np.array(cfl), init_sha256=np.array(sha))
         print("wrote", os.path.join(HERE, "c1_32x33x32.npz"), "cfl lines", len(cfl),
               "amp rows", amp.shape)

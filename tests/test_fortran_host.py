@@ -4,8 +4,8 @@ blagpu reads bla.i and the .u start file, drives the RK3 substeps through
 the ISO_C_BINDING interface and writes .u files.  On the GPU it is run on
 the golden 32x33x32 case and its t1.u / t10.u / end.u and CFL log lines are
 compared with what the reference bla wrote for the same inputs, to the
-bounds of the library's own golden test; on the CPU only its input checks
-run (they stop before any device call).
+bounds of the library's own golden tests, and so is its amp.stat file; on
+the CPU only its input checks run (they stop before any device call).
 """
 import math
 import os
@@ -34,8 +34,8 @@ def _c1_dir(**over):
                              kmax=8, rms=1e-3, seed=1)
     ufile.write_u(os.path.join(work, "init.u"), f)
     kw = dict(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=300, dt=0.01, nst=3,
-              xl=4 * math.pi, cflux=True, icfl=3, tsave=[0.01, 0.1],
-              nmsave=["t1.u", "t10.u"])
+              xl=4 * math.pi, cflux=True, icfl=3, iamp=3, namamp="amp.stat",
+              tsave=[0.01, 0.1], nmsave=["t1.u", "t10.u"])
     kw.update(over)
     with open(os.path.join(work, "bla.i"), "w") as fh:
         fh.write(blai.format_channel(blai.BlaParams(**kw)))
@@ -108,5 +108,12 @@ def test_fortran_host_matches_reference_bla(golden_c1):
         cfl = [float(m) for m in re.findall(r"CFL\*dt\s+([0-9.Ee+-]+)", r.stdout)]
         assert len(cfl) == len(golden_c1["cfl"])
         np.testing.assert_allclose(cfl, golden_c1["cfl"], rtol=1e-12)
+        # amp.stat: every line of the reference's, same columns
+        amp = np.loadtxt(os.path.join(work, "amp.stat"))
+        ref = golden_c1["amp"]
+        assert amp.shape == ref.shape
+        np.testing.assert_allclose(amp[:, 0], ref[:, 0], rtol=1e-14)
+        np.testing.assert_allclose(amp[:, 1:18], ref[:, 1:18], rtol=1e-9)
+        np.testing.assert_allclose(amp[:, 18], ref[:, 18], atol=1e-14)
     finally:
         shutil.rmtree(work)
