@@ -114,6 +114,17 @@ int simson_gpu_download(simson_gpu_t *g, double *ur, double *ui, int nslots,
  * (wampbl.f).  Synchronises. */
 int simson_gpu_amp(simson_gpu_t *g, double *amp);
 
+/* Pressure of the current state: nonlinp over all y planes and linearp over
+ * all z planes (bla.f:2486-2527, nonlinp.f, linearp.f with the Neumann tau
+ * solver of setmac.f/crhsc.f/tridc.f), i.e. what bla puts in slot 8 and
+ * wdiscp.f writes to <name>.p.  pr, pi receive p at physical y as C arrays
+ * [z][y][x]; the z-oddball plane holds the transform of u.u there, as in
+ * the reference.  The wall term -(v - v_prev)/dt of linearp.f is zero for the
+ * no-slip walls this library supports.  Needs nzp and nxp/2 among the
+ * specialised FFT lengths.  Allocates its work buffers at the first call;
+ * synchronises. */
+int simson_gpu_pressure(simson_gpu_t *g, double *pr, double *pi);
+
 /* Wait for all queued work. */
 int simson_gpu_sync(simson_gpu_t *g);
 

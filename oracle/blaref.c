@@ -951,3 +951,275 @@ int orc_run(orc_t *o, double *ur, double *ui, double *puw, int nsteps,
     }
     return status;
 }
+
+/* ------------------------------------------------------------------ */
+/* Pressure (bla.f:2486-2527: nonlinp.f then linearp.f)                */
+/* ------------------------------------------------------------------ */
+
+/* Tau solve of u'' - lam u = f with the Neumann boundary rows of
+ * setmac.f (neumann = 1): sum_{k even} k^2 a_k = bc_e,
+ * sum_{k odd} k^2 a_k = bc_o, i.e. du/dy at y = +-1 (T_k'(1) = k^2).  The
+ * interior rows are those of tau_solve (setmac.f:48-87 and crhsc.f build the
+ * same system as setmacrhs.f).  When a class's boundary row vanishes after
+ * elimination (lam = 0, even class: the constant is free) its first
+ * coefficient is set to zero, as tridc.f:127-131. */
+static void tau_solve_neumann(int n, double lam, const double *f, double bce, double bco,
+                              double *a, double *gam, double *rho)
+{
+    for (int par = 0; par < 2; par++) {
+        int k0 = par;
+        int klast = (n - 1 - par) % 2 == 0 ? n - 1 : n - 2;
+        double g_up = 0.0, r_up = 0.0;
+        for (int k = klast; k >= k0 + 2; k -= 2) {
+            double kk = (double)k;
+            double ckm2 = (k - 2 == 0) ? 2.0 : 1.0;
+            double L = -ckm2 * lam / (4.0 * kk * (kk - 1.0));
+            double D = 1.0 + (k <= n - 3 ? lam / (2.0 * (kk * kk - 1.0)) : 0.0);
+            double U = (k <= n - 5) ? -lam / (4.0 * kk * (kk + 1.0)) : 0.0;
+            double R = ckm2 * f[k - 2] / (4.0 * kk * (kk - 1.0));
+            if (k <= n - 3)
+                R -= f[k] / (2.0 * (kk * kk - 1.0));
+            if (k <= n - 5)
+                R += f[k + 2] / (4.0 * kk * (kk + 1.0));
+            double piv = D - U * g_up;
+            gam[k] = L / piv;
+            rho[k] = (R - U * r_up) / piv;
+            g_up = gam[k];
+            r_up = rho[k];
+        }
+        double P = 0.0, Q = 1.0, w0 = (double)(k0 * k0);
+        double sP = 0.0, sQ = w0;
+        for (int k = k0 + 2; k <= klast; k += 2) {
+            P = rho[k] - gam[k] * P;
+            Q = -gam[k] * Q;
+            sP += (double)k * (double)k * P;
+            sQ += (double)k * (double)k * Q;
+        }
+        double bc = par == 0 ? bce : bco;
+        double a0 = sQ == 0.0 ? 0.0 : (bc - sP) / sQ;
+        a[k0] = a0;
+        for (int k = k0 + 2; k <= klast; k += 2)
+            a[k] = rho[k] - gam[k] * a[k - 2];
+    }
+}
+
+void orc_tau_solve_neumann(int n, double lam, const double *f, double bc_e, double bc_o,
+                           double *u)
+{
+    double *g = malloc(sizeof(double) * (size_t)n), *r = malloc(sizeof(double) * (size_t)n);
+    tau_solve_neumann(n, lam, f, bc_e, bc_o, u, g, r);
+    free(g);
+    free(r);
+}
+
+/* nonlinp.f for one y plane: H = u x (om + 2 Omega e_z) and W = |u|^2 in
+ * physical space, forward transforms, then D = i(alpha H1 + beta H3); the
+ * results (unnormalised, times nxp nzp) go to q[3][z][y][x]: D, H2, W
+ * (putxz.f into slots 4, 5, 8 in the reference, z oddball row included). */
+static void nonlinp_plane(orc_t *o, const double *ur, const double *ui, int y, double *qr,
+                          double *qi)
+{
+    int nxh = o->nxh, nz = o->nz, nxp = o->nxp, nzp = o->nzp, hx = nxp / 2 + 1;
+    size_t box = (size_t)nzp * hx;
+    cplx *B = calloc(6 * box, sizeof(cplx));
+    double *R = malloc(sizeof(double) * 6 * (size_t)nzp * nxp);
+    int nmax = nxp > nzp ? nxp : nzp;
+    cplx *t1 = malloc(sizeof(cplx) * (size_t)nmax), *t2 = malloc(sizeof(cplx) * (size_t)nmax);
+    cplx *col = malloc(sizeof(cplx) * (size_t)nzp);
+    static const int slot_of[6] = {0, 1, 2, 3, -1, 4};
+    for (int f = 0; f < 6; f++) {
+        if (slot_of[f] < 0)
+            continue;
+        for (int z = 0; z < nz; z++) {
+            if (z == nz / 2)
+                continue;
+            int zp = zrow(o, z);
+            for (int x = 0; x < nxh; x++) {
+                size_t s = IDX(o, slot_of[f], z, y, x);
+                B[f * box + (size_t)zp * hx + x] = ur[s] + I * ui[s];
+            }
+        }
+    }
+    for (int z = 0; z < nz; z++) {
+        if (z == nz / 2)
+            continue;
+        int zp = zrow(o, z);
+        for (int x = 0; x < nxh; x++) {
+            cplx uu = B[0 * box + (size_t)zp * hx + x], ww = B[2 * box + (size_t)zp * hx + x];
+            B[4 * box + (size_t)zp * hx + x] = I * o->beta[z] * uu - I * o->alfa[x] * ww;
+        }
+    }
+    for (int f = 0; f < 6; f++) {
+        for (int x = 0; x < nxh; x++) {
+            for (int zp = 0; zp < nzp; zp++)
+                col[zp] = B[f * box + (size_t)zp * hx + x];
+            dft_inplace(&o->pz_plan, col, t1, +1);
+            for (int zp = 0; zp < nzp; zp++)
+                B[f * box + (size_t)zp * hx + x] = col[zp];
+        }
+        for (int zp = 0; zp < nzp; zp++)
+            rbwd(&o->px_plan, B + f * box + (size_t)zp * hx,
+                 R + ((size_t)f * nzp + zp) * nxp, t1, t2);
+    }
+    /* nonlinp.f: products and |u|^2 (fields 0-2 H, 3 W) */
+    size_t np = (size_t)nzp * nxp;
+    for (size_t p = 0; p < np; p++) {
+        double u = R[p], v = R[np + p], w = R[2 * np + p];
+        double ox = R[3 * np + p], oy = R[4 * np + p], oz = R[5 * np + p] + 2. * o->rot;
+        R[p] = v * oz - w * oy;
+        R[np + p] = w * ox - u * oz;
+        R[2 * np + p] = u * oy - v * ox;
+        R[3 * np + p] = u * u + v * v + w * w;
+    }
+    cplx *S = malloc(sizeof(cplx) * 4 * (size_t)nz * nxh);   /* [f][z][x] */
+    for (int f = 0; f < 4; f++) {
+        for (int zp = 0; zp < nzp; zp++)
+            rfwd(&o->px_plan, R + ((size_t)f * nzp + zp) * nxp,
+                 B + f * box + (size_t)zp * hx, t1, t2);
+        for (int x = 0; x < nxh; x++) {
+            for (int zp = 0; zp < nzp; zp++)
+                col[zp] = B[f * box + (size_t)zp * hx + x];
+            dft_inplace(&o->pz_plan, col, t1, -1);
+            for (int z = 0; z < nz; z++)
+                S[((size_t)f * nz + z) * nxh + x] = col[zrow(o, z)];
+        }
+    }
+    for (int z = 0; z < nz; z++)
+        for (int x = 0; x < nxh; x++) {
+            cplx h1 = S[((size_t)0 * nz + z) * nxh + x], h3 = S[((size_t)2 * nz + z) * nxh + x];
+            cplx d = I * (o->alfa[x] * h1 + o->beta[z] * h3);
+            cplx h2 = S[((size_t)1 * nz + z) * nxh + x], w = S[((size_t)3 * nz + z) * nxh + x];
+            size_t s0 = IDX(o, 0, z, y, x), s1 = IDX(o, 1, z, y, x), s2 = IDX(o, 2, z, y, x);
+            qr[s0] = creal(d);
+            qi[s0] = cimag(d);
+            qr[s1] = creal(h2);
+            qi[s1] = cimag(h2);
+            qr[s2] = creal(w);
+            qi[s2] = cimag(w);
+        }
+    free(S);
+    free(B);
+    free(R);
+    free(t1);
+    free(t2);
+    free(col);
+}
+
+/* linearp.f for one z plane (not the oddball): Neumann Poisson problem
+ * (D^2 - k^2) phi = dH2/dy + i(alpha H1 + beta H3) with
+ * dphi/dy(+-1) = H2 + (v'' - k^2 v)/Re at the walls (the time-derivative
+ * term of linearp.f vanishes for the no-slip walls of this path), the
+ * k = 0 mode from the integral of H2 with phi(-1) = E(-1) (icheb.f), then
+ * p = phi - E with E = |u|^2 / 2; p at physical y into pr, pi [z][y][x]. */
+static void linearp_plane(orc_t *o, const double *ur, const double *ui, const double *qr,
+                          const double *qi, int z, double *pr, double *pi)
+{
+    int n = o->ny, nxh = o->nxh;
+    double c4 = 2. / ((double)o->nxp * (double)(n - 1) * (double)o->nzp);
+    double cw = 1. / ((double)o->nxp * (double)o->nzp);
+    double *buf = malloc(sizeof(double) * 16 * (size_t)n);
+    double *dr = buf, *di = buf + n, *hr = buf + 2 * n, *hi = buf + 3 * n;
+    double *er = buf + 4 * n, *ei = buf + 5 * n, *fr = buf + 6 * n, *fi = buf + 7 * n;
+    double *pcr = buf + 8 * n, *pci = buf + 9 * n, *g = buf + 10 * n, *r = buf + 11 * n;
+    double *tr = buf + 12 * n, *ti = buf + 13 * n, *vr = buf + 14 * n, *vi = buf + 15 * n;
+    double beta = o->beta[z];
+    for (int x = 0; x < nxh; x++) {
+        double k2 = o->alfa[x] * o->alfa[x] + beta * beta;
+        for (int y = 0; y < n; y++) {
+            tr[y] = qr[IDX(o, 0, z, y, x)];
+            ti[y] = qi[IDX(o, 0, z, y, x)];
+        }
+        chbf_tab(n, o->ct, tr, dr, c4);
+        chbf_tab(n, o->ct, ti, di, c4);
+        double v1r = qr[IDX(o, 1, z, 0, x)] * cw, v1i = qi[IDX(o, 1, z, 0, x)] * cw;
+        double v2r = qr[IDX(o, 1, z, n - 1, x)] * cw, v2i = qi[IDX(o, 1, z, n - 1, x)] * cw;
+        for (int y = 0; y < n; y++) {
+            tr[y] = qr[IDX(o, 1, z, y, x)];
+            ti[y] = qi[IDX(o, 1, z, y, x)];
+        }
+        chbf_tab(n, o->ct, tr, hr, c4);
+        chbf_tab(n, o->ct, ti, hi, c4);
+        double bc0 = qr[IDX(o, 2, z, n - 1, x)] * cw * 0.5;
+        for (int y = 0; y < n; y++) {
+            tr[y] = qr[IDX(o, 2, z, y, x)];
+            ti[y] = qi[IDX(o, 2, z, y, x)];
+        }
+        chbf_tab(n, o->ct, tr, er, 0.5 * c4);
+        chbf_tab(n, o->ct, ti, ei, 0.5 * c4);
+        /* f = dH2/dy + D */
+        dcheb(n, hr, fr);
+        dcheb(n, hi, fi);
+        for (int k = 0; k < n; k++) {
+            fr[k] += dr[k];
+            fi[k] += di[k];
+        }
+        /* v'' at the walls from the Chebyshev coefficients of v
+         * (the weights wd1 of preprbl.f:91-113) */
+        for (int y = 0; y < n; y++) {
+            tr[y] = ur[IDX(o, 1, z, y, x)];
+            ti[y] = ui[IDX(o, 1, z, y, x)];
+        }
+        chbf_tab(n, o->ct, tr, vr, 2. / (double)(n - 1));
+        chbf_tab(n, o->ct, ti, vi, 2. / (double)(n - 1));
+        double ddr1 = 0., ddi1 = 0., ddr2 = 0., ddi2 = 0.;
+        for (int k = 0; k < n; k++) {
+            double kk = (double)k, w2 = kk * kk * (kk * kk - 1.) / 3.;
+            double sg = (k & 1) ? -1. : 1.;
+            ddr1 += w2 * vr[k];
+            ddi1 += w2 * vi[k];
+            ddr2 += sg * w2 * vr[k];
+            ddi2 += sg * w2 * vi[k];
+        }
+        double re = o->re;
+        v1r += (ddr1 - k2 * ur[IDX(o, 1, z, 0, x)]) / re;
+        v1i += (ddi1 - k2 * ui[IDX(o, 1, z, 0, x)]) / re;
+        v2r += (ddr2 - k2 * ur[IDX(o, 1, z, n - 1, x)]) / re;
+        v2i += (ddi2 - k2 * ui[IDX(o, 1, z, n - 1, x)]) / re;
+        tau_solve_neumann(n, k2, fr, 0.5 * (v1r - v2r), 0.5 * (v1r + v2r), pcr, g, r);
+        tau_solve_neumann(n, k2, fi, 0.5 * (v1i - v2i), 0.5 * (v1i + v2i), pci, g, r);
+        if (z == 0 && x == 0) {
+            /* mean pressure: integral of H2 with phi(-1) = E(-1) */
+            hr[n - 1] = 0.;
+            orc_icheb(n, hr, pcr, bc0);
+            for (int k = 1; k + 1 < n; k += 2)
+                pcr[0] += pcr[k] - pcr[k + 1];
+            for (int k = 0; k < n; k++)
+                pci[k] = 0.;
+        }
+        for (int k = 0; k < n; k++) {
+            pcr[k] -= er[k];
+            pci[k] -= ei[k];
+        }
+        chbb_tab(n, o->ct, pcr, tr);
+        chbb_tab(n, o->ct, pci, ti);
+        for (int y = 0; y < n; y++) {
+            pr[((size_t)z * n + y) * nxh + x] = tr[y];
+            pi[((size_t)z * n + y) * nxh + x] = ti[y];
+        }
+    }
+    free(buf);
+}
+
+void orc_pressure(orc_t *o, const double *ur, const double *ui, double *pr, double *pi)
+{
+    size_t F = (size_t)o->nz * o->ny * o->nxh;
+    double *qr = malloc(sizeof(double) * 3 * F), *qi = malloc(sizeof(double) * 3 * F);
+#pragma omp parallel for schedule(dynamic)
+    for (int y = 0; y < o->ny; y++)
+        nonlinp_plane(o, ur, ui, y, qr, qi);
+#pragma omp parallel for schedule(dynamic)
+    for (int z = 0; z < o->nz; z++) {
+        if (z == o->nz / 2) {
+            /* the oddball plane keeps what nonlinp stored in slot 8 */
+            for (int y = 0; y < o->ny; y++)
+                for (int x = 0; x < o->nxh; x++) {
+                    pr[((size_t)z * o->ny + y) * o->nxh + x] = qr[IDX(o, 2, z, y, x)];
+                    pi[((size_t)z * o->ny + y) * o->nxh + x] = qi[IDX(o, 2, z, y, x)];
+                }
+            continue;
+        }
+        linearp_plane(o, ur, ui, qr, qi, z, pr, pi);
+    }
+    free(qr);
+    free(qi);
+}

@@ -52,6 +52,12 @@ void orc_linear(orc_t *o, double *ur, double *ui, double *puw,
 int orc_run(orc_t *o, double *ur, double *ui, double *puw, int nsteps,
             double dt, int icfl, double *cfl_out);
 
+/* Pressure of the current state (bla.f:2486-2527: nonlinp.f over all y
+ * planes, then linearp.f over all z planes), the slot-8 content that
+ * wdiscp.f writes: pr, pi [z][y][x] at physical y.  Slots 0-4 of ur/ui must
+ * hold u, v, w, omega_x, omega_z. */
+void orc_pressure(orc_t *o, const double *ur, const double *ui, double *pr, double *pi);
+
 /* ---- building blocks exposed for unit tests ---- */
 /* complex DFT, y_k = sum_j x_j exp(sign 2 pi i jk/n), in place on (re,im) */
 void orc_cdft(int n, double *re, double *im, int sign);
@@ -69,6 +75,10 @@ void orc_icheb(int n, const double *a, double *b, double intcon);
  * Sum_odd u = bc_o (setmacrhs.f + trid.f tau branch). */
 void orc_tau_solve(int n, double lam, const double *f, double bc_e,
                    double bc_o, double *u);
+/* the same with the Neumann rows Sum_even k^2 u_k = bc_e, Sum_odd k^2 u_k =
+ * bc_o (setmac.f neumann = 1, tridc.f) */
+void orc_tau_solve_neumann(int n, double lam, const double *f, double bc_e,
+                           double bc_o, double *u);
 
 #ifdef __cplusplus
 }

@@ -46,6 +46,9 @@ def lib():
         L.orc_dcheb.argtypes = [C.c_int, _dp, _dp]
         L.orc_icheb.argtypes = [C.c_int, _dp, _dp, C.c_double]
         L.orc_tau_solve.argtypes = [C.c_int, C.c_double, _dp, C.c_double, C.c_double, _dp]
+        L.orc_tau_solve_neumann.argtypes = [C.c_int, C.c_double, _dp, C.c_double, C.c_double,
+                                            _dp]
+        L.orc_pressure.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -97,6 +100,13 @@ class Oracle:
         cfl = C.c_double(0.0)
         st = lib().orc_run(self.h, ur, ui, puw, nsteps, dt, icfl, C.byref(cfl))
         return st, cfl.value
+
+    def pressure(self, ur, ui):
+        """Pressure [z][y][x] (complex) of the state in ur, ui (slots 0-4)."""
+        pr = np.zeros((self.nz, self.ny, self.nx // 2))
+        pi = np.zeros_like(pr)
+        lib().orc_pressure(self.h, ur, ui, pr, pi)
+        return pr + 1j * pi
 
 
 def velocities(ur, ui):

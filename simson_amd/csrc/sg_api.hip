@@ -38,6 +38,11 @@ struct simson_gpu {
     double *W = nullptr, *puw = nullptr, *px = nullptr;
     double *cfl_blk = nullptr, *cfl = nullptr;
     double *amp = nullptr;                // [20][ny] plane statistics
+    // pressure work (allocated at the first pressure call): Q rows
+    // [5][nz][ny][nxh] (H1, H2, H3, W, v), Qc the same as Chebyshev columns,
+    // PR the pressure at physical y, pscr per-column scratch
+    double2 *Q = nullptr, *Qc = nullptr, *PR = nullptr;
+    double *pscr = nullptr;
     // tau factorisations: k^2 once, lam^2 for up to three values of a+b
     LamFactor kfac;
     LamFactor lam[3];
@@ -50,6 +55,7 @@ struct simson_gpu {
     // from U32 before a download (u64_stale) and U32 from U after an upload
     // or prhs (u32_stale)
     bool mixed = false, u32_stale = false, u64_stale = false;
+    bool mean64 = false;              // C holds the FP64 mean-flow column of U32's state
     float2 *U32 = nullptr, *P32 = nullptr, *C32 = nullptr, *twz32 = nullptr, *twx32 = nullptr;
     float *Tf32 = nullptr, *Tb32 = nullptr;
     FftPlan pz, pm;
@@ -142,7 +148,7 @@ void free_all(simson_gpu_t *g)
                     g->kfac.gp, g->lam[0].gp, g->lam[0].vh, g->lam[0].sh,
                     g->lam[1].gp, g->lam[1].vh, g->lam[1].sh,
                     g->lam[2].gp, g->lam[2].vh, g->lam[2].sh, g->U32, g->P32, g->C32,
-                    g->twz32, g->twx32, g->Tf32, g->Tb32};
+                    g->twz32, g->twx32, g->Tf32, g->Tb32, g->Q, g->Qc, g->PR, g->pscr};
     for (void *q : ptrs)
         if (q)
             (void)hipFree(q);
@@ -291,6 +297,19 @@ Step2Args<float2> nonlin_args32(simson_gpu_t *g, int do_cfl)
 
 size_t field_size(const simson_gpu_t *g) { return (size_t)g->nz * g->ny * g->nxh; }
 
+// mixed form: FP64 state from the FP32 one before a download, the
+// statistics, the pressure or prhs; after a linear step the mean-flow
+// column comes from its FP64 Chebyshev coefficients instead
+void refresh_u64(simson_gpu_t *g)
+{
+    if (g->mixed && g->u64_stale) {
+        launch_widen(g->U32, g->U, kNPhys * field_size(g), g->stream);
+        if (g->mean64)
+            launch_mean_synth(g->Tb, g->C, g->U, g->ny, g->nz, g->nxh, g->stream);
+        g->u64_stale = false;
+    }
+}
+
 // mixed form: FP32 state from the FP64 one after an upload or prhs
 void refresh_u32(simson_gpu_t *g)
 {
@@ -313,6 +332,7 @@ int do_nonlin(simson_gpu_t *g, int do_cfl)
             });
         timed(g, F_ZANA, [&] { launch_zana4(a, g->twz32, g->stream); });
         g->u64_stale = true;
+        g->mean64 = false;
         CK(hipGetLastError());
         return 0;
     }
@@ -367,8 +387,10 @@ int do_linear(simson_gpu_t *g, double an, double bn, double anp1, double bnp1)
         else
             launch_cheb_gemm(g->Tb, bw, g->stream);
     });
-    if (g->mixed)
+    if (g->mixed) {
         g->u64_stale = true;
+        g->mean64 = true;
+    }
     CK(hipGetLastError());
     return 0;
 }
@@ -376,10 +398,7 @@ int do_linear(simson_gpu_t *g, double an, double bn, double anp1, double bnp1)
 // The start-up partial RHS runs in FP64 in both forms, on the FP64 fields.
 int do_prhs(simson_gpu_t *g, double an)
 {
-    if (g->mixed && g->u64_stale) {
-        launch_widen(g->U32, g->U, kNPhys * field_size(g), g->stream);
-        g->u64_stale = false;
-    }
+    refresh_u64(g);
     GemmIO fw = gemm_io(g, g->U, 0, g->C, 1, {0, 1, 2}, {0, 1, 2}, -1);
     timed(g, F_PCHF, [&] { launch_cheb_gemm(g->Tp, fw, g->stream); });
     ColArgs ca = col_args(g);
@@ -603,6 +622,7 @@ int simson_gpu_upload(simson_gpu_t *g, const double *ur, const double *ui, int n
     g->w0low = v[3];
     g->u32_stale = g->mixed;
     g->u64_stale = false;
+    g->mean64 = false;
     CK(hipStreamSynchronize(g->stream));
     return 0;
 }
@@ -717,11 +737,8 @@ int simson_gpu_download(simson_gpu_t *g, double *ur, double *ui, int nslots, dou
     if (!g || nslots < 0 || nslots > kNSlot || (nslots > 0 && (!ur || !ui)))
         return SIMSON_E_ARG;
     const size_t F = (size_t)g->nz * g->ny * g->nxh;
-    if (g->mixed && g->u64_stale) {
-        launch_widen(g->U32, g->U, kNPhys * F, g->stream);
-        g->u64_stale = false;
-        CK(hipGetLastError());
-    }
+    refresh_u64(g);
+    CK(hipGetLastError());
     if (nslots > 0) {
         const int nphys = nslots < kNPhys ? nslots : kNPhys;
         std::vector<double2> h(nphys * F);
@@ -763,15 +780,59 @@ int simson_gpu_amp(simson_gpu_t *g, double *amp)
 {
     if (!g || !amp)
         return SIMSON_E_ARG;
-    if (g->mixed && g->u64_stale) {
-        launch_widen(g->U32, g->U, kNPhys * field_size(g), g->stream);
-        g->u64_stale = false;
-    }
+    refresh_u64(g);
     launch_amp(g->U, g->alfa, g->beta, g->nz, g->ny, g->nxh, g->amp, g->stream);
     CK(hipGetLastError());
     CK(hipMemcpyAsync(amp, g->amp, 20 * (size_t)g->ny * sizeof(double), hipMemcpyDeviceToHost,
                       g->stream));
     CK(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int simson_gpu_pressure(simson_gpu_t *g, double *pr, double *pi)
+{
+    if (!g || !pr || !pi)
+        return SIMSON_E_ARG;
+    if (!has_zfft4(g->nzp) || !has_xrow4(g->nxp / 2) || g->ny > pressure_max_ny()) {
+        g->err = "pressure: nzp and nxp/2 need a specialised FFT length";
+        return SIMSON_E_ARG;
+    }
+    const size_t F = field_size(g);
+    const int odd = g->nz / 2;
+    if (!g->Q) {
+        const size_t ncol = (size_t)g->nz * g->nxh;
+        if (dalloc(g, &g->Q, 5 * F) || dalloc(g, &g->Qc, 5 * F) || dalloc(g, &g->PR, F) ||
+            dalloc(g, &g->pscr, 6 * (size_t)g->ny * ncol))
+            return SIMSON_E_HIP;
+    }
+    refresh_u64(g);
+    // nonlinp.f: H and W = u.u of the current state into Q slots 0-3
+    NonlinArgs a = nonlin_args(g, 0);
+    a.Uw = g->Q;
+    launch_zsyn4(a, (const double2 *)g->twz, g->stream);
+    launch_xrow4_press(a, (const double2 *)g->twx, g->stream);
+    launch_zana4(a, (const double2 *)g->twz, g->stream, 4);
+    CK(hipMemcpyAsync(g->Q + 4 * F, g->U + F, F * sizeof(double2), hipMemcpyDeviceToDevice,
+                      g->stream));
+    // Chebyshev analysis, linearp.f per column, synthesis of p
+    GemmIO fw = gemm_io(g, g->Q, 0, g->Qc, 1, {0, 1, 2, 3, 4}, {0, 1, 2, 3, 4}, odd);
+    launch_cheb_gemm(g->Tf, fw, g->stream);
+    launch_pressure_cols(g->Qc, g->kfac.gp, g->alfa, g->beta, g->nz, g->nxh, g->ny, g->p.re,
+                         (double)g->nxp * (double)g->nzp, g->pscr, g->Qc, g->stream);
+    GemmIO bw = gemm_io(g, g->Qc, 1, g->PR, 0, {0}, {0}, odd);
+    launch_cheb_gemm(g->Tb, bw, g->stream);
+    // the z-oddball plane keeps W, as slot 8 does after nonlinp.f
+    const size_t plane = (size_t)g->ny * g->nxh;
+    CK(hipMemcpyAsync(g->PR + (size_t)odd * plane, g->Q + 3 * F + (size_t)odd * plane,
+                      plane * sizeof(double2), hipMemcpyDeviceToDevice, g->stream));
+    CK(hipGetLastError());
+    std::vector<double2> h(F);
+    CK(hipMemcpyAsync(h.data(), g->PR, F * sizeof(double2), hipMemcpyDeviceToHost, g->stream));
+    CK(hipStreamSynchronize(g->stream));
+    for (size_t i = 0; i < F; i++) {
+        pr[i] = h[i].x;
+        pi[i] = h[i].y;
+    }
     return 0;
 }
 

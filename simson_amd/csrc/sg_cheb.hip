@@ -305,6 +305,34 @@ unsigned copy_blocks(size_t n)
 
 }  // namespace
 
+namespace {
+
+// u(y) of the kx = kz = 0 column of five slots from its Chebyshev
+// coefficients (Tb is the synthesis matrix, symmetric)
+__global__ void k_mean_synth(const double *__restrict__ Tb, const double2 *__restrict__ C,
+                             double2 *__restrict__ U, int ny, int nz, int nxh)
+{
+    const int s = blockIdx.x;
+    const double2 *c = C + (size_t)s * nz * nxh * ny;          // column (z 0, x 0)
+    for (int y = threadIdx.x; y < ny; y += blockDim.x) {
+        double re = 0., im = 0.;
+        for (int k = 0; k < ny; k++) {
+            const double t = Tb[(size_t)y * ny + k];
+            re = fma(t, c[k].x, re);
+            im = fma(t, c[k].y, im);
+        }
+        U[((size_t)s * nz * ny + y) * nxh] = make_double2(re, im);
+    }
+}
+
+}  // namespace
+
+void launch_mean_synth(const double *Tb, const double2 *C, double2 *U, int ny, int nz, int nxh,
+                       hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mean_synth, dim3(5), dim3(128), 0, s, Tb, C, U, ny, nz, nxh);
+}
+
 void launch_narrow(const double2 *src, float2 *dst, size_t n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_narrow, dim3(copy_blocks(n)), dim3(256), 0, s, src, dst, n);

@@ -104,8 +104,15 @@ int xrow4_blocks(const Step2Args<V> &a);
 template <typename V>
 bool launch_zsyn4(const Step2Args<V> &a, const V *tw, hipStream_t s);
 template <typename V>
-bool launch_zana4(const Step2Args<V> &a, const V *tw, hipStream_t s);
+bool launch_zana4(const Step2Args<V> &a, const V *tw, hipStream_t s, int nf = 3);
+// the pressure variant of the x-row kernel (H and W = u.u, nonlinp.f)
+bool launch_xrow4_press(const Step2Args<double2> &a, const double2 *wx, hipStream_t s);
 void launch_cfl_reduce(const double *blk, int n, double *out, hipStream_t s);
+// linearp.f per (kx,kz) column (pressure), see sg_press.hip
+int  pressure_max_ny();
+void launch_pressure_cols(const double2 *Qc, const double2 *gk, const double *alfa,
+                          const double *beta, int nz, int nxh, int ny, double re,
+                          double nxpnzp, double *scratch, double2 *out, hipStream_t s);
 // per-y amplitude sums of boxamp.f into amp[20][ny]
 void launch_amp(const double2 *U, const double *alfa, const double *beta, int nz, int ny, int nxh,
                 double *amp, hipStream_t s);
@@ -123,6 +130,9 @@ bool has_xrow4(int m);
 // FP64 <-> FP32 copies of n complex values
 void launch_narrow(const double2 *src, float2 *dst, size_t n, hipStream_t s);
 void launch_widen(const float2 *src, double2 *dst, size_t n, hipStream_t s);
+// the kx = kz = 0 column of slots 0-4 of U from FP64 Chebyshev columns C
+void launch_mean_synth(const double *Tb, const double2 *C, double2 *U, int ny, int nz, int nxh,
+                       hipStream_t s);
 void launch_prhs(const ColArgs &a, double an, hipStream_t s);
 
 // tau factorisations: k^2 once, lam^2 for one value of a+b

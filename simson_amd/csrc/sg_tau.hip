@@ -526,6 +526,16 @@ __global__ void __launch_bounds__(256, (SW == 64 && E == 1) ? 3 : 1)
             put(Cw + 2 * fs + k, uw[1][c][e], 0.);
             put(Cw + 3 * fs + k, duw[1][c][e], 0.);
             put(Cw + 4 * fs + k, -duw[0][c][e], 0.);
+            if constexpr (sizeof(CV) == sizeof(float2)) {
+                // the mixed form also keeps the mean-flow column in FP64, so
+                // that downloads and the pressure see the FP64 mean profiles
+                double2 *C64 = a.C + (size_t)t * n;
+                C64[k] = make_double2(uw[0][c][e], 0.);
+                C64[fs + k] = make_double2(0., 0.);
+                C64[2 * fs + k] = make_double2(uw[1][c][e], 0.);
+                C64[3 * fs + k] = make_double2(duw[1][c][e], 0.);
+                C64[4 * fs + k] = make_double2(-duw[0][c][e], 0.);
+            }
         }
 }
 

@@ -50,7 +50,9 @@ struct XBlock {
     static constexpr size_t lds = (size_t)(XPlan<M>::TABN + 6 * RW * XPlan<M>::SROW) * sizeof(V);
 };
 
-template <typename V, int M>
+// PRESS: the pressure variant (nonlinp.f) also forms W = u.u on the
+// physical row and transforms it back as a fourth output field.
+template <typename V, int M, bool PRESS>
 __global__ void __launch_bounds__((XBlock<V, M>::NT)) k_xrow4(Step2Args<V> a, const V *__restrict__ twg)
 {
     typedef typename ScalarOf<V>::T T;
@@ -141,6 +143,13 @@ __global__ void __launch_bounds__((XBlock<V, M>::NT)) k_xrow4(Step2Args<V> a, co
         reg_dft<V, M1, +1, TABN>(u, tw);
         reg_dft<V, M1, +1, TABN>(v, tw);
         reg_dft<V, M1, +1, TABN>(w, tw);
+        V hw[PRESS ? M1 : 1];
+        if constexpr (PRESS) {
+#pragma unroll
+            for (int i = 0; i < M1; i++)
+                hw[i] = vmk<V>(u[i].x * u[i].x + v[i].x * v[i].x + w[i].x * w[i].x,
+                               u[i].y * u[i].y + v[i].y * v[i].y + w[i].y * w[i].y);
+        }
         if (a.do_cfl && r < nr) {
 #pragma unroll
             for (int i = 0; i < M1; i++) {
@@ -197,6 +206,8 @@ __global__ void __launch_bounds__((XBlock<V, M>::NT)) k_xrow4(Step2Args<V> a, co
         reg_dft<V, M1, -1, TABN>(h1, tw);
         reg_dft<V, M1, -1, TABN>(h2, tw);
         reg_dft<V, M1, -1, TABN>(h3, tw);
+        if constexpr (PRESS)
+            reg_dft<V, M1, -1, TABN>(hw, tw);
 #pragma unroll
         for (int c = 0; c < M1; c++) {
             if (c > 0 && k2 > 0) {
@@ -205,16 +216,21 @@ __global__ void __launch_bounds__((XBlock<V, M>::NT)) k_xrow4(Step2Args<V> a, co
                 h1[c] = vmul(h1[c], wt);
                 h2[c] = vmul(h2[c], wt);
                 h3[c] = vmul(h3[c], wt);
+                if constexpr (PRESS)
+                    hw[c] = vmul(hw[c], wt);
             }
             blk[c] = h1[c];
             blk[FS + c] = h2[c];
             blk[2 * FS + c] = h3[c];
+            if constexpr (PRESS)
+                blk[3 * FS + c] = hw[c];
         }
     }
     __syncthreads();
 
-    // forward pass B: three fields, natural order
-    for (int t = tid; t < 3 * RW * M1; t += NT) {
+    // forward pass B: three (four) fields, natural order
+    constexpr int NOUT = PRESS ? 4 : 3;
+    for (int t = tid; t < NOUT * RW * M1; t += NT) {
         const int hr = t / M1, c = t - hr * M1;
         V *row = A + hr * SROW + c;
         V v[M2];
@@ -229,7 +245,7 @@ __global__ void __launch_bounds__((XBlock<V, M>::NT)) k_xrow4(Step2Args<V> a, co
     __syncthreads();
 
     // packed spectrum -> X_k = E_k + exp(-2 pi i k/nxp) O_k for k < nx/2
-    for (int idx = tid; idx < 3 * RW * nxh; idx += NT) {
+    for (int idx = tid; idx < NOUT * RW * nxh; idx += NT) {
         const int h = idx / (RW * nxh);
         const int rem = idx - h * RW * nxh;
         const int r = rem / nxh, k = rem - r * nxh;
@@ -263,12 +279,12 @@ __global__ void __launch_bounds__((XBlock<V, M>::NT)) k_xrow4(Step2Args<V> a, co
     }
 }
 
-template <typename V, int M>
+template <typename V, int M, bool PRESS = false>
 void xrow4(const Step2Args<V> &a, const V *tw, hipStream_t s)
 {
     using B = XBlock<V, M>;
     dim3 grid((a.nzp + B::RW - 1) / B::RW, a.ny);
-    hipLaunchKernelGGL((k_xrow4<V, M>), grid, dim3(B::NT), B::lds, s, a, tw);
+    hipLaunchKernelGGL((k_xrow4<V, M, PRESS>), grid, dim3(B::NT), B::lds, s, a, tw);
 }
 
 }  // namespace
@@ -311,6 +327,20 @@ int xrow4_blocks(const Step2Args<V> &a)
 #undef SG_CASE
     default:
         return 0;
+    }
+}
+
+bool launch_xrow4_press(const Step2Args<double2> &a, const double2 *wx, hipStream_t s)
+{
+    switch (a.nxp / 2) {
+#define SG_CASE(N)                                 \
+    case N:                                        \
+        xrow4<double2, N, true>(a, wx, s);         \
+        return true;
+        SG_X4LENS(SG_CASE)
+#undef SG_CASE
+    default:
+        return false;
     }
 }
 

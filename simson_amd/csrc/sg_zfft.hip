@@ -229,13 +229,13 @@ bool launch_zsyn4(const Step2Args<V> &a, const V *tw, hipStream_t s)
 }
 
 template <typename V>
-bool launch_zana4(const Step2Args<V> &a, const V *tw, hipStream_t s)
+bool launch_zana4(const Step2Args<V> &a, const V *tw, hipStream_t s, int nf)
 {
     switch (a.nzp) {
 #define SG_CASE(N)                                                                       \
     case N: {                                                                            \
         using P = ZPlan<V, N>;                                                           \
-        dim3 grid((a.nxh + P::B - 1) / P::B, 3, a.ny);                                   \
+        dim3 grid((a.nxh + P::B - 1) / P::B, nf, a.ny);                                  \
         hipLaunchKernelGGL((k_zana4<V, N>), grid, dim3(P::NT), P::lds, s, a, tw);        \
         return true;                                                                     \
     }
@@ -248,7 +248,8 @@ bool launch_zana4(const Step2Args<V> &a, const V *tw, hipStream_t s)
 
 template bool launch_zsyn4<double2>(const Step2Args<double2> &, const double2 *, hipStream_t);
 template bool launch_zsyn4<float2>(const Step2Args<float2> &, const float2 *, hipStream_t);
-template bool launch_zana4<double2>(const Step2Args<double2> &, const double2 *, hipStream_t);
-template bool launch_zana4<float2>(const Step2Args<float2> &, const float2 *, hipStream_t);
+template bool launch_zana4<double2>(const Step2Args<double2> &, const double2 *, hipStream_t,
+                                    int);
+template bool launch_zana4<float2>(const Step2Args<float2> &, const float2 *, hipStream_t, int);
 
 }  // namespace sg

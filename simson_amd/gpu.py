@@ -26,7 +26,8 @@ NSLOT = 7
 EXPORTS = (
     "simson_gpu_init", "simson_gpu_upload", "simson_gpu_prhs", "simson_gpu_nonlin",
     "simson_gpu_linear", "simson_gpu_substep",
-    "simson_gpu_run", "simson_gpu_download", "simson_gpu_amp", "simson_gpu_sync",
+    "simson_gpu_run", "simson_gpu_download", "simson_gpu_amp", "simson_gpu_pressure",
+    "simson_gpu_sync",
     "simson_gpu_timing_enable", "simson_gpu_timing_read", "simson_gpu_last_error",
     "simson_gpu_finalize",
 )
@@ -72,6 +73,7 @@ def lib():
         L.simson_gpu_download.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                           C.POINTER(C.c_double)]
         L.simson_gpu_amp.argtypes = [vp, _dp]
+        L.simson_gpu_pressure.argtypes = [vp, _dp, _dp]
         L.simson_gpu_sync.argtypes = [vp]
         L.simson_gpu_timing_enable.argtypes = [vp, C.c_int]
         L.simson_gpu_timing_read.argtypes = [vp, C.POINTER(C.c_char_p),
@@ -196,6 +198,14 @@ class BlaGPU:
         a = np.zeros((20, self.ny))
         self._check(lib().simson_gpu_amp(self.h, a))
         return a
+
+    def pressure(self) -> np.ndarray:
+        """Pressure of the current state, complex [z][y][kx] at physical y
+        (the slot bla writes to <name>.p)."""
+        pr = np.zeros((self.nz, self.ny, self.nx // 2))
+        pi = np.zeros_like(pr)
+        self._check(lib().simson_gpu_pressure(self.h, pr, pi))
+        return pr + 1j * pi
 
     def timing_read(self) -> dict:
         cap = 32

@@ -2,7 +2,8 @@
 
 Runs in the build container only (it needs /root/reference and amdflang):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # c1_32x33x32.npz
+    python tests/golden/make_golden.py pressure   # p_32x33x32.npz
 
 1. builds the reference `bla` for 32x33x32 (serial, pressure off) with
    oracle/build_ref.sh into oracle/_ref/;
@@ -40,6 +41,7 @@ from simson_amd import blai, fields, ufile  # noqa: E402
 
 NX, NY, NZ = 32, 33, 32
 BIN = os.path.join(REPO, "oracle", "_ref", "bla_32x33x32_serial")
+PBIN = os.path.join(REPO, "oracle", "_ref", "bla_32x33x32_press")
 
 
 def c1_field():
@@ -82,5 +84,40 @@ def main():
         shutil.rmtree(work)
 
 
+def pressure():
+    """Pressure fixture: the same run with bla built with pressure = 1 (par.f),
+    10 steps, saving t1.u/.u.p and t10.u/.u.p; the .u.p files hold the
+    pressure slot written by wdiscp.f (linearp.f after nonlinp.f at the last
+    substep before each save)."""
+    if not os.path.exists(PBIN):
+        subprocess.check_call([os.path.join(REPO, "oracle", "build_ref.sh"),
+                               str(NX), str(NY), str(NZ), "1", "no", "1",
+                               os.path.basename(PBIN)])
+    work = tempfile.mkdtemp(prefix="simson_golden_p.")
+    try:
+        ufile.write_u(os.path.join(work, "init.u"), c1_field())
+        p = blai.BlaParams(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=30,
+                           dt=0.01, nst=3, xl=4 * math.pi, cflux=True, icfl=3,
+                           tsave=[0.01, 0.1], nmsave=["t1.u", "t10.u"])
+        with open(os.path.join(work, "bla.i"), "w") as f:
+            f.write(blai.format_channel(p))
+        with open(os.path.join(work, "out.txt"), "w") as out:
+            subprocess.check_call([PBIN], cwd=work, stdout=out, stderr=subprocess.STDOUT)
+        log = open(os.path.join(work, "out.txt")).read()
+        if "bla terminated normally" not in log:
+            raise RuntimeError("reference bla (pressure) did not terminate normally")
+        res = {}
+        for n in ("1", "10"):
+            res["u" + n] = ufile.read_u(os.path.join(work, f"t{n}.u")).u
+            res["p" + n] = ufile.read_u(os.path.join(work, f"t{n}.u.p"), ncomp=1).u[0]
+        np.savez_compressed(os.path.join(HERE, "p_32x33x32.npz"), **res)
+        print("wrote", os.path.join(HERE, "p_32x33x32.npz"))
+    finally:
+        shutil.rmtree(work)
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "pressure":
+        pressure()
+    else:
+        main()
