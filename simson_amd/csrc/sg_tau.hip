@@ -551,9 +551,10 @@ void launch_seg(const ColArgs &a, LamFactor lf, LamFactor kf, double an, double 
     hipLaunchKernelGGL((k_tau_w<SW, E, CV>), grid, dim3(256), 0, s, a, lf, kf, an, bn, anp1, bnp1);
 }
 
-// Segment width: the narrowest of 64 or 32 lanes whose elements per lane
-// and parity class stay at most 2, unless SIMSON_TAU_SEG (64 or 32) says
-// otherwise.
+// Segment width: 64 lanes (one column per wave, three waves per SIMD at
+// ny <= 127), unless SIMSON_TAU_SEG=32 asks for two columns per wave (fewer
+// lane moves per column but one wave per SIMD: slower at 128x97x128,
+// 91 against 62 us per substep).
 int tau_segment(int ny)
 {
     static int forced = -1;
@@ -562,9 +563,9 @@ int tau_segment(int ny)
         forced = e ? atoi(e) : 0;
     }
     const int ce = (ny + 1) / 2;              // elements of the even class
-    if (forced == 64 || forced == 32)
-        return (ce + forced - 1) / forced <= 3 ? forced : 64;
-    return (ce + 31) / 32 <= 2 ? 32 : 64;
+    if (forced == 32 && (ce + 31) / 32 <= 3)
+        return 32;
+    return 64;
 }
 
 template <typename CV>
