@@ -64,8 +64,10 @@ def alg_per_launch(mixed: bool):
         "xrow": ("hbm", (6 * Fp + 3 * Fp) * b),
         # read three products, write H into three storage slots
         "zana": ("hbm", (3 * Fp + 3 * F) * b),
-        # read H (3 fields), write u,v,w,om_x,om_z; read and write p_v, p_om (FP64)
-        "tau": ("hbm", (3 * F3 + 5 * F3) * b + 4 * F3 * 16),
+        # read H (3 fields), write u,v,w,om_x,om_z; read and write p_v, p_om
+        # (FP64); read the per-column tau factors (gamma, 1/pivot) of lam^2 and
+        # k^2 and the homogeneous solution v_h (40 B per element)
+        "tau": ("hbm", (3 * F3 + 5 * F3) * b + 4 * F3 * 16 + F3 * 40),
         # dense ny x ny transform of 3 (fwd) / 5 (bwd) complex fields
         "cheb_fwd": ("flops", 2.0 * NY * NY * 2 * nxh * (NZ - 1) * 3),
         "cheb_bwd": ("flops", 2.0 * NY * NY * 2 * nxh * (NZ - 1) * 5),

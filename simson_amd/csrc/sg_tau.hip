@@ -91,37 +91,34 @@ __device__ __forceinline__ void dcheb_w(const double (&a)[NC][2][E], double (&b)
             b[q][0][0] *= 0.5;
 }
 
-// Row weights of the tau system for the elements of this lane (they depend
-// on k only): wm = c_{k-2}/(4k(k-1)), w0 = [k<=n-3]/(2(k^2-1)),
+// Row weights of the tau system (they depend on k only), one table per
+// block in LDS: wm = c_{k-2}/(4k(k-1)), w0 = [k<=n-3]/(2(k^2-1)),
 // wp = [k<=n-5]/(4k(k+1)), so that R_k = wm f_{k-2} - w0 f_k + wp f_{k+2}
 // and U_k = -lam wp (sg_tau.h); zero outside the rows k = 2..n-1.
-template <int E>
-struct TauW {
-    double wm[2][E], w0[2][E], wp[2][E];
-    __device__ __forceinline__ TauW(int n, int L)
+template <int KT>
+struct TauTab {
+    double wm[KT], w0[KT], wp[KT];
+    __device__ __forceinline__ void fill(int n)
     {
-#pragma unroll
-        for (int c = 0; c < 2; c++)
-#pragma unroll
-            for (int e = 0; e < E; e++) {
-                const int M = L * E + e, k = 2 * M + c;
-                const double kk = (double)k;
-                const bool row = (M >= 1) && (k < n);
-                wm[c][e] = row ? ((k == 2) ? 2.0 : 1.0) / (4.0 * kk * (kk - 1.0)) : 0.0;
-                w0[c][e] = (row && k <= n - 3) ? 1.0 / (2.0 * (kk * kk - 1.0)) : 0.0;
-                wp[c][e] = (row && k <= n - 5) ? 1.0 / (4.0 * kk * (kk + 1.0)) : 0.0;
-            }
+        for (int k = threadIdx.x; k < KT; k += blockDim.x) {
+            const double kk = (double)k;
+            const bool row = k >= 2 && k < n;
+            wm[k] = row ? ((k == 2) ? 2.0 : 1.0) / (4.0 * kk * (kk - 1.0)) : 0.0;
+            w0[k] = (row && k <= n - 3) ? 1.0 / (2.0 * (kk * kk - 1.0)) : 0.0;
+            wp[k] = (row && k <= n - 5) ? 1.0 / (4.0 * kk * (kk + 1.0)) : 0.0;
+        }
     }
 };
 
-// Tau solve of u'' - lam u = f for NC right-hand sides sharing lam (one
-// column), boundary rows bc[q][class]; gp holds (gamma, 1/pivot).
-template <int SW, int NC, int E>
+// Tau solve of u'' - lam u = f, in place, for NC real channels sharing lam
+// (one column), boundary rows bc[q][class]; gp holds (gamma, 1/pivot).
+template <int SW, int NC, int E, int KT>
 __device__ __forceinline__ void solve_w(int n, double lam, const double2 (&gp)[2][E],
-                                        const TauW<E> &tw, const double (&f)[NC][2][E],
-                                        const double (&bc)[NC][2], double (&u)[NC][2][E], int l)
+                                        const TauTab<KT> &tw, double (&f)[NC][2][E],
+                                        const double (&bc)[NC][2], int l)
 {
     const int L = l & (SW - 1);
+    double (&u)[NC][2][E] = f;
     double al[NC][2][E], be[2][E], bq[2][E];
     double fprev[NC][2], fnext[NC][2];
 #pragma unroll
@@ -138,14 +135,15 @@ __device__ __forceinline__ void solve_w(int n, double lam, const double2 (&gp)[2
             const int M = L * E + e, k = 2 * M + c;
             const bool row = (M >= 1) && (k < n);
             const double ip = gp[c][e].y;
-            // -U_k / piv_k and R_k / piv_k with the row weights of TauW
-            be[c][e] = row ? lam * tw.wp[c][e] * ip : 0.0;
+            const double wm = tw.wm[k], w0 = tw.w0[k], wp = tw.wp[k];
+            // -U_k / piv_k and R_k / piv_k with the row weights
+            be[c][e] = row ? lam * wp * ip : 0.0;
             bq[c][e] = (M == 0) ? 1.0 : (k < n ? -gp[c][e].x : 0.0);
 #pragma unroll
             for (int q = 0; q < NC; q++) {
                 const double fm = e > 0 ? f[q][c][e - 1] : fprev[q][c];
                 const double fp = e + 1 < E ? f[q][c][e + 1] : fnext[q][c];
-                const double R = tw.wm[c][e] * fm - tw.w0[c][e] * f[q][c][e] + tw.wp[c][e] * fp;
+                const double R = wm * fm - w0 * f[q][c][e] + wp * fp;
                 al[q][c][e] = row ? R * ip : 0.0;
             }
         }
@@ -286,23 +284,144 @@ __device__ __forceinline__ void put(float2 *p, double x, double y)
     *p = make_float2((float)x, (float)y);
 }
 
+__device__ __forceinline__ double2 widen(double v) { return make_double2(v, 0.); }
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Column I/O of one wave in the lane order of the scans (lane L of a
+// segment holds elements k = 2(L E + e) + c).  With 16-lane segments the
+// four columns of a wave, t0..t0+3, are contiguous in memory; they pass
+// through a per-wave LDS buffer so that memory is read and written with
+// consecutive lanes on consecutive elements (reading the blocked order
+// straight from memory would touch a new cache line on every lane).  The
+// buffer holds segment s at s*SEGP, element k at k + k/(2E) (one pad slot
+// per lane block against bank conflicts).  Wider segments read directly.
+template <int SW, int E>
+struct ColIO {
+    static constexpr int SEGP = 16 * (2 * E + 1);
+    static constexpr int NBUF = SW == 16 ? 4 * SEGP : 1;
+    double2 *buf;
+    size_t t0n, tn;      // element offsets of column t0 (first of the wave) and t
+    int n, l, L, nseg;   // nseg: segments with a column (t0 + s < ncol)
+    unsigned ownmask;    // segments whose column is stored
+    bool own;
+
+    template <typename T>
+    __device__ __forceinline__ void load(const T *f, double2 (&v)[2][E]) const
+    {
+        if constexpr (SW == 16) {
+            for (int s = 0; s < nseg; s++)
+                for (int k = l; k < n; k += 64)
+                    buf[s * SEGP + k + k / (2 * E)] = widen(f[t0n + (size_t)s * n + k]);
+            wave_sync();
+            const int s = l >> 4;
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    const int k = 2 * (L * E + e) + c;
+                    v[c][e] = (k < n && s < nseg) ? buf[s * SEGP + L * (2 * E + 1) + 2 * e + c]
+                                                  : make_double2(0., 0.);
+                }
+            wave_sync();
+        } else {
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    const int k = 2 * (L * E + e) + c;
+                    v[c][e] = k < n ? widen(f[tn + k]) : make_double2(0., 0.);
+                }
+        }
+    }
+
+    template <typename T>
+    __device__ __forceinline__ void store(T *f, const double2 (&v)[2][E]) const
+    {
+        if constexpr (SW == 16) {
+            const int s = l >> 4;
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int e = 0; e < E; e++)
+                    buf[s * SEGP + L * (2 * E + 1) + 2 * e + c] = v[c][e];
+            wave_sync();
+            for (int s2 = 0; s2 < nseg; s2++)
+                if (ownmask & (1u << s2))
+                    for (int k = l; k < n; k += 64) {
+                        const double2 x = buf[s2 * SEGP + k + k / (2 * E)];
+                        put(f + t0n + (size_t)s2 * n + k, x.x, x.y);
+                    }
+            wave_sync();
+        } else {
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    const int k = 2 * (L * E + e) + c;
+                    if (own && k < n)
+                        put(f + tn + k, v[c][e].x, v[c][e].y);
+                }
+        }
+    }
+};
+
 // One column per SW-lane segment (64 / SW columns per wave).  A segment
-// without a column of its own (past the end, or the z oddball plane,
-// linearbl.f:227) computes on column 1 and stores nothing, so that every
+// without a column of its own (the z oddball plane, linearbl.f:227, or past
+// the end) computes on a valid column and stores nothing, so that every
 // lane of the wave stays active through the lane moves.
+//
+// Register use is kept down by loading each input just before it is needed.
+// With 16-lane segments (four elements per class and lane) the -c2 h_v,
+// -c2 h_omega parts of the next partial right-hand sides also go to their
+// output slots early and are read back and completed at the end by the same
+// lanes that wrote them; wider segments keep them in registers, which saves
+// that extra memory traffic.
+template <int SW, int E>
+constexpr int tau_waves_per_eu()
+{
+    return E == 1 ? 3 : 2;
+}
+
 template <int SW, int E, typename CV>
-__global__ void __launch_bounds__(256, (SW == 64 && E == 1) ? 3 : 1)
+__global__ void __launch_bounds__(256, (tau_waves_per_eu<SW, E>()))
     k_tau_w(ColArgs a, LamFactor lf, LamFactor kf, double an, double bn, double anp1, double bnp1)
 {
+    constexpr int KT = 2 * SW * E;
+    using IO = ColIO<SW, E>;
+    __shared__ TauTab<KT> tw;
+    __shared__ double2 sbuf[4][IO::NBUF];
+    tw.fill(a.ny);
+    __syncthreads();
     const int l = threadIdx.x & (WV - 1);
     const int L = l & (SW - 1);
-    const int tw_ = (blockIdx.x * 4 + (threadIdx.x >> 6)) * (WV / SW) + l / SW;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int t0 = wave * (WV / SW);
+    const int tw_ = t0 + l / SW;
     const int ncol = a.nz * a.nxh;
-    const bool own = tw_ < ncol && tw_ / a.nxh != a.nz / 2;
-    const int t = own ? tw_ : 1;
+    auto owns = [&](int tt) { return tt < ncol && tt / a.nxh != a.nz / 2; };
+    const bool own = owns(tw_);
+    const int t = SW == 16 ? min(tw_, ncol - 1) : (own ? tw_ : 1);
     const int z = t / a.nxh, x = t - z * a.nxh;
     const int n = a.ny;
-    const TauW<E> tw(n, L);
+    IO io;
+    io.buf = sbuf[threadIdx.x >> 6];
+    io.t0n = (size_t)t0 * n;
+    io.tn = (size_t)t * n;
+    io.n = n;
+    io.l = l;
+    io.L = L;
+    io.nseg = min(WV / SW, ncol - t0);
+    io.ownmask = 0;
+    for (int s = 0; s < WV / SW; s++)
+        if (owns(t0 + s))
+            io.ownmask |= 1u << s;
+    io.own = own;
     const double re = a.re;
     const double alfa = a.alfa[x], beta = a.beta[z];
     const double k2 = alfa * alfa + beta * beta;
@@ -313,69 +432,93 @@ __global__ void __launch_bounds__(256, (SW == 64 && E == 1) ? 3 : 1)
     const double c1n = 2. * re / (anp1 + bnp1);
     const double c2 = 2. * re * bnp1 / (anp1 + bnp1);
     const size_t fs = (size_t)ncol * n;
-    const CV *Cc = cols<CV>(a) + (size_t)t * n;
-    const double2 *Pc = a.Pp + (size_t)t * n;
-    const double2 *glp = lf.gp + (size_t)t * n;
-    const double2 *gkp = kf.gp + (size_t)t * n;
-    const double *vhp = lf.vh + (size_t)t * n;
+    const CV *C0 = cols<CV>(a);
+    double2 *P0 = a.Pp;
+    auto kidx = [&](int c, int e) { return 2 * (L * E + e) + c; };
+    double2 tmp[2][E];
 
-    double H1[2][2][E], H2[2][2][E], H3[2][2][E], pv[2][2][E], pw[2][2][E], vh[2][E];
-    double2 gl[2][E], gk[2][E];
-#pragma unroll
-    for (int c = 0; c < 2; c++)
-#pragma unroll
-        for (int e = 0; e < E; e++) {
-            const int k = 2 * (L * E + e) + c;
-            const bool ok = k < n;
-            const double2 z2 = make_double2(0., 0.);
-            const double2 h1 = ok ? widen(Cc[k]) : z2, h2 = ok ? widen(Cc[fs + k]) : z2;
-            const double2 h3 = ok ? widen(Cc[2 * fs + k]) : z2;
-            const double2 p0 = ok ? Pc[k] : z2, p1 = ok ? Pc[fs + k] : z2;
-            H1[0][c][e] = h1.x; H1[1][c][e] = h1.y;
-            H2[0][c][e] = h2.x; H2[1][c][e] = h2.y;
-            H3[0][c][e] = h3.x; H3[1][c][e] = h3.y;
-            pv[0][c][e] = p0.x; pv[1][c][e] = p0.y;
-            pw[0][c][e] = p1.x; pw[1][c][e] = p1.y;
-            gl[c][e] = ok ? glp[k] : z2;
-            gk[c][e] = ok ? gkp[k] : z2;
-            vh[c][e] = ok ? vhp[k] : 0.;
-        }
-    // h_omega, f_omega, g = i alpha H1 + i beta H3 (linearbl.f:325-343)
-    double hw[2][2][E], fw[2][2][E], g[2][2][E], dg[2][2][E];
-#pragma unroll
-    for (int c = 0; c < 2; c++)
-#pragma unroll
-        for (int e = 0; e < E; e++) {
-            hw[0][c][e] = -beta * H1[1][c][e] + alfa * H3[1][c][e];
-            hw[1][c][e] = beta * H1[0][c][e] - alfa * H3[0][c][e];
-            fw[0][c][e] = pw[0][c][e] - c1 * hw[0][c][e];
-            fw[1][c][e] = pw[1][c][e] - c1 * hw[1][c][e];
-            g[0][c][e] = -alfa * H1[1][c][e] - beta * H3[1][c][e];
-            g[1][c][e] = alfa * H1[0][c][e] + beta * H3[0][c][e];
-        }
-    dcheb_w<SW, 2, E>(g, dg, l);
-    // h_v = -(dg/dy + k^2 H2), f_v = p_v - c1 h_v (linearbl.f:345-361)
-    double hv[2][2][E], fv[2][2][E];
-#pragma unroll
-    for (int q = 0; q < 2; q++)
+    // h_omega and g = i alpha H1 + i beta H3 (linearbl.f:325-343)
+    double hw[2][2][E], dg[2][2][E];
+    {
+        double g[2][2][E];
+        io.load(C0, tmp);                           // H1
 #pragma unroll
         for (int c = 0; c < 2; c++)
 #pragma unroll
             for (int e = 0; e < E; e++) {
-                hv[q][c][e] = -(dg[q][c][e] + k2 * H2[q][c][e]);
-                fv[q][c][e] = pv[q][c][e] - c1 * hv[q][c][e];
-                if (k0) {
-                    fv[q][c][e] = 0.;
-                    fw[q][c][e] = 0.;
-                }
+                hw[0][c][e] = -beta * tmp[c][e].y;
+                hw[1][c][e] = beta * tmp[c][e].x;
+                g[0][c][e] = -alfa * tmp[c][e].y;
+                g[1][c][e] = alfa * tmp[c][e].x;
             }
-    const double bc0[2][2] = {{0., 0.}, {0., 0.}};
-    // Helmholtz solves with lam^2 (linearbl.f:625-626) and v with k^2
-    // (linearbl.f:1000-1001); the homogeneous pair phi_h, v_h is precomputed
-    double ph[2][2][E], om[2][2][E], v[2][2][E];
-    solve_w<SW, 2, E>(n, lam2, gl, tw, fv, bc0, ph, l);
-    solve_w<SW, 2, E>(n, lam2, gl, tw, fw, bc0, om, l);
-    solve_w<SW, 2, E>(n, k2, gk, tw, ph, bc0, v, l);
+        io.load(C0 + 2 * fs, tmp);                  // H3
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                hw[0][c][e] += alfa * tmp[c][e].y;
+                hw[1][c][e] -= alfa * tmp[c][e].x;
+                g[0][c][e] -= beta * tmp[c][e].y;
+                g[1][c][e] += beta * tmp[c][e].x;
+            }
+        dcheb_w<SW, 2, E>(g, dg, l);
+    }
+    // h_v = -(dg/dy + k^2 H2), f_v = p_v - c1 h_v, f_omega = p_omega - c1 h_omega
+    // (linearbl.f:345-361); channels 0, 1 of f are f_v, channels 2, 3 f_omega.
+    constexpr bool early = SW == 16;
+    double f[4][2][E], hv[2][2][E];
+    io.load(C0 + fs, tmp);                          // H2
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            dg[0][c][e] = -(dg[0][c][e] + k2 * tmp[c][e].x);   // h_v from here on
+            dg[1][c][e] = -(dg[1][c][e] + k2 * tmp[c][e].y);
+        }
+    io.load(P0, tmp);                               // p_v
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            f[0][c][e] = k0 ? 0. : tmp[c][e].x - c1 * dg[0][c][e];
+            f[1][c][e] = k0 ? 0. : tmp[c][e].y - c1 * dg[1][c][e];
+            tmp[c][e] = make_double2(-c2 * dg[0][c][e], -c2 * dg[1][c][e]);
+            hv[0][c][e] = dg[0][c][e];
+            hv[1][c][e] = dg[1][c][e];
+        }
+    if (early)
+        io.store(P0, tmp);
+    io.load(P0 + fs, tmp);                          // p_omega
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            f[2][c][e] = k0 ? 0. : tmp[c][e].x - c1 * hw[0][c][e];
+            f[3][c][e] = k0 ? 0. : tmp[c][e].y - c1 * hw[1][c][e];
+            tmp[c][e] = make_double2(-c2 * hw[0][c][e], -c2 * hw[1][c][e]);
+        }
+    if (early)
+        io.store(P0 + fs, tmp);
+    const double bc0[4][2] = {{0., 0.}, {0., 0.}, {0., 0.}, {0., 0.}};
+    // phi and omega: Helmholtz solves with lam^2 (linearbl.f:625-626)
+    io.load(lf.gp, tmp);
+    solve_w<SW, 4, E, KT>(n, lam2, tmp, tw, f, bc0, l);
+    double v[2][2][E], om[2][2][E];
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            v[0][c][e] = f[0][c][e];
+            v[1][c][e] = f[1][c][e];
+            om[0][c][e] = f[2][c][e];
+            om[1][c][e] = f[3][c][e];
+        }
+    // v from phi with k^2 (linearbl.f:1000-1001)
+    {
+        io.load(kf.gp, tmp);
+        const double bcv[2][2] = {{0., 0.}, {0., 0.}};
+        solve_w<SW, 2, E, KT>(n, k2, tmp, tw, v, bcv, l);
+    }
     // superpose v_h per parity so that dv/dy = 0 at the walls
     // (linearbl.f:1023-1070): class-c wall slope sum_k k^2 v_k
     {
@@ -384,128 +527,182 @@ __global__ void __launch_bounds__(256, (SW == 64 && E == 1) ? 3 : 1)
         for (int c = 0; c < 2; c++)
 #pragma unroll
             for (int e = 0; e < E; e++) {
-                const double kk = (double)(2 * (L * E + e) + c);
+                const double kk = (double)kidx(c, e);
                 s[0 + c] += kk * kk * v[0][c][e];
                 s[2 + c] += kk * kk * v[1][c][e];
             }
         wave_sum<SW, 4>(s, l);
         const double2 sh = lf.sh[t];
         const double shc[2] = {sh.x, sh.y};
+        io.load(lf.vh, tmp);
 #pragma unroll
         for (int c = 0; c < 2; c++) {
             const double cr = (0. - s[c]) * (1. / shc[c]);
             const double ci = (0. - s[2 + c]) * (1. / shc[c]);
 #pragma unroll
             for (int e = 0; e < E; e++) {
-                v[0][c][e] = v[0][c][e] + cr * vh[c][e];
-                v[1][c][e] = v[1][c][e] + ci * vh[c][e];
+                v[0][c][e] = v[0][c][e] + cr * tmp[c][e].x;
+                v[1][c][e] = v[1][c][e] + ci * tmp[c][e].x;
             }
         }
     }
-    // dv, phi = D^2 v - k^2 v, d omega (linearbl.f:1151-1166)
-    double dv[2][2][E], d2v[2][2][E], dom[2][2][E];
+    // dv, phi = D^2 v - k^2 v (linearbl.f:1151-1166)
+    double dv[2][2][E], ph[2][2][E];
     dcheb_w<SW, 2, E>(v, dv, l);
-    dcheb_w<SW, 2, E>(dv, d2v, l);
-    dcheb_w<SW, 2, E>(om, dom, l);
+    dcheb_w<SW, 2, E>(dv, ph, l);
 #pragma unroll
     for (int q = 0; q < 2; q++)
 #pragma unroll
         for (int c = 0; c < 2; c++)
 #pragma unroll
             for (int e = 0; e < E; e++)
-                ph[q][c][e] = d2v[q][c][e] - k2 * v[q][c][e];
-    // next partial right-hand sides (linearbl.f:1761-1784, tau branch)
-    double d1[2][2][E], d2[2][2][E];
-    dcheb_w<SW, 2, E>(ph, d1, l);
-    dcheb_w<SW, 2, E>(d1, d2, l);
+                ph[q][c][e] = ph[q][c][e] - k2 * v[q][c][e];
+    CV *Cw = cols<CV>(a);
+    // u, v, w (linearbl.f:1190-1212) in Chebyshev space; the k = 0 column
+    // stores nothing here.  Its segment (segment 0 of the wave with t0 = 0)
+    // is masked off for the whole wave, since every lane of the wave takes
+    // part in the staged stores of every segment.
+    const unsigned ownmask = io.ownmask;
+    if (t0 == 0)
+        io.ownmask &= ~1u;
+    io.own = own && !k0;
+    {
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int e = 0; e < E; e++)
+                tmp[c][e] = make_double2((-alfa * dv[1][c][e] + beta * om[1][c][e]) * k2i,
+                                         (alfa * dv[0][c][e] - beta * om[0][c][e]) * k2i);
+        io.store(Cw, tmp);
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int e = 0; e < E; e++)
+                tmp[c][e] = make_double2(v[0][c][e], v[1][c][e]);
+        io.store(Cw + fs, tmp);
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int e = 0; e < E; e++)
+                tmp[c][e] = make_double2((-alfa * om[1][c][e] - beta * dv[1][c][e]) * k2i,
+                                         (alfa * om[0][c][e] + beta * dv[0][c][e]) * k2i);
+        io.store(Cw + 2 * fs, tmp);
+    }
+    // om_x, om_z
+    double dom[2][2][E];
+    dcheb_w<SW, 2, E>(om, dom, l);
+    {
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int e = 0; e < E; e++)
+                tmp[c][e] = make_double2((-alfa * dom[1][c][e] - beta * ph[1][c][e]) * k2i,
+                                         (alfa * dom[0][c][e] + beta * ph[0][c][e]) * k2i);
+        io.store(Cw + 3 * fs, tmp);
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int e = 0; e < E; e++)
+                tmp[c][e] = make_double2((-beta * dom[1][c][e] + alfa * ph[1][c][e]) * k2i,
+                                         (beta * dom[0][c][e] - alfa * ph[0][c][e]) * k2i);
+        io.store(Cw + 4 * fs, tmp);
+    }
+    // next partial right-hand sides (linearbl.f:1761-1784, tau branch),
+    // (with 16-lane segments completing the -c2 h parts stored above)
+    io.ownmask = ownmask;
+    io.own = own;
     const double k2m = k2 - c1n;
-    CV *Cw = cols<CV>(a) + (size_t)t * n;
-    double2 *Pw = a.Pp + (size_t)t * n;
-#pragma unroll
-    for (int c = 0; c < 2; c++)
-#pragma unroll
-        for (int e = 0; e < E; e++) {
-            const int k = 2 * (L * E + e) + c;
-            if (own && k < n)
-                Pw[k] = make_double2(-d2[0][c][e] + k2m * ph[0][c][e] - c2 * hv[0][c][e],
-                                     -d2[1][c][e] + k2m * ph[1][c][e] - c2 * hv[1][c][e]);
-        }
-    dcheb_w<SW, 2, E>(dom, d1, l);
-#pragma unroll
-    for (int c = 0; c < 2; c++)
-#pragma unroll
-        for (int e = 0; e < E; e++) {
-            const int k = 2 * (L * E + e) + c;
-            if (own && k < n)
-                Pw[fs + k] = make_double2(-d1[0][c][e] + k2m * om[0][c][e] - c2 * hw[0][c][e],
-                                          -d1[1][c][e] + k2m * om[1][c][e] - c2 * hw[1][c][e]);
-        }
-    if (!k0) {
-        // u, w, om_x, om_z (linearbl.f:1190-1212) in Chebyshev space
+    {
+        double d1[2][2][E], d2[2][2][E];
+        dcheb_w<SW, 2, E>(ph, d1, l);
+        dcheb_w<SW, 2, E>(d1, d2, l);
+        if (early)
+            io.load(P0, tmp);
 #pragma unroll
         for (int c = 0; c < 2; c++)
 #pragma unroll
             for (int e = 0; e < E; e++) {
-                const int k = 2 * (L * E + e) + c;
-                if (!own || k >= n)
-                    continue;
-                const double dvr = dv[0][c][e], dvi = dv[1][c][e];
-                const double omr = om[0][c][e], omi = om[1][c][e];
-                const double dor = dom[0][c][e], doi = dom[1][c][e];
-                const double phr = ph[0][c][e], phi = ph[1][c][e];
-                put(Cw + k, (-alfa * dvi + beta * omi) * k2i, (alfa * dvr - beta * omr) * k2i);
-                put(Cw + fs + k, v[0][c][e], v[1][c][e]);
-                put(Cw + 2 * fs + k, (-alfa * omi - beta * dvi) * k2i,
-                    (alfa * omr + beta * dvr) * k2i);
-                put(Cw + 3 * fs + k, (-alfa * doi - beta * phi) * k2i,
-                    (alfa * dor + beta * phr) * k2i);
-                put(Cw + 4 * fs + k, (-beta * doi + alfa * phi) * k2i,
-                    (beta * dor - alfa * phr) * k2i);
+                if (!early)
+                    tmp[c][e] = make_double2(-c2 * hv[0][c][e], -c2 * hv[1][c][e]);
+                tmp[c][e] = make_double2(tmp[c][e].x - d2[0][c][e] + k2m * ph[0][c][e],
+                                         tmp[c][e].y - d2[1][c][e] + k2m * ph[1][c][e]);
             }
-        return;
+        io.store(P0, tmp);
     }
+    {
+        double d1[2][2][E];
+        dcheb_w<SW, 2, E>(dom, d1, l);
+        if (early)
+            io.load(P0 + fs, tmp);
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                if (!early)
+                    tmp[c][e] = make_double2(-c2 * hw[0][c][e], -c2 * hw[1][c][e]);
+                tmp[c][e] = make_double2(tmp[c][e].x - d1[0][c][e] + k2m * om[0][c][e],
+                                         tmp[c][e].y - d1[1][c][e] + k2m * om[1][c][e]);
+            }
+        io.store(P0 + fs, tmp);
+    }
+    if (!k0)
+        return;
     // k = 0: mean flow and constant-flux pressure gradient
-    // (linearbl.f:303-319, 1216-1417, 1734-1746; tau branch, ibc = 0)
-    double fuw[2][2][E], uw[2][2][E];
+    // (linearbl.f:303-319, 1216-1417, 1734-1746; tau branch, ibc = 0); the
+    // real parts of H1, H3 are read again (slots 0-2 are not yet rewritten)
+    const CV *Cc = C0 + (size_t)t * n;
+    double h1r[2][E], h3r[2][E], uw[2][2][E];
+    double2 gl[2][E];
+    const double2 *glp = lf.gp + (size_t)t * n;
     double px = *a.px;
+    const double2 z2 = make_double2(0., 0.);
 #pragma unroll
     for (int c = 0; c < 2; c++)
 #pragma unroll
         for (int e = 0; e < E; e++) {
-            const int k = 2 * (L * E + e) + c;
+            const int k = kidx(c, e);
             const bool ok = k < n;
-            fuw[0][c][e] = ok ? a.puw[k] - c1 * H1[0][c][e] : 0.;
-            fuw[1][c][e] = ok ? a.puw[n + k] - c1 * H3[0][c][e] : 0.;
+            h1r[c][e] = ok ? widen(Cc[k]).x : 0.;
+            h3r[c][e] = ok ? widen(Cc[2 * fs + k]).x : 0.;
+            gl[c][e] = ok ? glp[k] : z2;
         }
-    if (L == 0)
-        fuw[0][0][0] += 2. * re * px;
+    auto rhs = [&](double pxs) {
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int k = kidx(c, e);
+                const bool ok = k < n;
+                uw[0][c][e] = ok ? a.puw[k] - c1 * h1r[c][e] : 0.;
+                uw[1][c][e] = ok ? a.puw[n + k] - c1 * h3r[c][e] : 0.;
+            }
+        if (L == 0)
+            uw[0][0][0] += pxs;
+    };
     const double bcu[2][2] = {{.5 * (a.u0upp + a.u0low), .5 * (a.u0upp - a.u0low)},
                               {.5 * (a.w0upp + a.w0low), .5 * (a.w0upp - a.w0low)}};
-    solve_w<SW, 2, E>(n, lam2, gl, tw, fuw, bcu, uw, l);
+    rhs(2. * re * px);
+    solve_w<SW, 2, E, KT>(n, lam2, gl, tw, uw, bcu, l);
     if (a.cflux) {
-        auto flux = [&](const double (&u)[2][2][E]) {
+        auto flux = [&]() {
             // integral over [-1,1] of sum_k u_k T_k (linearbl.f:1367-1370)
             double m[1] = {0.};
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 const int k = 2 * (L * E + e);
                 if (k < n)
-                    m[0] -= u[0][0][e] * 2. / (double)(k * k - 1);
+                    m[0] -= uw[0][0][e] * 2. / (double)(k * k - 1);
             }
             wave_sum<SW, 1>(m, l);
             return m[0];
         };
-        const double m1 = flux(uw);
-        if (L == 0)
-            fuw[0][0][0] = fuw[0][0][0] + 2. * re * px;
-        solve_w<SW, 2, E>(n, lam2, gl, tw, fuw, bcu, uw, l);
-        const double m2 = flux(uw);
-        if (L == 0)
-            fuw[0][0][0] = fuw[0][0][0] - 4. * re * px;
+        const double m1 = flux();
+        rhs(4. * re * px);
+        solve_w<SW, 2, E, KT>(n, lam2, gl, tw, uw, bcu, l);
+        const double m2 = flux();
         px = px * (1. + (a.mflux - m1) / (m2 - m1));
-        if (L == 0)
-            fuw[0][0][0] = fuw[0][0][0] + 2. * re * px;
-        solve_w<SW, 2, E>(n, lam2, gl, tw, fuw, bcu, uw, l);
+        rhs(2. * re * px);
+        solve_w<SW, 2, E, KT>(n, lam2, gl, tw, uw, bcu, l);
         if (L == 0)
             *a.px = px;
     }
@@ -516,16 +713,17 @@ __global__ void __launch_bounds__(256, (SW == 64 && E == 1) ? 3 : 1)
     for (int c = 0; c < 2; c++)
 #pragma unroll
         for (int e = 0; e < E; e++) {
-            const int k = 2 * (L * E + e) + c;
+            const int k = kidx(c, e);
             if (k >= n)
                 continue;
-            a.puw[k] = -d2uw[0][c][e] - c1n * uw[0][c][e] - c2 * H1[0][c][e];
-            a.puw[n + k] = -d2uw[1][c][e] - c1n * uw[1][c][e] - c2 * H3[0][c][e];
-            put(Cw + k, uw[0][c][e], 0.);
-            put(Cw + fs + k, 0., 0.);
-            put(Cw + 2 * fs + k, uw[1][c][e], 0.);
-            put(Cw + 3 * fs + k, duw[1][c][e], 0.);
-            put(Cw + 4 * fs + k, -duw[0][c][e], 0.);
+            a.puw[k] = -d2uw[0][c][e] - c1n * uw[0][c][e] - c2 * h1r[c][e];
+            a.puw[n + k] = -d2uw[1][c][e] - c1n * uw[1][c][e] - c2 * h3r[c][e];
+            CV *Ck = Cw + (size_t)t * n;
+            put(Ck + k, uw[0][c][e], 0.);
+            put(Ck + fs + k, 0., 0.);
+            put(Ck + 2 * fs + k, uw[1][c][e], 0.);
+            put(Ck + 3 * fs + k, duw[1][c][e], 0.);
+            put(Ck + 4 * fs + k, -duw[0][c][e], 0.);
             if constexpr (sizeof(CV) == sizeof(float2)) {
                 // the mixed form also keeps the mean-flow column in FP64, so
                 // that downloads and the pressure see the FP64 mean profiles
@@ -551,10 +749,12 @@ void launch_seg(const ColArgs &a, LamFactor lf, LamFactor kf, double an, double 
     hipLaunchKernelGGL((k_tau_w<SW, E, CV>), grid, dim3(256), 0, s, a, lf, kf, an, bn, anp1, bnp1);
 }
 
-// Segment width: 64 lanes (one column per wave, three waves per SIMD at
-// ny <= 127), unless SIMSON_TAU_SEG=32 asks for two columns per wave (fewer
-// lane moves per column but one wave per SIMD: slower at 128x97x128,
-// 91 against 62 us per substep).
+// Segment width: 64 lanes (one column per wave).  SIMSON_TAU_SEG=16 or 32
+// selects narrower segments where a class fits (4 or 2 columns per wave,
+// fewer lane moves per column).  At 128x97x128 they measured slower, since
+// 8128 columns then make only 2 (16 lanes) or 4 (32 lanes) waves per SIMD
+// and the scans' dependent chains are no longer hidden: tau 82 us (16) and
+// 91 us (32, before the staged I/O) against 62 us (64) per FP64 substep.
 int tau_segment(int ny)
 {
     static int forced = -1;
@@ -565,6 +765,8 @@ int tau_segment(int ny)
     const int ce = (ny + 1) / 2;              // elements of the even class
     if (forced == 32 && (ce + 31) / 32 <= 3)
         return 32;
+    if (forced == 16 && (ce + 15) / 16 <= 4)
+        return 16;
     return 64;
 }
 
@@ -575,6 +777,15 @@ int launch_tau_t(const ColArgs &a, LamFactor lf, LamFactor kf, double an, double
     const int ce = (a.ny + 1) / 2;
     const int SW = tau_segment(a.ny);
     const int E = (ce + SW - 1) / SW;
+    if (SW == 16) {
+        switch (E) {
+        case 1: launch_seg<16, 1, CV>(a, lf, kf, an, bn, anp1, bnp1, s); return 0;
+        case 2: launch_seg<16, 2, CV>(a, lf, kf, an, bn, anp1, bnp1, s); return 0;
+        case 3: launch_seg<16, 3, CV>(a, lf, kf, an, bn, anp1, bnp1, s); return 0;
+        case 4: launch_seg<16, 4, CV>(a, lf, kf, an, bn, anp1, bnp1, s); return 0;
+        default: return SIMSON_E_ARG;
+        }
+    }
     if (SW == 32) {
         switch (E) {
         case 1: launch_seg<32, 1, CV>(a, lf, kf, an, bn, anp1, bnp1, s); return 0;

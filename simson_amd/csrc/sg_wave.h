@@ -66,6 +66,8 @@ __device__ __forceinline__ double readlane(double x, int lane)
 template <int SW>
 __device__ __forceinline__ double from_next(double x, double fill, int l)
 {
+    if (SW == 16)              // row_shl:1 stays in the row; lane 15 keeps fill
+        return (fill == 0.0) ? dpp0<0x101>(x) : dpp<0x101>(fill, x);
     double r = (fill == 0.0) ? dpp0<0x130>(x) : dpp<0x130>(fill, x);
     if (SW < kWave && (l & (SW - 1)) == SW - 1)
         r = fill;
@@ -76,6 +78,8 @@ __device__ __forceinline__ double from_next(double x, double fill, int l)
 template <int SW>
 __device__ __forceinline__ double from_prev(double x, double fill, int l)
 {
+    if (SW == 16)              // row_shr:1; lane 0 of the row keeps fill
+        return (fill == 0.0) ? dpp0<0x111>(x) : dpp<0x111>(fill, x);
     double r = (fill == 0.0) ? dpp0<0x138>(x) : dpp<0x138>(fill, x);
     if (SW < kWave && (l & (SW - 1)) == 0)
         r = fill;
@@ -177,6 +181,20 @@ __device__ __forceinline__ void scan_suffix(double (&X)[NX], double &Y, int l)
 template <int SW, int NX>
 __device__ __forceinline__ void sum_all(double (&X)[NX], int l)
 {
+    if (SW == 16) {
+        // butterfly within the row: swap pairs, swap pair halves of each
+        // quad (quad_perm 1,0,3,2 and 2,3,0,1), mirror each half row, mirror
+        // the row.  Every lane adds the same operands (in commuted order),
+        // so all 16 lanes hold the identical total.
+#pragma unroll
+        for (int q = 0; q < NX; q++) {
+            X[q] += dpp0<0xb1>(X[q]);
+            X[q] += dpp0<0x4e>(X[q]);
+            X[q] += dpp0<0x141>(X[q]);
+            X[q] += dpp0<0x140>(X[q]);
+        }
+        return;
+    }
     double one = 1.0;
     dpp_step<0x111, NX, false>(X, one);
     dpp_step<0x112, NX, false>(X, one);
