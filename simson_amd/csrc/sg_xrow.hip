@@ -71,24 +71,71 @@ __global__ void __launch_bounds__((XBlock<V, M>::NT)) k_xrow4(Step2Args<V> a, co
         tw[i] = twg[i];
     __syncthreads();
 
-    // half spectrum X_k (k < nx/2, zero above) -> packed Z_k, k < M
-    for (int idx = tid; idx < 6 * RW * M; idx += NT) {
-        const int f = idx / (RW * M);
-        const int rem = idx - f * RW * M;
-        const int r = rem / M, k = rem - r * M;
-        V Z = vmk<V>(0, 0);
-        if (r < nr) {
-            const V *row = a.P + (((size_t)f * ny + y) * nzp + zp0 + r) * nxh;
-            V Xk = k < nxh ? row[k] : vmk<V>(0, 0);
-            if (k == 0)
-                Xk.y = 0;                        // only Re X_0 enters vrfftb
-            const int km = M - k;                // X_M (k = 0) is beyond nx/2
-            V Xm = (k > 0 && km < nxh) ? row[km] : vmk<V>(0, 0);
-            Xm.y = -Xm.y;
-            const V s = vadd(Xk, Xm), d = vsub(Xk, Xm);
-            Z = vadd(s, vmul_is(vmul(tw[k], d), (T)1));
+    // half spectrum X_k (k < nx/2, zero above) -> packed Z_k, k < M.  With
+    // M <= NT each thread keeps one k (its twiddle, flags and LDS offset)
+    // and walks the (field, row) pairs in batches of NB: the loads of a batch
+    // are unconditional (clamped addresses, zeroed by select), so they issue
+    // back to back instead of one round trip per pair.
+    constexpr int NXH = 2 * M / 3;               // nx/2 (nxp = 3 nx/2 = 2M)
+    constexpr int TPK = NT / M;                  // threads per k
+    if constexpr (TPK >= 1) {
+        constexpr int NPAIR = 6 * RW, NJ = (NPAIR + TPK - 1) / TPK;
+        constexpr int NB = NJ < 8 ? NJ : 8;
+        const int k = tid < TPK * M ? tid % M : 0;
+        const int p0 = tid < TPK * M ? tid / M : NPAIR;
+        const V twk = tw[k];
+        const int km = M - k;
+        const bool hk = k < NXH, hm = k > 0 && km < NXH;
+        const int ik = hk ? k : 0, im = hm ? km : 0;
+        const int ko = k + k / M1;
+#pragma unroll
+        for (int j0 = 0; j0 < NJ; j0 += NB) {
+            V xk[NB], xm[NB];
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                const int p = min(p0 + TPK * (j0 + b), NPAIR - 1);
+                const int f = p / RW, r = min(p - f * RW, nr - 1);
+                const V *row = a.P + (((size_t)f * ny + y) * nzp + zp0 + r) * NXH;
+                xk[b] = row[ik];
+                xm[b] = row[im];
+            }
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                const int p = p0 + TPK * (j0 + b);
+                if (j0 + b >= NJ || p >= NPAIR)
+                    continue;
+                const int r = p % RW;
+                V Xk = hk ? xk[b] : vmk<V>(0, 0);
+                if (k == 0)
+                    Xk.y = 0;                    // only Re X_0 enters vrfftb
+                V Xm = hm ? xm[b] : vmk<V>(0, 0);
+                Xm.y = -Xm.y;
+                const V sm = vadd(Xk, Xm), d = vsub(Xk, Xm);
+                V Z = vadd(sm, vmul_is(vmul(twk, d), (T)1));
+                if (r >= nr)
+                    Z = vmk<V>(0, 0);
+                A[p * SROW + ko] = Z;
+            }
         }
-        A[(f * RW + r) * SROW + k + k / M1] = Z;
+    } else {
+        for (int idx = tid; idx < 6 * RW * M; idx += NT) {
+            const int f = idx / (RW * M);
+            const int rem = idx - f * RW * M;
+            const int r = rem / M, k = rem - r * M;
+            V Z = vmk<V>(0, 0);
+            if (r < nr) {
+                const V *row = a.P + (((size_t)f * ny + y) * nzp + zp0 + r) * nxh;
+                V Xk = k < nxh ? row[k] : vmk<V>(0, 0);
+                if (k == 0)
+                    Xk.y = 0;
+                const int km = M - k;            // X_M (k = 0) is beyond nx/2
+                V Xm = (k > 0 && km < nxh) ? row[km] : vmk<V>(0, 0);
+                Xm.y = -Xm.y;
+                const V sm = vadd(Xk, Xm), d = vsub(Xk, Xm);
+                Z = vadd(sm, vmul_is(vmul(tw[k], d), (T)1));
+            }
+            A[(f * RW + r) * SROW + k + k / M1] = Z;
+        }
     }
     __syncthreads();
 
@@ -245,20 +292,37 @@ __global__ void __launch_bounds__((XBlock<V, M>::NT)) k_xrow4(Step2Args<V> a, co
     __syncthreads();
 
     // packed spectrum -> X_k = E_k + exp(-2 pi i k/nxp) O_k for k < nx/2
-    for (int idx = tid; idx < NOUT * RW * nxh; idx += NT) {
-        const int h = idx / (RW * nxh);
-        const int rem = idx - h * RW * nxh;
-        const int r = rem / nxh, k = rem - r * nxh;
-        if (r < nr) {
-            const V *row = A + (h * RW + r) * SROW;
-            const int km = k == 0 ? 0 : M - k;
-            const V Zk = row[k + k / M1];
-            const V Zm = vconj(row[km + km / M1]);
-            const V E = vmk<V>((T)0.5 * (Zk.x + Zm.x), (T)0.5 * (Zk.y + Zm.y));
-            const V dd = vsub(Zk, Zm);
-            const V O = vmk<V>((T)0.5 * dd.y, (T)-0.5 * dd.x);
-            const V X = vadd(E, vmul(vconj(tw[k]), O));
-            a.P[(((size_t)h * ny + y) * nzp + zp0 + r) * nxh + k] = X;
+    auto unpack = [&](int h, int r, int k, V twk) {
+        const V *row = A + (h * RW + r) * SROW;
+        const int km = k == 0 ? 0 : M - k;
+        const V Zk = row[k + k / M1];
+        const V Zm = vconj(row[km + km / M1]);
+        const V E = vmk<V>((T)0.5 * (Zk.x + Zm.x), (T)0.5 * (Zk.y + Zm.y));
+        const V dd = vsub(Zk, Zm);
+        const V O = vmk<V>((T)0.5 * dd.y, (T)-0.5 * dd.x);
+        a.P[(((size_t)h * ny + y) * nzp + zp0 + r) * NXH + k] = vadd(E, vmul(vconj(twk), O));
+    };
+    constexpr int TPX = NT / NXH;                // threads per output k
+    if constexpr (TPX >= 1) {
+        constexpr int NQ = NOUT * RW, NJ = (NQ + TPX - 1) / TPX;
+        const int k = tid % NXH, q0 = tid / NXH;
+        const V twk = tw[k];
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const int q = q0 + TPX * j;
+            if (tid >= TPX * NXH || q >= NQ)
+                continue;
+            const int h = q / RW, r = q - h * RW;
+            if (r < nr)
+                unpack(h, r, k, twk);
+        }
+    } else {
+        for (int idx = tid; idx < NOUT * RW * NXH; idx += NT) {
+            const int h = idx / (RW * NXH);
+            const int rem = idx - h * RW * NXH;
+            const int r = rem / NXH, k = rem - r * NXH;
+            if (r < nr)
+                unpack(h, r, k, tw[k]);
         }
     }
     if (a.do_cfl) {

@@ -77,24 +77,44 @@ __global__ void __launch_bounds__((ZPlan<V, N>::NT)) k_zsyn4(Step2Args<V> a, con
     const int nb = min(B, nxh - x0);
     for (int i = tid; i < N; i += NT)
         tw[i] = twg[i];
-    for (int idx = tid; idx < N * B; idx += NT) {
-        const int zp = idx / B, b = idx - zp * B;
-        V v = vmk<V>(0, 0);
-        // storage row of padded mode zp; rows nz/2 .. N-nz/2 stay zero
-        const int z = zp < nz / 2 ? zp : (zp > N - nz / 2 ? zp - (N - nz) : -1);
-        // with split_mean the kx = kz = 0 mode enters the products directly
-        if (b < nb && z >= 0 && !(a.split_mean && z == 0 && x0 + b == 0)) {
-            const int x = x0 + b;
-            if (f == 4) {
-                const V u = a.U[uidx(0, z, y, x, nz, ny, nxh)];
-                const V w = a.U[uidx(2, z, y, x, nz, ny, nxh)];
-                const T be = (T)a.beta[z], al = (T)a.alfa[x];
-                v = vmk<V>(-be * u.y + al * w.y, be * u.x - al * w.x);
-            } else {
-                v = a.U[uidx(f < 4 ? f : 4, z, y, x, nz, ny, nxh)];
+    // thread: column b = tid % B, padded rows zp = tid / B + ZS j; the loads
+    // of a batch of NBT rows use clamped indices and issue back to back
+    static_assert(NT % B == 0, "block must cover whole rows");
+    constexpr int ZS = NT / B, NI = (N + ZS - 1) / ZS, NBT = NI < 6 ? NI : 6;
+    const int b = tid % B, zq = tid / B;
+    const int xc = min(x0 + b, nxh - 1);
+    const T al = (T)a.alfa[xc];
+    const bool om_y = f == 4;                   // block-uniform
+#pragma unroll
+    for (int j0 = 0; j0 < NI; j0 += NBT) {
+        V u[NBT], w[NBT];
+        T bt[NBT];
+#pragma unroll
+        for (int q = 0; q < NBT; q++) {
+            const int zp = min(zq + ZS * (j0 + q), N - 1);
+            // storage row of padded mode zp; rows nz/2 .. N-nz/2 stay zero
+            const int z = zp < nz / 2 ? zp : (zp > N - nz / 2 ? zp - (N - nz) : 0);
+            // fields u, v, w, om_x, om_y, om_z from storage slots 0-3, (0, 2), 4
+            u[q] = a.U[uidx(om_y ? 0 : (f < 4 ? f : 4), z, y, xc, nz, ny, nxh)];
+            if (om_y) {
+                w[q] = a.U[uidx(2, z, y, xc, nz, ny, nxh)];
+                bt[q] = (T)a.beta[z];
             }
         }
-        A[b * SCOL + zp + zp / N1] = v;
+#pragma unroll
+        for (int q = 0; q < NBT; q++) {
+            const int zp = zq + ZS * (j0 + q);
+            if (j0 + q >= NI || zp >= N)
+                continue;
+            const int z = zp < nz / 2 ? zp : (zp > N - nz / 2 ? zp - (N - nz) : -1);
+            V v = u[q];
+            if (om_y)
+                v = vmk<V>(-bt[q] * u[q].y + al * w[q].y, bt[q] * u[q].x - al * w[q].x);
+            // with split_mean the kx = kz = 0 mode enters the products directly
+            if (b >= nb || z < 0 || (a.split_mean && z == 0 && x0 + b == 0))
+                v = vmk<V>(0, 0);
+            A[b * SCOL + zp + zp / N1] = v;
+        }
     }
     __syncthreads();
     for (int t = tid; t < B * N1; t += NT) {
@@ -150,9 +170,23 @@ __global__ void __launch_bounds__((ZPlan<V, N>::NT)) k_zana4(Step2Args<V> a, con
     for (int i = tid; i < N; i += NT)
         tw[i] = twg[i];
     const V *Pp = a.P + (((size_t)h * ny + y) * N) * nxh + x0;
-    for (int idx = tid; idx < N * B; idx += NT) {
-        const int zp = idx / B, b = idx - zp * B;
-        A[b * SCOL + dr_off<N1, N2>(zp)] = b < nb ? Pp[(size_t)zp * nxh + b] : vmk<V>(0, 0);
+    {
+        static_assert(NT % B == 0, "block must cover whole rows");
+        constexpr int ZS = NT / B, NI = (N + ZS - 1) / ZS, NBT = NI < 8 ? NI : 8;
+        const int b = tid % B, zq = tid / B, bc = min(b, nb - 1);
+#pragma unroll
+        for (int j0 = 0; j0 < NI; j0 += NBT) {
+            V v[NBT];
+#pragma unroll
+            for (int q = 0; q < NBT; q++)
+                v[q] = Pp[(size_t)min(zq + ZS * (j0 + q), N - 1) * nxh + bc];
+#pragma unroll
+            for (int q = 0; q < NBT; q++) {
+                const int zp = zq + ZS * (j0 + q);
+                if (j0 + q < NI && zp < N)
+                    A[b * SCOL + dr_off<N1, N2>(zp)] = b < nb ? v[q] : vmk<V>(0, 0);
+            }
+        }
     }
     __syncthreads();
     for (int t = tid; t < B * N2; t += NT) {
