@@ -58,7 +58,7 @@ constexpr int lds_rows()
 }
 
 template <typename T, int KS, bool IN_COL, bool OUT_COL>
-__global__ void __launch_bounds__(256) k_cheb_mfma(const T *__restrict__ A, const T *__restrict__ in,
+__global__ void __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 3)) k_cheb_mfma(const T *__restrict__ A, const T *__restrict__ in,
                                                     T *__restrict__ out, GemmIO io, int ntile_x)
 {
     typedef typename Mfma<T>::v4 v4;
@@ -85,42 +85,83 @@ __global__ void __launch_bounds__(256) k_cheb_mfma(const T *__restrict__ A, cons
 
     const int nplane = io.nf * io.nz;
     const int ntile = ntile_x * nplane;
+    const size_t plane = (size_t)ny * nc;
+    const int len = 2 * ny;                      // reals per Chebyshev column
+
+    // The LDS rows k >= ny are never loaded: A is zero there, and the rows
+    // only need finite values (zero now; later, column-layout output).
+    for (int e = tid; e < (lds_rows<T, KS>() - ny) * kLdb; e += 256)
+        Bs[ny * kLdb + e] = (T)0;
+
+    // Tile input is fetched into registers one tile ahead (the loads of the
+    // next tile are in flight during the products of this one), then staged
+    // into LDS.  Row layout: element e = k * kNB + n; column layout: the 16
+    // columns of a tile are contiguous, element e = xl * len + rem.
+    constexpr int NE = (4 * KS * kNB + 255) / 256;
+    T buf[NE];
+    auto fetch = [&](int tile) {
+        const int pl = tile / ntile_x, tx = tile - pl * ntile_x;
+        const int f = pl / io.nz, z = pl - f * io.nz;
+        const T *X = in + ((size_t)io.in_slot[f] * io.nz + z) * plane;
+        const int n0 = tx * kNB;
+        if (!IN_COL) {
+#pragma unroll
+            for (int i = 0; i < NE; i++) {
+                const int e = tid + 256 * i;
+                const int k = min(e / kNB, ny - 1), n = min(n0 + (e % kNB), nc - 1);
+                buf[i] = X[(size_t)k * nc + n];
+            }
+        } else {
+            const int x0 = n0 >> 1;
+            const int last = min(kNB / 2, nxh - x0) * len - 1;
+            const T *Xt = X + (size_t)x0 * len;
+#pragma unroll
+            for (int i = 0; i < NE; i++)
+                buf[i] = Xt[min(tid + 256 * i, last)];
+        }
+    };
+    auto stage = [&](int tile) {
+        const int tx = tile % ntile_x;
+        const int n0 = tx * kNB;
+        if (!IN_COL) {
+#pragma unroll
+            for (int i = 0; i < NE; i++) {
+                const int e = tid + 256 * i;
+                const int k = e / kNB, n = e % kNB;
+                if (k < ny)
+                    Bs[k * kLdb + n] = n0 + n < nc ? buf[i] : (T)0;
+            }
+        } else {
+            const int xn = min(kNB / 2, nxh - (n0 >> 1));
+            int xl = tid / len, rem = tid - xl * len;
+#pragma unroll
+            for (int i = 0; i < NE; i++) {
+                if (xl < kNB / 2)
+                    Bs[(rem >> 1) * kLdb + 2 * xl + (rem & 1)] = xl < xn ? buf[i] : (T)0;
+                rem += 256;
+                while (rem >= len) {
+                    rem -= len;
+                    xl++;
+                }
+            }
+        }
+    };
+
+    if (blockIdx.x < ntile)
+        fetch(blockIdx.x);
     for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
         const int pl = tile / ntile_x, tx = tile - pl * ntile_x;
         const int f = pl / io.nz, z = pl - f * io.nz;
-        if (z == io.skip_z)
-            continue;
-        const size_t plane = (size_t)ny * nc;
-        const T *X = in + ((size_t)io.in_slot[f] * io.nz + z) * plane;
         T *Y = out + ((size_t)io.out_slot[f] * io.nz + z) * plane;
         const int n0 = tx * kNB, x0 = n0 >> 1;
 
         __syncthreads();                  // previous tile's LDS reads are done
-        if (!IN_COL) {
-            for (int e = tid; e < 4 * KS * kNB; e += 256) {
-                const int k = e / kNB, n = e - k * kNB;
-                Bs[k * kLdb + n] = (k < ny && n0 + n < nc) ? X[(size_t)k * nc + n0 + n] : (T)0;
-            }
-        } else {
-            // the tile's columns x0..x0+15 are contiguous: [x][k][re,im];
-            // thread tid walks the element offsets rem = tid, tid + 256 ...
-            // of every column (no division by the runtime length)
-            const int len = 2 * ny;
-            const T *Xt = X + (size_t)x0 * len;
-            const int xn = min(kNB / 2, nxh - x0);
-            for (int rem = tid; rem < len; rem += 256) {
-                const int k = rem >> 1, r = rem & 1;
-                T *dst = Bs + k * kLdb + r;
-#pragma unroll 4
-                for (int xl = 0; xl < kNB / 2; xl++)
-                    dst[2 * xl] = xl < xn ? Xt[(size_t)xl * len + rem] : (T)0;
-            }
-            for (int e = tid; e < (4 * KS - ny) * kNB; e += 256) {
-                const int k = ny + e / kNB, n = e % kNB;
-                Bs[k * kLdb + n] = (T)0;
-            }
-        }
+        stage(tile);
         __syncthreads();
+        if (tile + (int)gridDim.x < ntile)
+            fetch(tile + gridDim.x);
+        if (z == io.skip_z)
+            continue;
 
         v4 acc[2][2];
 #pragma unroll
@@ -180,7 +221,6 @@ __global__ void __launch_bounds__(256) k_cheb_mfma(const T *__restrict__ A, cons
                         acc[j][nb][r];
         }
         __syncthreads();
-        const int len = 2 * ny;
         T *Yt = Y + (size_t)x0 * len;
         const int xn = min(kNB / 2, nxh - x0);
         for (int rem = tid; rem < len; rem += 256) {
