@@ -125,6 +125,29 @@ int simson_gpu_amp(simson_gpu_t *g, double *amp);
  * synchronises. */
 int simson_gpu_pressure(simson_gpu_t *g, double *pr, double *pi);
 
+/* ADM-RT large-eddy model: the relaxation term of nonlinbl.f:920-966 with
+ * the high-pass filtered velocity of linearbl.f:1871-1962 (ihighorder = 2,
+ * the filters of init_filter.f and bla.f:1494-1518), as bla runs it when
+ * bla.i sets sgs and sgs.i selects iles = 1 (rparamles.f:62-71).
+ * iles 0 turns the model off.  cutoff = 1/cutoff_inv of sgs.i (0 < cutoff
+ * <= 1), 0 <= iord <= 5, and chi is the relaxation rate chi/dstar of
+ * bla.f:2220 (dstar = 1 for the channel).  Call after init, before the
+ * first step; needs nzp among the specialised z transform lengths. */
+typedef struct simson_les {
+    int iles;            /* 0: no model; 1: ADM-RT */
+    int iord;            /* sgs.i iord */
+    double cutoff;       /* 1 / sgs.i cutoff_inv */
+    double chi;          /* sgs.i chi (divided by dstar) */
+} simson_les_t;
+
+int simson_gpu_les(simson_gpu_t *g, const simson_les_t *p);
+
+/* The ADM-RT filter operators of one grid, without a device: lpf [nz][nx/2]
+ * is the x-z transfer function lpfxz of bla.f:1503-1518 and L [ny][ny] the
+ * wall-normal low-pass I - H_y^(iord+1), H_y the compact high-pass filter of
+ * hpf_y.f on the Gauss-Lobatto points.  Returns 0 or SIMSON_E_ARG. */
+int simson_les_filter(int nx, int ny, int nz, double cutoff, int iord, double *lpf, double *L);
+
 /* Wait for all queued work. */
 int simson_gpu_sync(simson_gpu_t *g);
 

@@ -58,7 +58,15 @@ int orc_run(orc_t *o, double *ur, double *ui, double *puw, int nsteps,
  * hold u, v, w, omega_x, omega_z. */
 void orc_pressure(orc_t *o, const double *ur, const double *ui, double *pr, double *pi);
 
+/* ADM-RT LES (sgs.i iles = 1, ihighorder = 2): iles 0 turns it off.
+ * cutoff = 1/cutoff_inv, iord, chi as read by rparamles.f; dstar = 1.
+ * Returns 0, or -1 for unsupported parameters. */
+int orc_set_les(orc_t *o, int iles, double cutoff, int iord, double chi);
+
 /* ---- building blocks exposed for unit tests ---- */
+/* the ADM-RT filter operators: lpf[nz][nx/2] (bla.f:1503-1518) and the
+ * wall-normal low-pass L = I - H_y^(iord+1) as a dense [ny][ny] matrix */
+int orc_les_filter(int nx, int ny, int nz, double cutoff, int iord, double *lpf, double *L);
 /* complex DFT, y_k = sum_j x_j exp(sign 2 pi i jk/n), in place on (re,im) */
 void orc_cdft(int n, double *re, double *im, int sign);
 /* vrfftb convention: half spectrum X[0..n/2] (re,im) -> x[0..n-1] real */

@@ -49,6 +49,10 @@ def lib():
         L.orc_tau_solve_neumann.argtypes = [C.c_int, C.c_double, _dp, C.c_double, C.c_double,
                                             _dp]
         L.orc_pressure.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_set_les.restype = C.c_int
+        L.orc_set_les.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double]
+        L.orc_les_filter.restype = C.c_int
+        L.orc_les_filter.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _dp, _dp]
         _lib = L
     return _lib
 
@@ -101,12 +105,25 @@ class Oracle:
         st = lib().orc_run(self.h, ur, ui, puw, nsteps, dt, icfl, C.byref(cfl))
         return st, cfl.value
 
+    def set_les(self, iles=1, cutoff=2.0 / 3.0, iord=5, chi=0.2):
+        """ADM-RT relaxation model (sgs.i iles = 1) on, or off with iles = 0."""
+        if lib().orc_set_les(self.h, iles, cutoff, iord, chi) != 0:
+            raise ValueError("unsupported LES parameters")
+
     def pressure(self, ur, ui):
         """Pressure [z][y][x] (complex) of the state in ur, ui (slots 0-4)."""
         pr = np.zeros((self.nz, self.ny, self.nx // 2))
         pi = np.zeros_like(pr)
         lib().orc_pressure(self.h, ur, ui, pr, pi)
         return pr + 1j * pi
+
+
+def les_filter(nx, ny, nz, cutoff, iord):
+    """(lpf [nz][nx/2], L [ny][ny]) of the ADM-RT filters."""
+    lpf = np.zeros((nz, nx // 2))
+    L = np.zeros((ny, ny))
+    lib().orc_les_filter(nx, ny, nz, cutoff, iord, lpf, L)
+    return lpf, L
 
 
 def velocities(ur, ui):

@@ -170,9 +170,7 @@ def parse(text: str) -> BlaParams:
     if p.spat:
         raise BlaInputError("spatial simulations are not supported")
     p.cdev = _num(tok())
-    p.sgs = _logical(tok())
-    if p.sgs:
-        raise BlaInputError("LES (sgs) is not supported by the step library")
+    p.sgs = _logical(tok())          # LES parameters then come from sgs.i (simson_amd.sgsi)
     p.isfd = int(_num(tok()))
     if p.isfd != 0:
         raise BlaInputError("selective frequency damping is not supported")
@@ -258,7 +256,7 @@ def format_channel(p: BlaParams) -> str:
         ".false.         Constant wall-suction (suction)",
         ".false.         Spatial simulation (spat)",
         f"{p.cdev!r}      Temporal development vel. (cdev)",
-        ".false.         Read sgs.i (sgs)",
+        f"{L(p.sgs)}     Read sgs.i (sgs)",
         "0               SFD (isfd)",
         "0               MHD (imhd)",
         "0               Type of localized disturbance (loctyp)",

@@ -15,9 +15,9 @@ namespace {
 
 const char *const kFamilies[] = {"zsyn", "xrow", "zana", "cfl_reduce", "cheb_fwd",
                                  "tau", "cheb_bwd", "prhs_cheb_fwd", "prhs",
-                                 "prhs_cheb_bwd", "tau_factor"};
+                                 "prhs_cheb_bwd", "tau_factor", "les_filter"};
 enum Fam { F_ZSYN, F_XROW, F_ZANA, F_CFL, F_CHF, F_TAU, F_CHB, F_PCHF, F_PRHS, F_PCHB,
-           F_FAC, F_N };
+           F_FAC, F_LES, F_N };
 
 struct TimedLaunch {
     int fam;
@@ -58,6 +58,15 @@ struct simson_gpu {
     bool mean64 = false;              // C holds the FP64 mean-flow column of U32's state
     float2 *U32 = nullptr, *P32 = nullptr, *C32 = nullptr, *twz32 = nullptr, *twx32 = nullptr;
     float *Tf32 = nullptr, *Tb32 = nullptr;
+    // ADM-RT model (simson_gpu_les): the wall-normal low-pass L (FP64 and
+    // FP32), the x-z transfer function lpf [nz][nxh] and the filtered
+    // velocities Wf = L u of slots 0-2
+    bool les = false;
+    double les_chi = 0.;
+    double *Lm = nullptr, *lpf = nullptr;
+    float *Lm32 = nullptr;
+    double2 *Wf = nullptr;
+    float2 *Wf32 = nullptr;
     FftPlan pz, pm;
     double mflux = 0., u0upp = 0., u0low = 0., w0upp = 0., w0low = 0.;
     bool timing = false;
@@ -148,7 +157,8 @@ void free_all(simson_gpu_t *g)
                     g->kfac.gp, g->lam[0].gp, g->lam[0].vh, g->lam[0].sh,
                     g->lam[1].gp, g->lam[1].vh, g->lam[1].sh,
                     g->lam[2].gp, g->lam[2].vh, g->lam[2].sh, g->U32, g->P32, g->C32,
-                    g->twz32, g->twx32, g->Tf32, g->Tb32, g->Q, g->Qc, g->PR, g->pscr};
+                    g->twz32, g->twx32, g->Tf32, g->Tb32, g->Q, g->Qc, g->PR, g->pscr,
+                    g->Lm, g->lpf, g->Lm32, g->Wf, g->Wf32};
     for (void *q : ptrs)
         if (q)
             (void)hipFree(q);
@@ -198,6 +208,10 @@ NonlinArgs nonlin_args(simson_gpu_t *g, int do_cfl)
     a.zl = g->p.zl;
     a.do_cfl = do_cfl;
     a.split_mean = 0;
+    a.les = 0;
+    a.Wf = nullptr;
+    a.lpf = g->lpf;
+    a.chi_n = g->les_chi * (double)g->nxp * (double)g->nzp;
     return a;
 }
 
@@ -292,6 +306,10 @@ Step2Args<float2> nonlin_args32(simson_gpu_t *g, int do_cfl)
     a.zl = d.zl;
     a.do_cfl = d.do_cfl;
     a.split_mean = 1;
+    a.les = 0;
+    a.Wf = nullptr;
+    a.lpf = d.lpf;
+    a.chi_n = d.chi_n;
     return a;
 }
 
@@ -321,9 +339,20 @@ void refresh_u32(simson_gpu_t *g)
 
 int do_nonlin(simson_gpu_t *g, int do_cfl)
 {
+    const int odd = g->nz / 2;
     if (g->mixed) {
         refresh_u32(g);
         Step2Args<float2> a = nonlin_args32(g, do_cfl);
+        if (g->les) {
+            // Wf = L u of the velocities (linearbl.f:1871-1962), row layout
+            GemmIO io = gemm_io(g, g->U32, 0, g->Wf32, 0, {0, 1, 2}, {0, 1, 2}, odd);
+            timed(g, F_LES, [&] {
+                launch_cheb_mfma_f32(g->Lm32, (const float *)g->U32, (float *)g->Wf32, io,
+                                     g->stream);
+            });
+            a.les = 1;
+            a.Wf = g->Wf32;
+        }
         timed(g, F_ZSYN, [&] { launch_zsyn4(a, g->twz32, g->stream); });
         timed(g, F_XROW, [&] { launch_xrow4(a, g->twx32, g->stream); });
         if (do_cfl)
@@ -337,6 +366,12 @@ int do_nonlin(simson_gpu_t *g, int do_cfl)
         return 0;
     }
     NonlinArgs a = nonlin_args(g, do_cfl);
+    if (g->les) {
+        GemmIO io = gemm_io(g, g->U, 0, g->Wf, 0, {0, 1, 2}, {0, 1, 2}, odd);
+        timed(g, F_LES, [&] { launch_cheb_gemm(g->Lm, io, g->stream); });
+        a.les = 1;
+        a.Wf = g->Wf;
+    }
     timed(g, F_ZSYN, [&] {
         if (!launch_zsyn4(a, (const double2 *)g->twz, g->stream))
             launch_zsyn(a, g->pz, g->stream);
@@ -786,6 +821,55 @@ int simson_gpu_amp(simson_gpu_t *g, double *amp)
     CK(hipMemcpyAsync(amp, g->amp, 20 * (size_t)g->ny * sizeof(double), hipMemcpyDeviceToHost,
                       g->stream));
     CK(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int simson_les_filter(int nx, int ny, int nz, double cutoff, int iord, double *lpf, double *L)
+{
+    if (!lpf || !L)
+        return SIMSON_E_ARG;
+    return les_filter(nx, ny, nz, cutoff, iord, lpf, L);
+}
+
+int simson_gpu_les(simson_gpu_t *g, const simson_les_t *p)
+{
+    if (!g || !p)
+        return SIMSON_E_ARG;
+    if (p->iles == 0) {
+        g->les = false;
+        return 0;
+    }
+    if (p->iles != 1) {
+        g->err = "only ADM-RT (iles = 1) is supported";
+        return SIMSON_E_ARG;
+    }
+    if (!has_zfft4(g->nzp) || (g->mixed && cheb_mfma_ks(g->ny) == 0)) {
+        g->err = "ADM-RT needs nzp among the specialised z transform lengths";
+        return SIMSON_E_ARG;
+    }
+    const int ny = g->ny;
+    std::vector<double> lpf((size_t)g->nz * g->nxh), L((size_t)ny * ny);
+    if (les_filter(g->nx, ny, g->nz, p->cutoff, p->iord, lpf.data(), L.data()) != 0) {
+        g->err = "invalid ADM-RT filter parameters (0 < cutoff <= 1, 0 <= iord <= 5)";
+        return SIMSON_E_ARG;
+    }
+    const size_t n3 = 3 * field_size(g);
+    if (!g->Lm) {
+        if (dalloc(g, &g->Lm, L.size()) || dalloc(g, &g->lpf, lpf.size()))
+            return SIMSON_E_HIP;
+        if (g->mixed ? dalloc(g, &g->Lm32, L.size()) || dalloc(g, &g->Wf32, n3)
+                     : dalloc(g, &g->Wf, n3))
+            return SIMSON_E_HIP;
+    }
+    if (h2d(g, g->Lm, L.data(), L.size()) || h2d(g, g->lpf, lpf.data(), lpf.size()))
+        return SIMSON_E_HIP;
+    if (g->mixed) {
+        std::vector<float> L32(L.begin(), L.end());
+        if (h2d(g, g->Lm32, L32.data(), L32.size()))
+            return SIMSON_E_HIP;
+    }
+    g->les_chi = p->chi;
+    g->les = true;
     return 0;
 }
 

@@ -87,8 +87,20 @@ struct Step2Args {
     // products as exact per-plane coefficients, so that FP32 fields carry
     // the fluctuations at full FP32 relative precision (mixed form)
     int split_mean;
+    // ADM-RT relaxation (nonlinbl.f:920-966), applied by the z-analysis
+    // kernel: H_i -= chi_n (u_i - lpf(x, z) Wf_i) on every storage row but the
+    // z oddball, with u_i the velocity still in Uw slot i, Wf_i = L_y u_i
+    // ([3][nz][ny][nxh]) and chi_n = chi nxp nzp
+    int les;
+    const V *Wf;
+    const double *lpf;
+    double chi_n;
 };
 typedef Step2Args<double2> NonlinArgs;
+
+// ADM-RT filter operators (host): lpf [nz][nx/2] and L = I - H_y^(iord+1)
+// [ny][ny]; 0 or SIMSON_E_ARG
+int les_filter(int nx, int ny, int nz, double cutoff, int iord, double *lpf, double *L);
 
 void launch_zsyn(const NonlinArgs &a, const FftPlan &pz, hipStream_t s);
 void launch_xrow(const NonlinArgs &a, const FftPlan &pm, const double2 *wx, hipStream_t s);

@@ -221,11 +221,43 @@ __global__ void __launch_bounds__((ZPlan<V, N>::NT)) k_zana4(Step2Args<V> a, con
             col[S1 * d] = v[d];
     }
     __syncthreads();
-    for (int idx = tid; idx < nz * B; idx += NT) {
-        const int z = idx / B, b = idx - z * B;
-        if (b < nb) {
+    if (!a.les) {
+        for (int idx = tid; idx < nz * B; idx += NT) {
+            const int z = idx / B, b = idx - z * B;
+            if (b < nb) {
+                const int zp = z < nz / 2 ? z : z + N - nz;
+                a.Uw[uidx(h, z, y, x0 + b, nz, ny, nxh)] = A[b * SCOL + zp + zp / N1];
+            }
+        }
+        return;
+    }
+    // with the ADM-RT relaxation: thread column b = tid % B, rows
+    // z = tid / B + ZS j; u (still in Uw) and Wf are read in batches
+    typedef typename ScalarOf<V>::T T;
+    constexpr int ZS = NT / B;
+    const int b = tid % B, zq = tid / B, xc = min(x0 + b, nxh - 1);
+    const T cn = (T)a.chi_n;
+    for (int z0 = zq; z0 < nz; z0 += 4 * ZS) {
+        V u[4], w[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int z = min(z0 + q * ZS, nz - 1);
+            u[q] = a.Uw[uidx(h, z, y, xc, nz, ny, nxh)];
+            w[q] = a.Wf[uidx(h, z, y, xc, nz, ny, nxh)];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int z = z0 + q * ZS;
+            if (z >= nz || b >= nb)
+                continue;
             const int zp = z < nz / 2 ? z : z + N - nz;
-            a.Uw[uidx(h, z, y, x0 + b, nz, ny, nxh)] = A[b * SCOL + zp + zp / N1];
+            V v = A[b * SCOL + zp + zp / N1];
+            if (z != nz / 2) {
+                const T lp = (T)a.lpf[(size_t)z * nxh + xc];
+                v.x -= cn * (u[q].x - lp * w[q].x);
+                v.y -= cn * (u[q].y - lp * w[q].y);
+            }
+            a.Uw[uidx(h, z, y, xc, nz, ny, nxh)] = v;
         }
     }
 }

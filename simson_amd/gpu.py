@@ -27,7 +27,7 @@ EXPORTS = (
     "simson_gpu_init", "simson_gpu_upload", "simson_gpu_prhs", "simson_gpu_nonlin",
     "simson_gpu_linear", "simson_gpu_substep",
     "simson_gpu_run", "simson_gpu_download", "simson_gpu_amp", "simson_gpu_pressure",
-    "simson_gpu_sync",
+    "simson_gpu_les", "simson_les_filter", "simson_gpu_sync",
     "simson_gpu_timing_enable", "simson_gpu_timing_read", "simson_gpu_last_error",
     "simson_gpu_finalize",
 )
@@ -40,6 +40,11 @@ class SimsonParams(C.Structure):
         ("re", C.c_double), ("xl", C.c_double), ("zl", C.c_double),
         ("rot", C.c_double), ("retau", C.c_double),
     ]
+
+
+class SimsonLes(C.Structure):
+    _fields_ = [("iles", C.c_int), ("iord", C.c_int), ("cutoff", C.c_double),
+                ("chi", C.c_double)]
 
 
 class SimsonError(RuntimeError):
@@ -74,6 +79,9 @@ def lib():
                                           C.POINTER(C.c_double)]
         L.simson_gpu_amp.argtypes = [vp, _dp]
         L.simson_gpu_pressure.argtypes = [vp, _dp, _dp]
+        L.simson_gpu_les.argtypes = [vp, C.POINTER(SimsonLes)]
+        L.simson_les_filter.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                                        _dp, _dp]
         L.simson_gpu_sync.argtypes = [vp]
         L.simson_gpu_timing_enable.argtypes = [vp, C.c_int]
         L.simson_gpu_timing_read.argtypes = [vp, C.POINTER(C.c_char_p),
@@ -193,6 +201,12 @@ class BlaGPU:
     def timing(self, on: bool):
         self._check(lib().simson_gpu_timing_enable(self.h, int(on)))
 
+    def les(self, iles: int = 1, cutoff: float = 2.0 / 3.0, iord: int = 5, chi: float = 0.2):
+        """ADM-RT relaxation model (sgs.i iles = 1; cutoff = 1/cutoff_inv),
+        or none with iles = 0."""
+        p = SimsonLes(iles, iord, cutoff, chi)
+        self._check(lib().simson_gpu_les(self.h, C.byref(p)))
+
     def amp(self) -> np.ndarray:
         """Per-y plane statistics of boxamp.f, [20][ny] (row i = amp(:, i+1))."""
         a = np.zeros((20, self.ny))
@@ -216,6 +230,17 @@ class BlaGPU:
         if n < 0:
             self._check(n)
         return {names[i].decode(): (ms[i], cnt[i]) for i in range(n) if cnt[i] > 0}
+
+
+def les_filter(nx: int, ny: int, nz: int, cutoff: float, iord: int):
+    """The library's ADM-RT filter operators (lpf [nz][nx/2], L [ny][ny]);
+    host computation, no device needed."""
+    lpf = np.zeros((nz, nx // 2))
+    L = np.zeros((ny, ny))
+    st = lib().simson_les_filter(nx, ny, nz, cutoff, iord, lpf, L)
+    if st != 0:
+        raise SimsonError(st, "invalid ADM-RT filter parameters")
+    return lpf, L
 
 
 def rk3_weights(dt: float):

@@ -4,6 +4,7 @@ Runs in the build container only (it needs /root/reference and amdflang):
 
     python tests/golden/make_golden.py            # c1_32x33x32.npz
     python tests/golden/make_golden.py pressure   # p_32x33x32.npz
+    python tests/golden/make_golden.py les        # les_32x33x32.npz
 
 1. builds the reference `bla` for 32x33x32 (serial, pressure off) with
    oracle/build_ref.sh into oracle/_ref/;
@@ -37,7 +38,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, REPO)
 
-from simson_amd import blai, fields, ufile  # noqa: E402
+from simson_amd import blai, fields, sgsi, ufile  # noqa: E402
 
 NX, NY, NZ = 32, 33, 32
 BIN = os.path.join(REPO, "oracle", "_ref", "bla_32x33x32_serial")
@@ -116,8 +117,67 @@ def pressure():
         shutil.rmtree(work)
 
 
+def les_field():
+    """The LES runs start from a field with every resolved wavenumber
+    excited (|kx|, |kz| <= 15, rms 1e-2, seed 5), so that the high-pass
+    filtered relaxation term moves v and w by about 1e-2 relative within 10
+    steps (with the reference defaults) and parity at 1e-12 pins it."""
+    return fields.channel_field(NX, NY, NZ, 4 * math.pi, 2 * math.pi, 3000.0,
+                                kmax=15, rms=1e-2, seed=5)
+
+
+LES_CASES = {
+    # sgs.i defaults of rparamles.f (ADM-RT: cutoff_inv 1.5, iord 5, chi 0.2),
+    # 100 steps, saves after 1, 10 and 100 steps
+    "a": (sgsi.SgsParams(iles=1, cutoff_inv=1.5, iord=5, chi=0.2), 100, [0.01, 0.1]),
+    # another filter order and cutoff with a stronger relaxation, 10 steps
+    "b": (sgsi.SgsParams(iles=1, cutoff_inv=3.0, iord=2, chi=2.0), 10, []),
+}
+
+
+def les():
+    """ADM-RT fixture: bla.i sets sgs, sgs.i selects iles = 1; the reference
+    binary is the serial 32x33x32 one (bla.f:27 compiles LES support in).
+    Keys: a_u1, a_u10, a_u100 (case a), b_u10 (case b), and the DNS field
+    dns_u10 of the same start without the model (it shows the size of the
+    model's effect)."""
+    if not os.path.exists(BIN):
+        subprocess.check_call([os.path.join(REPO, "oracle", "build_ref.sh"),
+                               str(NX), str(NY), str(NZ), "1", "no", "0",
+                               os.path.basename(BIN)])
+    res = {}
+    runs = [(k, sp, n, ts) for k, (sp, n, ts) in LES_CASES.items()] + [("dns", None, 10, [])]
+    for key, sp, nsteps, tsave in runs:
+        work = tempfile.mkdtemp(prefix="simson_golden_les.")
+        try:
+            ufile.write_u(os.path.join(work, "init.u"), les_field())
+            names = [f"t{round(t / 0.01)}.u" for t in tsave]
+            p = blai.BlaParams(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=3 * nsteps,
+                               dt=0.01, nst=3, xl=4 * math.pi, cflux=True, icfl=3,
+                               sgs=sp is not None, tsave=tsave, nmsave=names)
+            with open(os.path.join(work, "bla.i"), "w") as f:
+                f.write(blai.format_channel(p))
+            if sp is not None:
+                with open(os.path.join(work, "sgs.i"), "w") as f:
+                    f.write(sgsi.format_sgs(sp))
+            with open(os.path.join(work, "out.txt"), "w") as out:
+                subprocess.check_call([BIN], cwd=work, stdout=out, stderr=subprocess.STDOUT)
+            log = open(os.path.join(work, "out.txt")).read()
+            if "bla terminated normally" not in log:
+                raise RuntimeError(f"reference bla (LES {key}) did not terminate normally")
+            for name in names:
+                res[f"{key}_u{name[1:-2]}"] = ufile.read_u(os.path.join(work, name)).u
+            res[f"{key}_u{nsteps}"] = ufile.read_u(os.path.join(work, "end.u")).u
+        finally:
+            shutil.rmtree(work)
+    np.savez_compressed(os.path.join(HERE, "les_32x33x32.npz"), **res)
+    print("wrote", os.path.join(HERE, "les_32x33x32.npz"), sorted(res))
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "pressure":
         pressure()
+    elif len(sys.argv) > 1 and sys.argv[1] == "les":
+        les()
     else:
         main()
