@@ -7,7 +7,8 @@
 ! bla/rdiscbl.f:93-411 and bla/wdiscbl.f:50-136, the RK weight bookkeeping of
 ! bla/bla.f:1465-1477 and 2384-2397, the CFL stop of bla/wcflbl.f:87-98 and
 ! the stop criteria of bla/bla.f:2779-2833 and the amp.stat writer of
-! bla/wampbl.f (channel) -- and calls the library for prhs (bla.f:1562),
+! bla/wampbl.f (channel), the sgs.i reader of bla/rparamles.f (ADM-RT) --
+! and calls the library for prhs (bla.f:1562),
 ! the nonlinbl plane loop (bla.f:2251-2288), its plane statistics
 ! (boxamp.f, nonlinbl.f:430) and the linearbl plane loop (bla.f:2433-2475)
 ! through ISO_C_BINDING.
@@ -25,7 +26,7 @@ program blagpu
   integer :: mwave, npl, isfd, imhd, loctyp, wbci, i, it, isave, st
   real(dp) :: tmax, cpumax, wallmax, dt, cflmaxin, xl, rot, retau, cdev
   logical :: write_inter, varsiz, cflux, pert, cim, gall, suction, spat
-  logical :: sgs, tripf, longli, fileurms, pou, stopf
+  logical :: sgs, tripf, longli, fileurms, pou, stopf, docfl
   ! field and header
   integer :: nx, nyp, nzc, nfzsym, fltype
   real(dp) :: re, xlf, zl, t, xs, dstar
@@ -34,6 +35,9 @@ program blagpu
   real(dp) :: anrk(3), bnrk(3), dtn, dtnp1, an, bn, anp1, bnp1, cflp, cfl, px
   real(dp), allocatable :: amp(:,:), wint(:)
   type(simson_params_t) :: p
+  type(simson_les_t) :: les
+  integer :: sgsver
+  real(dp) :: cutoff_inv
   type(c_ptr) :: g
   integer(8) :: c0, c1, crate
 
@@ -95,7 +99,7 @@ program blagpu
   call nxt(line); read(line, *) mwave
   call nxt(line); read(line, *) npl
   close(10)
-  if (pert .or. cim .or. suction .or. spat .or. sgs .or. isfd /= 0 .or. imhd /= 0 .or. &
+  if (pert .or. cim .or. suction .or. spat .or. isfd /= 0 .or. imhd /= 0 .or. &
        loctyp /= 0 .or. tripf .or. wbci /= 0 .or. ixys > 0 .or. ibc /= 0 .or. nst /= 3 &
        .or. mwave > 0 .or. npl > 0 &
        .or. dt <= 0._dp) then
@@ -106,6 +110,27 @@ program blagpu
   if (icfl > 0 .and. icfl < nst) icfl = nst
   if (icfl > 0) icfl = min(icfl / nst * nst, 3 * nst)
   maxit = maxit / nst * nst
+
+  ! sgs.i (rparamles.f:23-165): only ADM-RT (iles = 1) or no model
+  if (sgs) then
+     les%iles = 1; cutoff_inv = 1.5_dp; les%iord = 5; les%chi = 0.2_dp
+     open(10, file='sgs.i', status='old')
+     call nxt(line); read(line, *) sgsver
+     if (sgsver /= 20060909) call stopnow(345432)
+     call nxt(line); read(line, *) les%iles
+     if (les%iles == 1) then
+        call nxt(line); read(line, *) cutoff_inv
+        call nxt(line); read(line, *) les%iord
+        call nxt(line); read(line, *) les%chi
+     else if (les%iles /= 0) then
+        write(*,*) 'blagpu: only the ADM-RT model (iles = 1) is supported'
+        call stopnow(45543)
+     end if
+     close(10)
+     les%cutoff = 1._dp / cutoff_inv
+     if (les%cutoff < 0._dp .or. les%cutoff > 1._dp .or. les%iord < 0 .or. les%iord > 5) &
+          call stopnow(45543)
+  end if
 
   ! ---------------- .u file (rdiscbl.f) ----------------
   open(12, file=trim(namnin), status='old', form='unformatted')
@@ -133,6 +158,10 @@ program blagpu
   p%cflux = merge(1, 0, cflux)
   p%re = re; p%xl = xl; p%zl = zl; p%rot = rot; p%retau = retau
   call chk(simson_gpu_init(p, g))
+  if (sgs) then
+     les%chi = les%chi / dstar                      ! chi_scaled, bla.f:2220
+     call chk(simson_gpu_les(g, les))
+  end if
   if (cflux) then
      px = -2._dp / re                             ! bla.f:1629-1630
   else
@@ -167,12 +196,17 @@ program blagpu
   do
      ! plane statistics at the start of every iamp-th step (nonlinbl.f:430)
      ! and the amp.stat line (bla.f:2326-2331)
-     if (iamp > 0 .and. mod(it - 1 - nst, iamp) == 0) then
-        call chk(simson_gpu_amp(g, amp))
-        call chk(simson_gpu_download(g, ur, ui, 0_c_int, c_null_ptr, px))
-        call write_amp(t, px)
+     ! (nested: Fortran's .and. need not short-circuit, and mod(., 0) traps)
+     if (iamp > 0) then
+        if (mod(it - 1 - nst, iamp) == 0) then
+           call chk(simson_gpu_amp(g, amp))
+           call chk(simson_gpu_download(g, ur, ui, 0_c_int, c_null_ptr, px))
+           call write_amp(t, px)
+        end if
      end if
-     if (icfl > 0 .and. mod(it - 1, icfl) == 0) then
+     docfl = .false.
+     if (icfl > 0) docfl = mod(it - 1, icfl) == 0
+     if (docfl) then
         call chk(simson_gpu_nonlin(g, 1_c_int, cflp))
         cfl = (cflp * pi + 2._dp * abs(rot)) * dtn   ! wcflbl.f:87
         write(*, '(a,f22.16,a,f22.16)') 'CFL*dt ', cfl, ' dt ', dtn / dstar

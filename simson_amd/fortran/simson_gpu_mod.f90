@@ -18,6 +18,14 @@ module simson_gpu_mod
      real(c_double) :: retau
   end type simson_params_t
 
+  ! ADM-RT LES parameters (sgs.i, rparamles.f)
+  type, bind(c) :: simson_les_t
+     integer(c_int) :: iles
+     integer(c_int) :: iord
+     real(c_double) :: cutoff
+     real(c_double) :: chi
+  end type simson_les_t
+
   interface
      integer(c_int) function simson_gpu_init(p, g) bind(c, name='simson_gpu_init')
        import :: c_int, c_ptr, simson_params_t
@@ -72,6 +80,12 @@ module simson_gpu_mod
        type(c_ptr), value :: g
        real(c_double), intent(out) :: amp(*)
      end function simson_gpu_amp
+
+     integer(c_int) function simson_gpu_les(g, les) bind(c, name='simson_gpu_les')
+       import :: c_int, c_ptr, simson_les_t
+       type(c_ptr), value :: g
+       type(simson_les_t), intent(in) :: les
+     end function simson_gpu_les
 
      subroutine simson_gpu_finalize(g) bind(c, name='simson_gpu_finalize')
        import :: c_ptr

@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 from conftest import ROOT
-from simson_amd import blai, fields, ufile
+from simson_amd import blai, fields, sgsi, ufile
 
 BLAGPU = os.path.join(ROOT, "simson_amd", "_lib", "blagpu")
 
@@ -66,14 +66,70 @@ def test_rejects_features_outside_the_library():
     work = _c1_dir()
     try:
         path = os.path.join(work, "bla.i")
-        # switch on the SGS model line (the step library has no LES closure)
+        # switch on constant wall suction (not part of the step library)
         lines = open(path).read().splitlines()
-        k = next(i for i, l in enumerate(lines) if "(sgs)" in l)
-        lines[k] = ".true.  SGS model (sgs)"
+        k = next(i for i, l in enumerate(lines) if "(suction)" in l)
+        lines[k] = ".true.  Constant wall-suction (suction)"
         open(path, "w").write("\n".join(lines) + "\n")
         r = _run(work)
         assert r.returncode != 0
         assert "outside the channel step library" in r.stdout
+    finally:
+        shutil.rmtree(work)
+
+
+def _les_dir(sp, **over):
+    """LES start field of tests/golden/les_32x33x32.npz, bla.i with sgs set
+    and the given sgs.i."""
+    work = tempfile.mkdtemp(prefix="blagpu_les.")
+    f = fields.channel_field(32, 33, 32, 4 * math.pi, 2 * math.pi, 3000.0,
+                             kmax=15, rms=1e-2, seed=5)
+    ufile.write_u(os.path.join(work, "init.u"), f)
+    kw = dict(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=30, dt=0.01, nst=3,
+              xl=4 * math.pi, cflux=True, icfl=3, sgs=True, tsave=[0.01], nmsave=["t1.u"])
+    kw.update(over)
+    with open(os.path.join(work, "bla.i"), "w") as fh:
+        fh.write(blai.format_channel(blai.BlaParams(**kw)))
+    with open(os.path.join(work, "sgs.i"), "w") as fh:
+        fh.write(sp if isinstance(sp, str) else sgsi.format_sgs(sp))
+    return work
+
+
+def test_rejects_unsupported_sgs_models():
+    """sgs.i is read before any device call: eddy-viscosity models (iles 2,
+    3) and out-of-range parameters stop with rparamles.f's code 45543, a
+    wrong version with 345432."""
+    _need_binary()
+    for text, code in (("20060909\n2\n0.\n1\n", "45543"),
+                       ("20060909\n1\n1.5\n7\n0.2\n", "45543"),
+                       ("20060101\n1\n", "345432")):
+        work = _les_dir(text)
+        try:
+            r = _run(work)
+            assert r.returncode != 0
+            assert code in r.stdout, r.stdout
+        finally:
+            shutil.rmtree(work)
+
+
+@pytest.mark.gpu
+def test_fortran_host_les_matches_reference_bla():
+    """The host with sgs.i (ADM-RT, the defaults of rparamles.f) writes the
+    fields the reference bla wrote (tests/golden/les_32x33x32.npz)."""
+    _need_binary()
+    print("[gpu] blagpu with sgs.i (ADM-RT) 32x33x32", flush=True)
+    golden = np.load(os.path.join(ROOT, "tests", "golden", "les_32x33x32.npz"))
+    work = _les_dir(sgsi.SgsParams(iles=1, cutoff_inv=1.5, iord=5, chi=0.2))
+    try:
+        r = _run(work)
+        assert r.returncode == 0, r.stdout + r.stderr
+        for name, key in (("t1.u", "a_u1"), ("end.u", "a_u10")):
+            u = np.delete(ufile.read_u(os.path.join(work, name)).u, 16, axis=1)
+            rr = np.delete(golden[key], 16, axis=1)
+            for c in range(3):
+                err = _rel(u[c], rr[c])
+                print(f"    {name} comp {c}: rel L2 {err:.2e}", flush=True)
+                assert err <= 1e-12, (name, c, err)
     finally:
         shutil.rmtree(work)
 
