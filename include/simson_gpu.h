@@ -148,6 +148,39 @@ int simson_gpu_les(simson_gpu_t *g, const simson_les_t *p);
  * hpf_y.f on the Gauss-Lobatto points.  Returns 0 or SIMSON_E_ARG. */
 int simson_les_filter(int nx, int ny, int nz, double cutoff, int iord, double *lpf, double *L);
 
+/* ---- Learned subgrid closure (no counterpart in the reference: it is the
+ * learned replacement of the tau_ij model of les.f / sgs_stress.f:62-131) ----
+ * A multilayer perceptron 10 -> 64 -> 64 -> 6 (GELU) maps the filtered
+ * velocity-gradient tensor d ubar_i/dx_j (9 values) and y+ to the six
+ * subgrid stresses tau_11, 12, 13, 22, 23, 33 (inputs and outputs
+ * standardised by the caller).  Training and inference run on the BF16
+ * matrix cores with FP32 accumulation; parameters and Adam state are FP32
+ * on the device.  Loss: mean squared error over points and outputs. */
+typedef struct simson_mlp simson_mlp_t;
+
+/* New network with seeded Glorot-uniform weights and zero biases. */
+int simson_mlp_create(unsigned long long seed, simson_mlp_t **out);
+/* Number of FP32 parameters (layout: W1 [16][64] (rows 10-15 zero), b1 [64],
+ * W2 [64][64], b2 [64], W3 [64][16] (columns 6-15 zero), b3 [16]). */
+int simson_mlp_nparams(void);
+/* Copy the parameters out (get) and/or in (set, which also resets Adam). */
+int simson_mlp_params(simson_mlp_t *h, float *get, const float *set);
+/* Keep a training set on the device: X [n][10], T [n][6] (rows shuffled by
+ * the caller; batches are consecutive tiles of 16 rows). */
+int simson_mlp_set_data(simson_mlp_t *h, const float *X, const float *T, int n);
+/* steps Adam steps (lr, beta 0.9 / 0.999) on batches of `batch` rows
+ * (a multiple of 16), cycling through the training set; losses (may be
+ * NULL) receives each step's batch loss.  Synchronises. */
+int simson_mlp_train(simson_mlp_t *h, int steps, int batch, float lr, float *losses);
+/* Gradient of the loss over the n rows of X [n][10], T [n][6] (host
+ * arrays) with respect to the parameters, and the loss; no update. */
+int simson_mlp_grad(simson_mlp_t *h, const float *X, const float *T, int n, float *grad,
+                    float *loss);
+/* Y [n][6] = network(X [n][10]) (host arrays). */
+int simson_mlp_infer(simson_mlp_t *h, const float *X, int n, float *Y);
+const char *simson_mlp_last_error(const simson_mlp_t *h);
+void simson_mlp_destroy(simson_mlp_t *h);
+
 /* Wait for all queued work. */
 int simson_gpu_sync(simson_gpu_t *g);
 

@@ -28,6 +28,9 @@ EXPORTS = (
     "simson_gpu_linear", "simson_gpu_substep",
     "simson_gpu_run", "simson_gpu_download", "simson_gpu_amp", "simson_gpu_pressure",
     "simson_gpu_les", "simson_les_filter", "simson_gpu_sync",
+    "simson_mlp_create", "simson_mlp_nparams", "simson_mlp_params", "simson_mlp_set_data",
+    "simson_mlp_train", "simson_mlp_grad", "simson_mlp_infer", "simson_mlp_last_error",
+    "simson_mlp_destroy",
     "simson_gpu_timing_enable", "simson_gpu_timing_read", "simson_gpu_last_error",
     "simson_gpu_finalize",
 )
