@@ -181,6 +181,21 @@ int simson_mlp_infer(simson_mlp_t *h, const float *X, int n, float *Y);
 const char *simson_mlp_last_error(const simson_mlp_t *h);
 void simson_mlp_destroy(simson_mlp_t *h);
 
+/* Use a trained closure network inside the step, in place of the subgrid
+ * model of les.f: at every substep the nine filtered velocity gradients
+ * d u_i/dx_j (from u, v, w and the vorticities) and y+ of every point of the
+ * dealiased x-z grid go through the network, and the divergence of its
+ * stresses, sigma_i = d tau_ij/dx_j (sgs_stress.f:62-131), is subtracted
+ * from H_i as nonlinbl.f:1011-1050 subtracts the eddy-viscosity force.
+ * params: the network (simson_mlp_params layout), NULL turns the closure off.
+ * Features: x_k = g_k in_scale[k] + in_shift[k] for the gradients (k < 9),
+ * yfeat[y] for y+ (already standardised); stresses tau_o = net_o
+ * out_scale[o] + out_shift[o] (o: 11 12 13 22 23 33).  Turns ADM-RT off.
+ * Needs the specialised FFT lengths and ny <= 128. */
+int simson_gpu_closure(simson_gpu_t *g, const float *params, const double *in_scale,
+                       const double *in_shift, const double *yfeat, const double *out_scale,
+                       const double *out_shift);
+
 /* Wait for all queued work. */
 int simson_gpu_sync(simson_gpu_t *g);
 

@@ -14,6 +14,7 @@ from __future__ import annotations
 import math
 
 import numpy as np
+from scipy.special import erf as _erf
 
 NIN, NOUT, KI, NH, KO = 10, 6, 16, 64, 16
 OW1 = 0
@@ -23,9 +24,6 @@ OB2 = OW2 + NH * NH
 OW3 = OB2 + NH
 OB3 = OW3 + NH * KO
 NPAR = OB3 + KO
-
-_erf = np.vectorize(math.erf)
-
 
 def bf16(x: np.ndarray) -> np.ndarray:
     u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)

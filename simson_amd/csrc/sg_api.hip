@@ -15,9 +15,11 @@ namespace {
 
 const char *const kFamilies[] = {"zsyn", "xrow", "zana", "cfl_reduce", "cheb_fwd",
                                  "tau", "cheb_bwd", "prhs_cheb_fwd", "prhs",
-                                 "prhs_cheb_bwd", "tau_factor", "les_filter"};
+                                 "prhs_cheb_bwd", "tau_factor", "les_filter",
+                                 "closure_zsyn", "closure_xrow", "closure_zana",
+                                 "closure_dy"};
 enum Fam { F_ZSYN, F_XROW, F_ZANA, F_CFL, F_CHF, F_TAU, F_CHB, F_PCHF, F_PRHS, F_PCHB,
-           F_FAC, F_LES, F_N };
+           F_FAC, F_LES, F_CZS, F_CXR, F_CZA, F_CDY, F_N };
 
 struct TimedLaunch {
     int fam;
@@ -67,6 +69,17 @@ struct simson_gpu {
     float *Lm32 = nullptr;
     double2 *Wf = nullptr;
     float2 *Wf32 = nullptr;
+    // learned closure (simson_gpu_closure): network parameters and feature
+    // maps, the nine gradient fields and six stresses on the padded x-z grid
+    // (PG [9][ny][nzp][nxh]), the stress spectra TAU [6][nz][ny][nxh], their
+    // y derivatives DT [3][...] and the y differentiation matrix Dm [ny][ny]
+    bool clos = false;
+    ClosArgs cl{};
+    float *clW = nullptr, *clY = nullptr;
+    double2 *PG = nullptr, *TAU = nullptr, *DT = nullptr;
+    float2 *PG32 = nullptr, *TAU32 = nullptr, *DT32 = nullptr;
+    double *Dm = nullptr;
+    float *Dm32 = nullptr;
     FftPlan pz, pm;
     double mflux = 0., u0upp = 0., u0low = 0., w0upp = 0., w0low = 0.;
     bool timing = false;
@@ -158,7 +171,8 @@ void free_all(simson_gpu_t *g)
                     g->lam[1].gp, g->lam[1].vh, g->lam[1].sh,
                     g->lam[2].gp, g->lam[2].vh, g->lam[2].sh, g->U32, g->P32, g->C32,
                     g->twz32, g->twx32, g->Tf32, g->Tb32, g->Q, g->Qc, g->PR, g->pscr,
-                    g->Lm, g->lpf, g->Lm32, g->Wf, g->Wf32};
+                    g->Lm, g->lpf, g->Lm32, g->Wf, g->Wf32, g->clW, g->clY, g->PG, g->TAU,
+                    g->DT, g->PG32, g->TAU32, g->DT32, g->Dm, g->Dm32};
     for (void *q : ptrs)
         if (q)
             (void)hipFree(q);
@@ -212,6 +226,9 @@ NonlinArgs nonlin_args(simson_gpu_t *g, int do_cfl)
     a.Wf = nullptr;
     a.lpf = g->lpf;
     a.chi_n = g->les_chi * (double)g->nxp * (double)g->nzp;
+    a.clos = 0;
+    a.TAU = nullptr;
+    a.DT = nullptr;
     return a;
 }
 
@@ -310,7 +327,31 @@ Step2Args<float2> nonlin_args32(simson_gpu_t *g, int do_cfl)
     a.Wf = nullptr;
     a.lpf = d.lpf;
     a.chi_n = d.chi_n;
+    a.clos = 0;
+    a.TAU = nullptr;
+    a.DT = nullptr;
     return a;
+}
+
+// Learned closure: stresses of the current velocities and their y
+// derivatives (the tau_ij of les.f and the first half of sgs_stress.f)
+template <typename V>
+void closure_stresses(simson_gpu_t *g, Step2Args<V> c, V *PG, V *TAU, V *DT, const V *twz,
+                      const V *twx, const void *Dm, bool fp32)
+{
+    c.P = PG;
+    c.Uw = TAU;
+    c.split_mean = 0;
+    timed(g, F_CZS, [&] { launch_gsyn4(c, twz, g->stream); });
+    timed(g, F_CXR, [&] { launch_xclos(c, twx, g->cl, g->stream); });
+    timed(g, F_CZA, [&] { launch_zana4(c, twz, g->stream, 6); });
+    GemmIO io = gemm_io(g, TAU, 0, DT, 0, {1, 3, 4}, {0, 1, 2}, g->nz / 2);
+    timed(g, F_CDY, [&] {
+        if (fp32)
+            launch_cheb_mfma_f32((const float *)Dm, (const float *)TAU, (float *)DT, io, g->stream);
+        else
+            launch_cheb_gemm((const double *)Dm, io, g->stream);
+    });
 }
 
 size_t field_size(const simson_gpu_t *g) { return (size_t)g->nz * g->ny * g->nxh; }
@@ -353,6 +394,13 @@ int do_nonlin(simson_gpu_t *g, int do_cfl)
             a.les = 1;
             a.Wf = g->Wf32;
         }
+        if (g->clos) {
+            closure_stresses<float2>(g, a, g->PG32, g->TAU32, g->DT32, g->twz32, g->twx32,
+                                     g->Dm32, true);
+            a.clos = 1;
+            a.TAU = g->TAU32;
+            a.DT = g->DT32;
+        }
         timed(g, F_ZSYN, [&] { launch_zsyn4(a, g->twz32, g->stream); });
         timed(g, F_XROW, [&] { launch_xrow4(a, g->twx32, g->stream); });
         if (do_cfl)
@@ -371,6 +419,13 @@ int do_nonlin(simson_gpu_t *g, int do_cfl)
         timed(g, F_LES, [&] { launch_cheb_gemm(g->Lm, io, g->stream); });
         a.les = 1;
         a.Wf = g->Wf;
+    }
+    if (g->clos) {
+        closure_stresses<double2>(g, a, g->PG, g->TAU, g->DT, (const double2 *)g->twz,
+                                  (const double2 *)g->twx, g->Dm, false);
+        a.clos = 1;
+        a.TAU = g->TAU;
+        a.DT = g->DT;
     }
     timed(g, F_ZSYN, [&] {
         if (!launch_zsyn4(a, (const double2 *)g->twz, g->stream))
@@ -870,6 +925,82 @@ int simson_gpu_les(simson_gpu_t *g, const simson_les_t *p)
     }
     g->les_chi = p->chi;
     g->les = true;
+    g->clos = false;
+    return 0;
+}
+
+int simson_gpu_closure(simson_gpu_t *g, const float *params, const double *in_scale,
+                       const double *in_shift, const double *yfeat, const double *out_scale,
+                       const double *out_shift)
+{
+    if (!g)
+        return SIMSON_E_ARG;
+    if (!params) {
+        g->clos = false;
+        return 0;
+    }
+    if (!in_scale || !in_shift || !yfeat || !out_scale || !out_shift)
+        return SIMSON_E_ARG;
+    if (!has_zfft4(g->nzp) || !has_xrow4(g->nxp / 2) || cheb_mfma_ks(g->ny) == 0) {
+        g->err = "the learned closure needs the specialised FFT lengths and ny <= 128";
+        return SIMSON_E_ARG;
+    }
+    const int npar = simson_mlp_nparams(), ny = g->ny;
+    if (!g->clW) {
+        const size_t npg = 9 * (size_t)ny * g->nzp * g->nxh, nt = 6 * field_size(g),
+                     nd = 3 * field_size(g);
+        if (dalloc(g, &g->clW, (size_t)npar) || dalloc(g, &g->clY, (size_t)ny) ||
+            dalloc(g, &g->Dm, (size_t)ny * ny))
+            return SIMSON_E_HIP;
+        if (g->mixed ? (dalloc(g, &g->PG32, npg) || dalloc(g, &g->TAU32, nt) ||
+                        dalloc(g, &g->DT32, nd) || dalloc(g, &g->Dm32, (size_t)ny * ny))
+                     : (dalloc(g, &g->PG, npg) || dalloc(g, &g->TAU, nt) || dalloc(g, &g->DT, nd)))
+            return SIMSON_E_HIP;
+        // d/dy on the Gauss-Lobatto points y_j = cos(pi j/(ny-1)): the
+        // physical-space form of vchbf, dcheb, vchbb (sgs_stress.f:98-121)
+        std::vector<double> D((size_t)ny * ny), y(ny), c(ny);
+        for (int j = 0; j < ny; j++) {
+            y[j] = cos(M_PI * j / (ny - 1));
+            c[j] = ((j == 0 || j == ny - 1) ? 2. : 1.) * ((j & 1) ? -1. : 1.);
+        }
+        for (int i = 0; i < ny; i++) {
+            double sum = 0.;
+            for (int j = 0; j < ny; j++)
+                if (i != j) {
+                    D[(size_t)i * ny + j] = c[i] / c[j] / (y[i] - y[j]);
+                    sum += D[(size_t)i * ny + j];
+                }
+            D[(size_t)i * ny + i] = -sum;
+        }
+        if (h2d(g, g->Dm, D.data(), D.size()))
+            return SIMSON_E_HIP;
+        if (g->mixed) {
+            std::vector<float> D32(D.begin(), D.end());
+            if (h2d(g, g->Dm32, D32.data(), D32.size()))
+                return SIMSON_E_HIP;
+        }
+    }
+    std::vector<float> yf(yfeat, yfeat + ny);
+    if (h2d(g, g->clW, params, (size_t)npar) || h2d(g, g->clY, yf.data(), yf.size()))
+        return SIMSON_E_HIP;
+    ClosArgs &c = g->cl;
+    c.W = g->clW;
+    c.yfeat = g->clY;
+    c.ob1 = 16 * 64;
+    c.ow2 = c.ob1 + 64;
+    c.ob2 = c.ow2 + 64 * 64;
+    c.ow3 = c.ob2 + 64;
+    c.ob3 = c.ow3 + 64 * 16;
+    for (int k = 0; k < 9; k++) {
+        c.in_a[k] = (float)in_scale[k];
+        c.in_b[k] = (float)in_shift[k];
+    }
+    for (int o = 0; o < 6; o++) {
+        c.out_a[o] = (float)out_scale[o];
+        c.out_b[o] = (float)out_shift[o];
+    }
+    g->clos = true;
+    g->les = false;
     return 0;
 }
 

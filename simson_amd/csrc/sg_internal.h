@@ -95,8 +95,32 @@ struct Step2Args {
     const V *Wf;
     const double *lpf;
     double chi_n;
+    // learned closure (sgs_stress.f:62-131 with the network's stresses):
+    // H_i -= sigma_i = i alpha tau_i1 + d tau_i2/dy + i beta tau_i3, from the
+    // unnormalised stress spectra TAU [6][nz][ny][nxh] (11 12 13 22 23 33)
+    // and DT = d/dy of tau_12, tau_22, tau_23 [3][nz][ny][nxh]
+    int clos;
+    const V *TAU, *DT;
 };
 typedef Step2Args<double2> NonlinArgs;
+
+// Learned closure evaluated on the x rows (sg_xrow.hip): network parameters
+// (FP32, the simson_mlp layout with the offsets below), the affine maps of the
+// nine gradient features (x_n = g in_a + in_b), the y+ feature per y plane
+// (standardised), and the map of the six outputs (tau = y_n out_a + out_b)
+struct ClosArgs {
+    const float *W;
+    int ow2, ow3, ob1, ob2, ob3;
+    float in_a[9], in_b[9];
+    const float *yfeat;
+    float out_a[6], out_b[6];
+};
+template <typename V>
+bool launch_xclos(const Step2Args<V> &a, const V *wx, const ClosArgs &c, hipStream_t s);
+// the nine velocity-gradient fields d u_i / d x_j of a y plane, padded and
+// synthesised in z, into a.P as [9][ny][nzp][nx/2]
+template <typename V>
+bool launch_gsyn4(const Step2Args<V> &a, const V *tw, hipStream_t s);
 
 // ADM-RT filter operators (host): lpf [nz][nx/2] and L = I - H_y^(iord+1)
 // [ny][ny]; 0 or SIMSON_E_ARG

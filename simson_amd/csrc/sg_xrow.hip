@@ -21,6 +21,8 @@
 // Pass 2, the products and pass A touch only one block per task and run
 // fused in registers; each pass reads and writes the same LDS slots, so one
 // buffer serves all passes.  Rows are padded by one slot per M1 block.
+#include <cstdint>
+
 #include "sg_cplx.h"
 #include "sg_internal.h"
 
@@ -351,6 +353,261 @@ void xrow4(const Step2Args<V> &a, const V *tw, hipStream_t s)
     hipLaunchKernelGGL((k_xrow4<V, M, PRESS>), grid, dim3(B::NT), B::lds, s, a, tw);
 }
 
+// ---------------------------------------------------------------------
+// Learned closure (sgs network) on the x rows: the nine velocity-gradient
+// fields of a (y, zp) block are synthesised in x, the network maps the
+// gradients and y+ of every physical point to the six subgrid stresses on
+// the BF16 matrix cores, and the stresses are transformed back.  Fields in
+// a.P: [9][ny][nzp][nx/2] in, stresses into fields 0-5 of the same rows.
+// ---------------------------------------------------------------------
+
+template <typename V, int M>
+struct CBlock {
+    static constexpr int NF = 9;
+    static constexpr int budget = 52 * 1024;
+    static constexpr int per_row = NF * XPlan<M>::SROW * (int)sizeof(V);
+    static constexpr int raw = (budget - XPlan<M>::TABN * (int)sizeof(V)) / per_row;
+    static constexpr int RW = raw < 1 ? 1 : (raw > 8 ? 8 : raw);
+    static constexpr int tasks1 = NF * RW * XPlan<M>::M1;
+    static constexpr int tasks2 = NF * RW * XPlan<M>::M2;
+    static constexpr int tmax = tasks1 > tasks2 ? tasks1 : tasks2;
+    static constexpr int NT = tmax >= 256 ? 256 : ((tmax + 63) / 64) * 64;
+    static constexpr size_t lds = (size_t)(XPlan<M>::TABN + NF * RW * XPlan<M>::SROW) * sizeof(V);
+};
+
+typedef short cs4 __attribute__((ext_vector_type(4)));
+typedef float cf4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short to_bf16(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+__device__ __forceinline__ float gelu_f(float z)
+{
+    return 0.5f * z * (1.f + erff(z * 0.70710678118654752f));
+}
+
+template <typename V, int M>
+__global__ void __launch_bounds__((CBlock<V, M>::NT))
+    k_xclos(Step2Args<V> a, const V *__restrict__ twg, ClosArgs c)
+{
+    typedef typename ScalarOf<V>::T T;
+    constexpr int M1 = XPlan<M>::M1, M2 = XPlan<M>::M2, SROW = XPlan<M>::SROW;
+    constexpr int TABN = XPlan<M>::TABN, NF = CBlock<V, M>::NF;
+    constexpr int RW = CBlock<V, M>::RW, NT = CBlock<V, M>::NT;
+    constexpr int S1 = M1 + 1, NXH = 2 * M / 3, FS = RW * SROW;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    V *tw = reinterpret_cast<V *>(lds_raw);
+    V *A = tw + TABN;                            // [NF][RW][SROW]
+    const int nzp = a.nzp, ny = a.ny;
+    const int zp0 = blockIdx.x * RW, y = blockIdx.y;
+    const int nr = min(RW, nzp - zp0);
+    const int tid = threadIdx.x;
+    for (int i = tid; i < TABN; i += NT)
+        tw[i] = twg[i];
+    __syncthreads();
+    // half spectra -> packed Z_k (as k_xrow4, for NF fields)
+    for (int idx = tid; idx < NF * RW * M; idx += NT) {
+        const int f = idx / (RW * M);
+        const int rem = idx - f * RW * M;
+        const int r = rem / M, k = rem - r * M;
+        V Z = vmk<V>(0, 0);
+        if (r < nr) {
+            const V *row = a.P + (((size_t)f * ny + y) * nzp + zp0 + r) * NXH;
+            V Xk = k < NXH ? row[k] : vmk<V>(0, 0);
+            if (k == 0)
+                Xk.y = 0;
+            const int km = M - k;
+            V Xm = (k > 0 && km < NXH) ? row[km] : vmk<V>(0, 0);
+            Xm.y = -Xm.y;
+            const V sm = vadd(Xk, Xm), d = vsub(Xk, Xm);
+            Z = vadd(sm, vmul_is(vmul(tw[k], d), (T)1));
+        }
+        A[(f * RW + r) * SROW + k + k / M1] = Z;
+    }
+    __syncthreads();
+    // inverse pass 1 (columns) and pass 2 (blocks) of every field
+    for (int t = tid; t < NF * RW * M1; t += NT) {
+        const int fr = t / M1, n1 = t - fr * M1;
+        V *row = A + fr * SROW + n1;
+        V v[M2];
+#pragma unroll
+        for (int n2 = 0; n2 < M2; n2++)
+            v[n2] = row[S1 * n2];
+        reg_dft<V, M2, +1, TABN>(v, tw);
+#pragma unroll
+        for (int k2 = 1; k2 < M2; k2++)
+            if (n1 > 0)
+                v[k2] = vmul(v[k2], tw[2 * n1 * k2]);
+#pragma unroll
+        for (int k2 = 0; k2 < M2; k2++)
+            row[S1 * k2] = v[k2];
+    }
+    __syncthreads();
+    for (int t = tid; t < NF * RW * M2; t += NT) {
+        const int fr = t / M2, k2 = t - fr * M2;
+        V *blk = A + fr * SROW + S1 * k2;
+        V v[M1];
+#pragma unroll
+        for (int i = 0; i < M1; i++)
+            v[i] = blk[i];
+        reg_dft<V, M1, +1, TABN>(v, tw);
+#pragma unroll
+        for (int i = 0; i < M1; i++)
+            blk[i] = v[i];
+    }
+    __syncthreads();
+    // the network on tiles of 16 physical points (8 packed positions, re and
+    // im); features-as-rows products: Z^T = W^T X^T, so each layer's result
+    // (lane l: unit 4(l/16)+r of point l%16) is the next layer's B operand
+    {
+        const int l = tid & 63, wv = tid >> 6, nwv = NT / 64;
+        const int lr = l & 15, lg = l >> 4;
+        cs4 w1[4], w2[4][4], w3[4];
+        float b1[4][4], b2[4][4], b3[4];
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                w1[mb][j] = to_bf16(c.W[(4 * lg + j) * 64 + 16 * mb + lr]);
+                w3[mb][j] = to_bf16(c.W[c.ow3 + (16 * mb + 4 * lg + j) * 16 + lr]);
+            }
+#pragma unroll
+            for (int kb = 0; kb < 4; kb++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    w2[mb][kb][j] = to_bf16(c.W[c.ow2 + (16 * kb + 4 * lg + j) * 64 + 16 * mb + lr]);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                b1[mb][r] = c.W[c.ob1 + 16 * mb + 4 * lg + r];
+                b2[mb][r] = c.W[c.ob2 + 16 * mb + 4 * lg + r];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            b3[r] = c.W[c.ob3 + 4 * lg + r];
+        const float yf = c.yfeat[y];
+        constexpr int NTILE = RW * M / 8;
+        for (int t = wv; t < NTILE; t += nwv) {
+            const int r = t / (M / 8), pos = 8 * (t % (M / 8)) + (lr >> 1);
+            const int off = r * SROW + pos + pos / M1;
+            const bool im = lr & 1;
+            cs4 xb;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int k = 4 * lg + j;
+                float x = 0.f;
+                if (k < 9) {
+                    const V v = A[k * FS + off];
+                    x = (float)(im ? v.y : v.x) * c.in_a[k] + c.in_b[k];
+                } else if (k == 9) {
+                    x = yf;
+                }
+                xb[j] = to_bf16(x);
+            }
+            cs4 h1[4], h2[4];
+#pragma unroll
+            for (int mb = 0; mb < 4; mb++) {
+                cf4 z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1[mb], xb, cf4{0.f, 0.f, 0.f, 0.f},
+                                                                 0, 0, 0);
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    h1[mb][q] = to_bf16(gelu_f(z[q] + b1[mb][q]));
+            }
+#pragma unroll
+            for (int mb = 0; mb < 4; mb++) {
+                cf4 z = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kb = 0; kb < 4; kb++)
+                    z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2[mb][kb], h1[kb], z, 0, 0, 0);
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    h2[mb][q] = to_bf16(gelu_f(z[q] + b2[mb][q]));
+            }
+            cf4 yo = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kb = 0; kb < 4; kb++)
+                yo = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3[kb], h2[kb], yo, 0, 0, 0);
+            // every lane read its inputs before any lane of the wave writes
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int o = 4 * lg + q;
+                if (o < 6) {
+                    const T tau = (T)((yo[q] + b3[q]) * c.out_a[o] + c.out_b[o]);
+                    V &dst = A[o * FS + off];
+                    if (im)
+                        dst.y = tau;
+                    else
+                        dst.x = tau;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // forward pass A (blocks) and pass B (columns) of the six stresses
+    for (int t = tid; t < 6 * RW * M2; t += NT) {
+        const int fr = t / M2, k2 = t - fr * M2;
+        V *blk = A + fr * SROW + S1 * k2;
+        V v[M1];
+#pragma unroll
+        for (int i = 0; i < M1; i++)
+            v[i] = blk[i];
+        reg_dft<V, M1, -1, TABN>(v, tw);
+#pragma unroll
+        for (int cc = 0; cc < M1; cc++) {
+            if (cc > 0 && k2 > 0) {
+                V wt = tw[2 * k2 * cc];
+                wt.y = -wt.y;
+                v[cc] = vmul(v[cc], wt);
+            }
+            blk[cc] = v[cc];
+        }
+    }
+    __syncthreads();
+    for (int t = tid; t < 6 * RW * M1; t += NT) {
+        const int hr = t / M1, cc = t - hr * M1;
+        V *row = A + hr * SROW + cc;
+        V v[M2];
+#pragma unroll
+        for (int n2 = 0; n2 < M2; n2++)
+            v[n2] = row[S1 * n2];
+        reg_dft<V, M2, -1, TABN>(v, tw);
+#pragma unroll
+        for (int d = 0; d < M2; d++)
+            row[S1 * d] = v[d];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < 6 * RW * NXH; idx += NT) {
+        const int h = idx / (RW * NXH);
+        const int rem = idx - h * RW * NXH;
+        const int r = rem / NXH, k = rem - r * NXH;
+        if (r >= nr)
+            continue;
+        const V *row = A + (h * RW + r) * SROW;
+        const int km = k == 0 ? 0 : M - k;
+        const V Zk = row[k + k / M1];
+        const V Zm = vconj(row[km + km / M1]);
+        const V E = vmk<V>((T)0.5 * (Zk.x + Zm.x), (T)0.5 * (Zk.y + Zm.y));
+        const V dd = vsub(Zk, Zm);
+        const V O = vmk<V>((T)0.5 * dd.y, (T)-0.5 * dd.x);
+        a.P[(((size_t)h * ny + y) * nzp + zp0 + r) * NXH + k] = vadd(E, vmul(vconj(tw[k]), O));
+    }
+}
+
+template <typename V, int M>
+void xclos(const Step2Args<V> &a, const V *tw, const ClosArgs &c, hipStream_t s)
+{
+    using B = CBlock<V, M>;
+    dim3 grid((a.nzp + B::RW - 1) / B::RW, a.ny);
+    hipLaunchKernelGGL((k_xclos<V, M>), grid, dim3(B::NT), B::lds, s, a, tw, c);
+}
+
 }  // namespace
 
 #define SG_X4LENS(X) X(12) X(24) X(48) X(96) X(192) X(288)
@@ -408,6 +665,25 @@ bool launch_xrow4_press(const Step2Args<double2> &a, const double2 *wx, hipStrea
     }
 }
 
+template <typename V>
+bool launch_xclos(const Step2Args<V> &a, const V *wx, const ClosArgs &c, hipStream_t s)
+{
+    switch (a.nxp / 2) {
+#define SG_CASE(N)                      \
+    case N:                             \
+        xclos<V, N>(a, wx, c, s);       \
+        return true;
+        SG_X4LENS(SG_CASE)
+#undef SG_CASE
+    default:
+        return false;
+    }
+}
+
+template bool launch_xclos<double2>(const Step2Args<double2> &, const double2 *, const ClosArgs &,
+                                    hipStream_t);
+template bool launch_xclos<float2>(const Step2Args<float2> &, const float2 *, const ClosArgs &,
+                                   hipStream_t);
 template bool launch_xrow4<double2>(const Step2Args<double2> &, const double2 *, hipStream_t);
 template bool launch_xrow4<float2>(const Step2Args<float2> &, const float2 *, hipStream_t);
 template int xrow4_blocks<double2>(const Step2Args<double2> &);

@@ -62,7 +62,33 @@ __device__ __forceinline__ int dr_off(int zp)
     return (N1 + 1) * k2 + k1;
 }
 
-template <typename V, int N>
+// gather of the velocity-gradient field f = 3 i + j (d u_i / d x_j) from
+// the storage slots u, v, w, om_x, om_z: x and z derivatives are i alpha,
+// i beta; the y derivatives follow from the vorticities and continuity,
+// du/dy = i alpha v - om_z, dv/dy = -(i alpha u + i beta w),
+// dw/dy = om_x + i beta v.  Term t of field f: slot, multiplier (0: 1,
+// 1: i alpha, 2: i beta) and sign; slot -1: no second term.
+struct GradTerm {
+    signed char s1, m1, g1, s2, m2, g2;
+};
+__constant__ GradTerm kGrad[9] = {
+    {0, 1, 1, -1, 0, 0}, {1, 1, 1, 4, 0, -1}, {0, 2, 1, -1, 0, 0},
+    {1, 1, 1, -1, 0, 0}, {0, 1, -1, 2, 2, -1}, {1, 2, 1, -1, 0, 0},
+    {2, 1, 1, -1, 0, 0}, {3, 0, 1, 1, 2, 1},  {2, 2, 1, -1, 0, 0}};
+
+template <typename V>
+__device__ __forceinline__ V grad_mul(V v, int m, int sg, typename ScalarOf<V>::T al,
+                                      typename ScalarOf<V>::T be)
+{
+    typedef typename ScalarOf<V>::T T;
+    const T k = m == 1 ? al : be;
+    V r = m == 0 ? v : vmk<V>(-k * v.y, k * v.x);
+    return sg < 0 ? vmk<V>(-r.x, -r.y) : r;
+}
+
+// MODE 0: the six step-2 fields u, v, w, om_x, om_y, om_z; MODE 1: the nine
+// velocity-gradient fields of the learned closure (into a.P as [9][ny][N][nxh])
+template <typename V, int N, int MODE = 0>
 __global__ void __launch_bounds__((ZPlan<V, N>::NT)) k_zsyn4(Step2Args<V> a, const V *__restrict__ twg)
 {
     typedef typename ScalarOf<V>::T T;
@@ -84,7 +110,9 @@ __global__ void __launch_bounds__((ZPlan<V, N>::NT)) k_zsyn4(Step2Args<V> a, con
     const int b = tid % B, zq = tid / B;
     const int xc = min(x0 + b, nxh - 1);
     const T al = (T)a.alfa[xc];
-    const bool om_y = f == 4;                   // block-uniform
+    const bool om_y = MODE == 0 && f == 4;      // block-uniform
+    const GradTerm gt = MODE == 1 ? kGrad[f] : GradTerm{0, 0, 0, -1, 0, 0};
+    const bool two = MODE == 1 ? gt.s2 >= 0 : om_y;
 #pragma unroll
     for (int j0 = 0; j0 < NI; j0 += NBT) {
         V u[NBT], w[NBT];
@@ -94,12 +122,14 @@ __global__ void __launch_bounds__((ZPlan<V, N>::NT)) k_zsyn4(Step2Args<V> a, con
             const int zp = min(zq + ZS * (j0 + q), N - 1);
             // storage row of padded mode zp; rows nz/2 .. N-nz/2 stay zero
             const int z = zp < nz / 2 ? zp : (zp > N - nz / 2 ? zp - (N - nz) : 0);
-            // fields u, v, w, om_x, om_y, om_z from storage slots 0-3, (0, 2), 4
-            u[q] = a.U[uidx(om_y ? 0 : (f < 4 ? f : 4), z, y, xc, nz, ny, nxh)];
-            if (om_y) {
-                w[q] = a.U[uidx(2, z, y, xc, nz, ny, nxh)];
-                bt[q] = (T)a.beta[z];
-            }
+            // MODE 0: fields u, v, w, om_x, om_y, om_z from storage slots
+            // 0-3, (0, 2), 4
+            const int s1 = MODE == 1 ? gt.s1 : (om_y ? 0 : (f < 4 ? f : 4));
+            const int s2 = MODE == 1 ? gt.s2 : 2;
+            u[q] = a.U[uidx(s1, z, y, xc, nz, ny, nxh)];
+            if (two)
+                w[q] = a.U[uidx(s2, z, y, xc, nz, ny, nxh)];
+            bt[q] = (T)a.beta[z];
         }
 #pragma unroll
         for (int q = 0; q < NBT; q++) {
@@ -108,10 +138,18 @@ __global__ void __launch_bounds__((ZPlan<V, N>::NT)) k_zsyn4(Step2Args<V> a, con
                 continue;
             const int z = zp < nz / 2 ? zp : (zp > N - nz / 2 ? zp - (N - nz) : -1);
             V v = u[q];
-            if (om_y)
+            if (MODE == 1) {
+                v = grad_mul(u[q], gt.m1, gt.g1, al, bt[q]);
+                if (two) {
+                    const V v2 = grad_mul(w[q], gt.m2, gt.g2, al, bt[q]);
+                    v = vadd(v, v2);
+                }
+            } else if (om_y) {
                 v = vmk<V>(-bt[q] * u[q].y + al * w[q].y, bt[q] * u[q].x - al * w[q].x);
-            // with split_mean the kx = kz = 0 mode enters the products directly
-            if (b >= nb || z < 0 || (a.split_mean && z == 0 && x0 + b == 0))
+            }
+            // with split_mean the kx = kz = 0 mode enters the products
+            // directly (step-2 fields only)
+            if (b >= nb || z < 0 || (MODE == 0 && a.split_mean && z == 0 && x0 + b == 0))
                 v = vmk<V>(0, 0);
             A[b * SCOL + zp + zp / N1] = v;
         }
@@ -221,7 +259,7 @@ __global__ void __launch_bounds__((ZPlan<V, N>::NT)) k_zana4(Step2Args<V> a, con
             col[S1 * d] = v[d];
     }
     __syncthreads();
-    if (!a.les) {
+    if (!a.les && !a.clos) {
         for (int idx = tid; idx < nz * B; idx += NT) {
             const int z = idx / B, b = idx - z * B;
             if (b < nb) {
@@ -231,19 +269,31 @@ __global__ void __launch_bounds__((ZPlan<V, N>::NT)) k_zana4(Step2Args<V> a, con
         }
         return;
     }
-    // with the ADM-RT relaxation: thread column b = tid % B, rows
-    // z = tid / B + ZS j; u (still in Uw) and Wf are read in batches
+    // with the ADM-RT relaxation or the learned closure's force: thread
+    // column b = tid % B, rows z = tid / B + ZS j; the operands (u still in
+    // Uw and Wf, or two stress spectra and a y derivative) are read in
+    // batches
     typedef typename ScalarOf<V>::T T;
     constexpr int ZS = NT / B;
     const int b = tid % B, zq = tid / B, xc = min(x0 + b, nxh - 1);
-    const T cn = (T)a.chi_n;
+    const T cn = (T)a.chi_n, al = (T)a.alfa[xc];
+    // tau_i1 and tau_i3 of row h of the stress tensor (11 12 13 22 23 33)
+    const int t1 = h, t3 = h == 0 ? 2 : (h == 1 ? 4 : 5);
     for (int z0 = zq; z0 < nz; z0 += 4 * ZS) {
-        V u[4], w[4];
+        V u[4], w[4], e[4];
+        T be[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int z = min(z0 + q * ZS, nz - 1);
-            u[q] = a.Uw[uidx(h, z, y, xc, nz, ny, nxh)];
-            w[q] = a.Wf[uidx(h, z, y, xc, nz, ny, nxh)];
+            if (a.les) {
+                u[q] = a.Uw[uidx(h, z, y, xc, nz, ny, nxh)];
+                w[q] = a.Wf[uidx(h, z, y, xc, nz, ny, nxh)];
+            } else {
+                u[q] = a.TAU[uidx(t1, z, y, xc, nz, ny, nxh)];
+                w[q] = a.TAU[uidx(t3, z, y, xc, nz, ny, nxh)];
+                e[q] = a.DT[uidx(h, z, y, xc, nz, ny, nxh)];
+                be[q] = (T)a.beta[z];
+            }
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -253,9 +303,14 @@ __global__ void __launch_bounds__((ZPlan<V, N>::NT)) k_zana4(Step2Args<V> a, con
             const int zp = z < nz / 2 ? z : z + N - nz;
             V v = A[b * SCOL + zp + zp / N1];
             if (z != nz / 2) {
-                const T lp = (T)a.lpf[(size_t)z * nxh + xc];
-                v.x -= cn * (u[q].x - lp * w[q].x);
-                v.y -= cn * (u[q].y - lp * w[q].y);
+                if (a.les) {
+                    const T lp = (T)a.lpf[(size_t)z * nxh + xc];
+                    v.x -= cn * (u[q].x - lp * w[q].x);
+                    v.y -= cn * (u[q].y - lp * w[q].y);
+                } else {
+                    v.x -= -al * u[q].y + e[q].x - be[q] * w[q].y;
+                    v.y -= al * u[q].x + e[q].y + be[q] * w[q].x;
+                }
             }
             a.Uw[uidx(h, z, y, xc, nz, ny, nxh)] = v;
         }
@@ -295,6 +350,24 @@ bool launch_zsyn4(const Step2Args<V> &a, const V *tw, hipStream_t s)
 }
 
 template <typename V>
+bool launch_gsyn4(const Step2Args<V> &a, const V *tw, hipStream_t s)
+{
+    switch (a.nzp) {
+#define SG_CASE(N)                                                                       \
+    case N: {                                                                            \
+        using P = ZPlan<V, N>;                                                           \
+        dim3 grid((a.nxh + P::B - 1) / P::B, 9, a.ny);                                   \
+        hipLaunchKernelGGL((k_zsyn4<V, N, 1>), grid, dim3(P::NT), P::lds, s, a, tw);     \
+        return true;                                                                     \
+    }
+        SG_Z4LENS(SG_CASE)
+#undef SG_CASE
+    default:
+        return false;
+    }
+}
+
+template <typename V>
 bool launch_zana4(const Step2Args<V> &a, const V *tw, hipStream_t s, int nf)
 {
     switch (a.nzp) {
@@ -314,6 +387,8 @@ bool launch_zana4(const Step2Args<V> &a, const V *tw, hipStream_t s, int nf)
 
 template bool launch_zsyn4<double2>(const Step2Args<double2> &, const double2 *, hipStream_t);
 template bool launch_zsyn4<float2>(const Step2Args<float2> &, const float2 *, hipStream_t);
+template bool launch_gsyn4<double2>(const Step2Args<double2> &, const double2 *, hipStream_t);
+template bool launch_gsyn4<float2>(const Step2Args<float2> &, const float2 *, hipStream_t);
 template bool launch_zana4<double2>(const Step2Args<double2> &, const double2 *, hipStream_t,
                                     int);
 template bool launch_zana4<float2>(const Step2Args<float2> &, const float2 *, hipStream_t, int);

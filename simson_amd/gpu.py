@@ -27,7 +27,7 @@ EXPORTS = (
     "simson_gpu_init", "simson_gpu_upload", "simson_gpu_prhs", "simson_gpu_nonlin",
     "simson_gpu_linear", "simson_gpu_substep",
     "simson_gpu_run", "simson_gpu_download", "simson_gpu_amp", "simson_gpu_pressure",
-    "simson_gpu_les", "simson_les_filter", "simson_gpu_sync",
+    "simson_gpu_les", "simson_les_filter", "simson_gpu_closure", "simson_gpu_sync",
     "simson_mlp_create", "simson_mlp_nparams", "simson_mlp_params", "simson_mlp_set_data",
     "simson_mlp_train", "simson_mlp_grad", "simson_mlp_infer", "simson_mlp_last_error",
     "simson_mlp_destroy",
@@ -85,6 +85,8 @@ def lib():
         L.simson_gpu_les.argtypes = [vp, C.POINTER(SimsonLes)]
         L.simson_les_filter.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
                                         _dp, _dp]
+        L.simson_gpu_closure.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p]
         L.simson_gpu_sync.argtypes = [vp]
         L.simson_gpu_timing_enable.argtypes = [vp, C.c_int]
         L.simson_gpu_timing_read.argtypes = [vp, C.POINTER(C.c_char_p),
@@ -209,6 +211,19 @@ class BlaGPU:
         or none with iles = 0."""
         p = SimsonLes(iles, iord, cutoff, chi)
         self._check(lib().simson_gpu_les(self.h, C.byref(p)))
+
+    def closure(self, params, in_scale=None, in_shift=None, yfeat=None, out_scale=None,
+                out_shift=None):
+        """Learned closure in the step (params None turns it off); see
+        simson_amd.sgsnet.closure_maps for the feature maps."""
+        if params is None:
+            self._check(lib().simson_gpu_closure(self.h, None, None, None, None, None, None))
+            return
+        arrs = [np.ascontiguousarray(params, np.float32)] + [
+            np.ascontiguousarray(a, np.float64)
+            for a in (in_scale, in_shift, yfeat, out_scale, out_shift)]
+        self._closure_keep = arrs
+        self._check(lib().simson_gpu_closure(self.h, *[a.ctypes.data_as(C.c_void_p) for a in arrs]))
 
     def amp(self) -> np.ndarray:
         """Per-y plane statistics of boxamp.f, [20][ny] (row i = amp(:, i+1))."""

@@ -175,3 +175,17 @@ def evaluate(net: SgsNet, sc: Standardiser, X, T, delta2_plus: float):
     out["corr_tau12_net"] = float(np.corrcoef(P[:, 1], T[:, 1])[0, 1])
     out["corr_tau12_smag"] = float(np.corrcoef(Sd[:, 1], T[:, 1])[0, 1])
     return out
+
+
+def closure_maps(sc: Standardiser, re: float, re_tau: float, ny: int):
+    """Feature and output maps of simson_gpu_closure for a network trained
+    on les_samples data (wall units of re_tau): (in_scale[9], in_shift[9],
+    yfeat[ny], out_scale[6], out_shift[6])."""
+    utau = re_tau / re
+    in_scale = 1.0 / (utau * utau * re * sc.xs[:9])
+    in_shift = -sc.xm[:9] / sc.xs[:9]
+    y = np.cos(np.pi * np.arange(ny) / (ny - 1))
+    yfeat = ((1.0 - np.abs(y)) * re_tau - sc.xm[9]) / sc.xs[9]
+    out_scale = sc.ts * utau * utau
+    out_shift = sc.tm * utau * utau
+    return in_scale, in_shift, yfeat, out_scale, out_shift
