@@ -130,15 +130,31 @@ def cpu_baseline(nsteps: int = 300, threads: int = 16):
 FP32_PEAK_TFLOPS = 157.3
 
 
-def time_form(precision: int, steps: int, warmup: int):
+def time_form(precision: int, steps: int, warmup: int, model: str = ""):
     """Run one form of the step on the bench workload; returns steps/s,
-    seconds, per-family (total ms, launches) and the last CFL*dt."""
+    seconds, per-family (total ms, launches) and the last CFL*dt.
+    model: "" (DNS), "adm-rt" (the relaxation model of sgs.i iles = 1 with
+    the default sgs.i) or "learned" (the closure network evaluated at every
+    point of the dealiased grid every substep; its cost does not depend on
+    the weights, so the network keeps its seeded initial weights)."""
+    import numpy as np
+
     from simson_amd import fields
     from simson_amd.gpu import BlaGPU
 
     f = fields.channel_field(NX, NY, NZ, XL, ZL, RE, kmax=16, rms=0.1, seed=2)
     g = BlaGPU(NX, NY, NZ, RE, XL, ZL, cflux=True, precision=precision)
     try:
+        if model == "adm-rt":
+            g.les(iles=1, cutoff=2.0 / 3.0, iord=5, chi=0.2)
+        elif model == "learned":
+            from simson_amd.sgsnet import SgsNet
+            with SgsNet(1) as net:
+                params = net.params
+            utau2 = (180.0 / RE) ** 2
+            y = np.cos(np.pi * np.arange(NY) / (NY - 1))
+            g.closure(params, np.full(9, 1.0 / (utau2 * RE)), np.zeros(9),
+                      (1.0 - np.abs(y)) * 180.0 / 100.0, np.full(6, 0.1 * utau2), np.zeros(6))
         g.upload_velocity(f.u, -2.0 / RE)
         g.run(warmup, DT, icfl=3, first=True)
         g.sync()
@@ -202,6 +218,9 @@ def main():
     ap.add_argument("--form", choices=["both", "mixed", "fp64"], default="both",
                     help="which forms of the step to time (the headline value is the "
                          "mixed form; 'fp64' reports the FP64 step as the value)")
+    ap.add_argument("--models", default="adm-rt,learned",
+                    help="subgrid models also timed on the mixed step (comma list of "
+                         "adm-rt, learned; empty for none), reported under forms")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy a device buffer of this size three times "
                          "(a plain copy of known bytes that scales rocprofv3 --pmc "
@@ -222,6 +241,8 @@ def main():
         res["fp64"] = time_form(SIMSON_FP64, args.steps, args.warmup)
     if args.form in ("both", "mixed"):
         res["mixed"] = time_form(SIMSON_MIXED, args.steps, args.warmup)
+        for m in [m for m in args.models.split(",") if m]:
+            res["mixed+" + m] = time_form(SIMSON_MIXED, max(1, args.steps // 2), args.warmup, m)
     if args.calib_copy_mib > 0:
         import ctypes
         from simson_amd import build as _b
@@ -237,9 +258,9 @@ def main():
     kernels = kernel_table(kern, mixed)
     forms = {}
     for name, (fs, fel, fk, fc) in res.items():
-        forms[name] = {"steps_per_s": round(fs, 3), "ms_per_step": round(1e3 * fel / args.steps, 4),
+        forms[name] = {"steps_per_s": round(fs, 3), "ms_per_step": round(1e3 / fs, 4),
                        "grid_points_steps_per_s": round(fs * NX * NY * NZ, 1),
-                       "cfl_dt": fc, "kernels": kernel_table(fk, name == "mixed")}
+                       "cfl_dt": fc, "kernels": kernel_table(fk, name.startswith("mixed"))}
     out = {
         "metric": METRIC,
         "value": round(sps, 3),
