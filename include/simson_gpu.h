@@ -114,6 +114,21 @@ int simson_gpu_download(simson_gpu_t *g, double *ur, double *ui, int nslots,
  * (wampbl.f).  Synchronises. */
 int simson_gpu_amp(simson_gpu_t *g, double *amp);
 
+/* x-y statistics of boxxys.f (the per-plane call of bla.f:2089-2105, made
+ * at the start of every ixys-th step from t = txys on) for the current
+ * velocity state, accumulated into xys, a C array [nxys][ny][nx] whose
+ * row i is xys(:, :, i+1) of the reference (x in physical order):
+ *   1-3 u, v, w; 4-6 u^2, v^2, w^2; 7-9 om_x, om_y, om_z;
+ *   10-12 their squares; 13-15 uv, uw, vw (boxxys.f:540-636),
+ * each summed over the nz physical z points with weight dtn/nz, after the
+ * lower-wall velocity u0low, w0low is subtracted from the mean mode and the
+ * frame is shifted by (xs, zs) as in xzsh.f.  As in boxxys.f, indices 1-12
+ * need nxys >= 12 and 13-15 need nxys = 15; 12 <= nxys <= 15 (the higher
+ * statistics of the reference's list are not computed: -1).  The caller
+ * keeps the sum of the weights (sumw) and divides by it when it writes the
+ * file (wxys.f).  Synchronises. */
+int simson_gpu_xys(simson_gpu_t *g, double dtn, double xs, double zs, int nxys, double *xys);
+
 /* Pressure of the current state: nonlinp over all y planes and linearp over
  * all z planes (bla.f:2486-2527, nonlinp.f, linearp.f with the Neumann tau
  * solver of setmac.f/crhsc.f/tridc.f), i.e. what bla puts in slot 8 and

@@ -69,6 +69,9 @@ class BlaParams:
     longli: bool = False
     iext: int = 0
     ixys: int = 0
+    namxys: str = "xy.stat"
+    ixyss: int = 0
+    txys: float = 0.0
     msave: int = 0
     tsave: list = field(default_factory=list)
     nmsave: list = field(default_factory=list)
@@ -94,6 +97,12 @@ class BlaParams:
         if p.dt <= 0 and p.icfl <= 0:
             p.icfl = p.nst
         p.maxit = max(p.maxit, 0) // p.nst * p.nst
+        # rparambl.f:1496, 1511
+        if p.ixys > 0:
+            p.ixys = max(p.ixys, p.nst) // p.nst * p.nst
+            p.ixyss = max(0, p.ixyss)
+            if p.ixyss > 0:
+                p.ixyss = max(p.ixyss, p.nst) // p.nst * p.nst
         return p
 
 
@@ -197,7 +206,14 @@ def parse(text: str) -> BlaParams:
         name()
     p.ixys = max(0, int(_num(tok())))
     if p.ixys > 0:
-        raise BlaInputError("xy statistics (ixys) are not supported yet")
+        # rparambl.f:1504-1660 with mxys = 1 (bla.f:200): no phase average
+        p.namxys = name()
+        p.ixyss = int(_num(tok()))
+        p.txys = _num(tok())
+        if _logical(tok()) or _logical(tok()):
+            raise BlaInputError("two-point correlations (corrf, corrf_x) are not supported")
+        if _logical(tok()):
+            raise BlaInputError("time series (serf) are not supported")
     p.msave = int(_num(tok()))
     if p.msave < 0 or p.msave == 100000:
         raise BlaInputError("periodic save modes are not supported")
@@ -270,7 +286,15 @@ def format_channel(p: BlaParams) -> str:
     out += [
         ".false.         y-dependent statistics (longli)",
         "0               Extremum calculation interval (iext)",
-        "0               xy-statistics calculation interval (ixys)",
+        f"{p.ixys}       xy-statistics calculation interval (ixys)",
+    ]
+    if p.ixys > 0:
+        out += [p.namxys, f"{p.ixyss}  xy-statistics saving interval (ixyss)",
+                f"{p.txys!r}  xy-statistics start time (txys)",
+                ".false.         Two-point correlations in z (corrf)",
+                ".false.         Two-point correlations in x (corrf_x)",
+                ".false.         Time series (serf)"]
+    out += [
         f"{len(p.tsave)}  Number of 3D fields to save (msave)",
     ]
     for t, n in zip(p.tsave, p.nmsave):

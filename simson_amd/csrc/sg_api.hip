@@ -81,6 +81,12 @@ struct simson_gpu {
     double *Dm = nullptr;
     float *Dm32 = nullptr;
     FftPlan pz, pm;
+    // x-y statistics (allocated at the first simson_gpu_xys call): unpadded
+    // transforms of lengths nz and nx/2, their twiddles and the sums
+    // [15][ny][nx]; the step-2 planes P serve as work space
+    FftPlan pzs, pms;
+    double2 *twzs = nullptr, *twxs = nullptr;
+    double *xysd = nullptr;
     double mflux = 0., u0upp = 0., u0low = 0., w0upp = 0., w0low = 0.;
     bool timing = false;
     std::vector<TimedLaunch> launches;
@@ -172,7 +178,8 @@ void free_all(simson_gpu_t *g)
                     g->lam[2].gp, g->lam[2].vh, g->lam[2].sh, g->U32, g->P32, g->C32,
                     g->twz32, g->twx32, g->Tf32, g->Tb32, g->Q, g->Qc, g->PR, g->pscr,
                     g->Lm, g->lpf, g->Lm32, g->Wf, g->Wf32, g->clW, g->clY, g->PG, g->TAU,
-                    g->DT, g->PG32, g->TAU32, g->DT32, g->Dm, g->Dm32};
+                    g->DT, g->PG32, g->TAU32, g->DT32, g->Dm, g->Dm32, g->twzs,
+                    g->twxs, g->xysd};
     for (void *q : ptrs)
         if (q)
             (void)hipFree(q);
@@ -876,6 +883,59 @@ int simson_gpu_amp(simson_gpu_t *g, double *amp)
     CK(hipMemcpyAsync(amp, g->amp, 20 * (size_t)g->ny * sizeof(double), hipMemcpyDeviceToHost,
                       g->stream));
     CK(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int simson_gpu_xys(simson_gpu_t *g, double dtn, double xs, double zs, int nxys, double *xys)
+{
+    if (!g || !xys)
+        return SIMSON_E_ARG;
+    if (nxys < 12 || nxys > 15) {
+        g->err = "xy statistics: 12 <= nxys <= 15 (indices 1-15 of boxxys.f)";
+        return SIMSON_E_ARG;
+    }
+    const int ny = g->ny, nx = g->nx;
+    if (!g->xysd) {
+        if (!xys_fits(g->nz, g->nxh) || !make_radices(g->nz, g->pzs) ||
+            !make_radices(g->nxh, g->pms)) {
+            g->err = "xy statistics: unsupported nx or nz";
+            return SIMSON_E_ARG;
+        }
+        std::vector<double2> tz = roots(g->nz), tx = roots(nx);
+        if (dalloc(g, &g->twzs, tz.size()) || dalloc(g, &g->twxs, tx.size()) ||
+            dalloc(g, &g->xysd, (size_t)15 * ny * nx) || h2d(g, g->twzs, tz.data(), tz.size()) ||
+            h2d(g, g->twxs, tx.data(), tx.size()))
+            return SIMSON_E_HIP;
+        g->pzs.tw = g->twzs;
+        g->pzs.tabn = g->nz;
+        g->pms.tw = g->twxs;      // length-nx/2 transform uses every other root
+        g->pms.tabn = nx;
+    }
+    refresh_u64(g);
+    XysArgs a{};
+    a.U = g->U;
+    a.Q = g->P;                   // [6][ny][nz][nxh] within [6][ny][nzp][nxh]
+    a.out = g->xysd;
+    a.alfa = g->alfa;
+    a.beta = g->beta;
+    a.ny = ny;
+    a.nz = g->nz;
+    a.nxh = g->nxh;
+    a.c1 = dtn / (double)g->nz;
+    a.u0low = g->u0low;
+    a.w0low = g->w0low;
+    a.xs = xs;
+    a.zs = zs;
+    a.shift = xs != 0. || zs != 0.;
+    launch_xys(a, g->pzs, g->pms, g->twxs, g->stream);
+    CK(hipGetLastError());
+    const int nout = nxys == 15 ? 15 : 12;
+    std::vector<double> h((size_t)nout * ny * nx);
+    CK(hipMemcpyAsync(h.data(), g->xysd, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                      g->stream));
+    CK(hipStreamSynchronize(g->stream));
+    for (size_t i = 0; i < h.size(); i++)
+        xys[i] += h[i];
     return 0;
 }
 

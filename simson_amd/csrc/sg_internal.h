@@ -149,6 +149,20 @@ int  pressure_max_ny();
 void launch_pressure_cols(const double2 *Qc, const double2 *gk, const double *alfa,
                           const double *beta, int nz, int nxh, int ny, double re,
                           double nxpnzp, double *scratch, double2 *out, hipStream_t s);
+// x-y statistics 1-15 of boxxys.f for the current field, times c1 = dtn/nz,
+// into out [15][ny][nx]; Q [6][ny][nz][nxh] is work space
+struct XysArgs {
+    const double2 *U;
+    double2 *Q;
+    double *out;
+    const double *alfa, *beta;
+    int ny, nz, nxh;
+    double c1, u0low, w0low, xs, zs;
+    int shift;
+};
+bool xys_fits(int nz, int nxh);
+void launch_xys(const XysArgs &a, const FftPlan &pz, const FftPlan &pm, const double2 *wx,
+                hipStream_t s);
 // per-y amplitude sums of boxamp.f into amp[20][ny]
 void launch_amp(const double2 *U, const double *alfa, const double *beta, int nz, int ny, int nxh,
                 double *amp, hipStream_t s);

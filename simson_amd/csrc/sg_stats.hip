@@ -12,6 +12,21 @@
 //   amp(16..18) sum w |u|^2, |v|^2, |w|^2
 // and from the (0,0) mode amp(9..12) = Re u, Re w, Re om_x, Re om_z.
 // amp(13), amp(19), amp(20) are left zero (the writer fills amp(13)).
+//
+// The x-y statistics of boxxys.f (called per y plane from bla.f:2089-2105)
+// on the unpadded nx x nz grid, two kernels per call:
+//   k_xys_z  gather u, v, w, om_x, om_y, om_z of a y plane (getxz.f with
+//            ipad = 0: the z oddball zeroed), subtract the lower-wall
+//            velocity from the mean mode and apply the frame shift
+//            exp(-i (alpha xs + beta zs)) (boxxys.f:382-412, xzsh.f), then
+//            the inverse complex z transform of length nz (vcfftb) in LDS;
+//   k_xys_x  per y plane: the inverse real x transform of length nx
+//            (vrfftb, packed as nx/2 complex points) of every z row and
+//            the z sums of the one-point statistics 1-15 (boxxys.f:540-636)
+//            u_i, u_i^2, om_i, om_i^2, uv, uw, vw, weighted by dtn/nz.
+#include <algorithm>
+
+#include "sg_fft.h"
 #include "sg_internal.h"
 
 namespace sg {
@@ -91,7 +106,174 @@ __global__ void __launch_bounds__(kAmpT) k_amp(const double2 *__restrict__ U,
     }
 }
 
+constexpr int kXysT = 256;
+constexpr int kXysLds = 2048;      // complex values per LDS buffer (two buffers)
+constexpr int kXysN = 15;
+
+// field f of plane y (0-2 velocities, 3 om_x, 4 om_y, 5 om_z) into
+// Q [6][ny][nz][nxh], transformed to physical z
+__global__ void __launch_bounds__(kXysT) k_xys_z(XysArgs a, FftPlan pz, int B)
+{
+    extern __shared__ double2 lds[];
+    const int nz = a.nz, nxh = a.nxh, ny = a.ny;
+    const size_t F = (size_t)nz * ny * nxh;
+    double2 *buf0 = lds, *buf1 = lds + (size_t)B * nz;
+    const int f = blockIdx.y, y = blockIdx.z, x0 = blockIdx.x * B;
+    const int nb = min(B, nxh - x0);
+    for (int idx = threadIdx.x; idx < nz * B; idx += blockDim.x) {
+        const int z = idx / B, b = idx - z * B;
+        double2 v = make_double2(0., 0.);
+        if (b < nb && z != nz / 2) {
+            const int x = x0 + b;
+            const size_t o = ((size_t)z * ny + y) * nxh + x;
+            const double al = a.alfa[x], be = a.beta[z];
+            if (f == 4) {
+                const double2 u = a.U[o], w = a.U[2 * F + o];
+                v = make_double2(-be * u.y + al * w.y, be * u.x - al * w.x);
+            } else {
+                v = a.U[(size_t)(f < 4 ? f : 4) * F + o];
+                if (x == 0 && z == 0) {
+                    if (f == 0)
+                        v.x -= a.u0low;
+                    if (f == 2)
+                        v.x -= a.w0low;
+                }
+            }
+            if (a.shift) {
+                double sn, cs;
+                sincos(-a.xs * al - a.zs * be, &sn, &cs);
+                v = make_double2(v.x * cs - v.y * sn, v.y * cs + v.x * sn);
+            }
+        }
+        buf0[b * nz + z] = v;
+    }
+    __syncthreads();
+    double2 *res = lds_fft(buf0, buf1, nz, B, pz, +1.0);
+    double2 *Q = a.Q + (((size_t)f * ny + y) * nz) * nxh + x0;
+    for (int idx = threadIdx.x; idx < nz * B; idx += blockDim.x) {
+        const int z = idx / B, b = idx - z * B;
+        if (b < nb)
+            Q[(size_t)z * nxh + b] = res[b * nz + z];
+    }
+}
+
+// one y plane per block: rows of RW z values at a time through the packed
+// inverse x transform, then the z sums; G groups of M threads split the
+// rows and are combined in a fixed order at the end
+__global__ void __launch_bounds__(kXysT) k_xys_x(XysArgs a, FftPlan pm,
+                                                 const double2 *__restrict__ wx, int RW)
+{
+    extern __shared__ double2 lds[];
+    const int M = a.nxh, nz = a.nz, ny = a.ny, y = blockIdx.x;
+    double2 *A = lds, *Bf = lds + (size_t)6 * RW * M;
+    const int G = kXysT / M;
+    const int tid = threadIdx.x, n = tid % M, grp = tid / M;
+    double acc[kXysN][2];
+#pragma unroll
+    for (int i = 0; i < kXysN; i++)
+        acc[i][0] = acc[i][1] = 0.;
+    for (int z0 = 0; z0 < nz; z0 += RW) {
+        const int nr = min(RW, nz - z0);
+        // half spectrum X_k, k < nx/2 -> packed Z_k (vrfftb: only Re X_0,
+        // no Nyquist term)
+        for (int idx = tid; idx < 6 * RW * M; idx += blockDim.x) {
+            const int f = idx / (RW * M);
+            const int rem = idx - f * RW * M;
+            const int r = rem / M, k = rem - r * M;
+            double2 Z = make_double2(0., 0.);
+            if (r < nr) {
+                const double2 *row = a.Q + (((size_t)f * ny + y) * nz + z0 + r) * M;
+                double2 Xk = row[k];
+                if (k == 0)
+                    Xk.y = 0.;
+                double2 Xm = k > 0 ? row[M - k] : make_double2(0., 0.);
+                Xm.y = -Xm.y;
+                const double2 sm = cadd(Xk, Xm), d = csub(Xk, Xm);
+                Z = cadd(sm, mul_is(cmul(wx[k], d), 1.0));
+            }
+            A[(f * RW + r) * M + k] = Z;
+        }
+        __syncthreads();
+        const double2 *res = lds_fft(A, Bf, M, 6 * RW, pm, +1.0);
+        if (grp < G) {
+            for (int r = grp; r < nr; r += G) {
+                double2 q[6];
+#pragma unroll
+                for (int f = 0; f < 6; f++)
+                    q[f] = res[(f * RW + r) * M + n];
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    double v[6];
+#pragma unroll
+                    for (int f = 0; f < 6; f++)
+                        v[f] = e ? q[f].y : q[f].x;
+#pragma unroll
+                    for (int i = 0; i < 3; i++) {
+                        acc[i][e] += v[i];
+                        acc[3 + i][e] += v[i] * v[i];
+                        acc[6 + i][e] += v[3 + i];
+                        acc[9 + i][e] += v[3 + i] * v[3 + i];
+                    }
+                    acc[12][e] += v[0] * v[1];
+                    acc[13][e] += v[0] * v[2];
+                    acc[14][e] += v[1] * v[2];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // fixed-order combination of the groups through LDS
+    double *red = (double *)lds;
+    if (grp < G) {
+#pragma unroll
+        for (int i = 0; i < kXysN; i++) {
+            red[((size_t)grp * kXysN + i) * 2 * M + 2 * n] = acc[i][0];
+            red[((size_t)grp * kXysN + i) * 2 * M + 2 * n + 1] = acc[i][1];
+        }
+    }
+    __syncthreads();
+    const int nx = 2 * M;
+    for (int idx = tid; idx < kXysN * nx; idx += blockDim.x) {
+        const int i = idx / nx, x = idx - i * nx;
+        double t = 0.;
+        for (int gq = 0; gq < G; gq++)
+            t += red[((size_t)gq * kXysN + i) * nx + x];
+        a.out[((size_t)i * ny + y) * nx + x] = t * a.c1;
+    }
+}
+
 }  // namespace
+
+int xys_zblock(int nz, int nxh)
+{
+    return max(1, min(nxh, kXysLds / nz));
+}
+
+int xys_rows(int nz, int nxh)
+{
+    return max(1, min(nz, kXysLds / (6 * nxh)));
+}
+
+bool xys_fits(int nz, int nxh)
+{
+    // one z column and six x rows in a buffer, at least one thread per
+    // packed x point (the group sums take at most kXysT * 2 * kXysN doubles,
+    // within the two buffers)
+    return nz <= kXysLds && 6 * nxh <= kXysLds && nxh <= kXysT;
+}
+
+void launch_xys(const XysArgs &a, const FftPlan &pz, const FftPlan &pm, const double2 *wx,
+                hipStream_t s)
+{
+    const int B = xys_zblock(a.nz, a.nxh);
+    const size_t lz = (size_t)2 * B * a.nz * sizeof(double2);
+    hipLaunchKernelGGL(k_xys_z, dim3((a.nxh + B - 1) / B, 6, a.ny), dim3(kXysT), lz, s, a, pz,
+                       B);
+    const int RW = xys_rows(a.nz, a.nxh);
+    const size_t lx = std::max((size_t)2 * 6 * RW * a.nxh * sizeof(double2),
+                               (size_t)(kXysT / a.nxh) * kXysN * 2 * a.nxh * sizeof(double));
+    hipLaunchKernelGGL(k_xys_x, dim3(a.ny), dim3(kXysT), lx, s, a, pm, wx, RW);
+}
 
 void launch_amp(const double2 *U, const double *alfa, const double *beta, int nz, int ny, int nxh,
                 double *amp, hipStream_t s)

@@ -6,11 +6,14 @@
 ! bla/rparambl.f:160-1860 (temporal channel subset), the .u reader/writer of
 ! bla/rdiscbl.f:93-411 and bla/wdiscbl.f:50-136, the RK weight bookkeeping of
 ! bla/bla.f:1465-1477 and 2384-2397, the CFL stop of bla/wcflbl.f:87-98 and
-! the stop criteria of bla/bla.f:2779-2833 and the amp.stat writer of
-! bla/wampbl.f (channel), the sgs.i reader of bla/rparamles.f (ADM-RT) --
+! the stop criteria of bla/bla.f:2779-2833, the amp.stat writer of
+! bla/wampbl.f (channel), the x-y statistics schedule of bla/bla.f:1830,
+! 2089-2142, 2907-2930 and file of bla/wxys.f, the sgs.i reader of
+! bla/rparamles.f (ADM-RT) --
 ! and calls the library for prhs (bla.f:1562),
 ! the nonlinbl plane loop (bla.f:2251-2288), its plane statistics
-! (boxamp.f, nonlinbl.f:430) and the linearbl plane loop (bla.f:2433-2475)
+! (boxamp.f, nonlinbl.f:430; boxxys.f) and the linearbl plane loop
+! (bla.f:2433-2475)
 ! through ISO_C_BINDING.
 program blagpu
   use iso_c_binding
@@ -19,14 +22,15 @@ program blagpu
 
   integer, parameter :: dp = kind(1.d0)
   real(dp), parameter :: pi = 3.14159265358979323846_dp
-  character(len=256) :: line, namnin, namnut, namamp
+  character(len=256) :: line, namnin, namnut, namamp, namxys
   character(len=256) :: nmsave(100)
   real(dp) :: tsave(101)
   integer :: version, maxit, nst, ibc, icfl, iamp, iext, ixys, msave
-  integer :: mwave, npl, isfd, imhd, loctyp, wbci, i, it, isave, st
+  integer :: mwave, npl, isfd, imhd, loctyp, wbci, i, it, isave, st, ixyss
   real(dp) :: tmax, cpumax, wallmax, dt, cflmaxin, xl, rot, retau, cdev
   logical :: write_inter, varsiz, cflux, pert, cim, gall, suction, spat
-  logical :: sgs, tripf, longli, fileurms, pou, stopf, docfl
+  logical :: sgs, tripf, longli, fileurms, pou, stopf, docfl, corrf, corrf_x, serf
+  logical :: chkstop
   ! field and header
   integer :: nx, nyp, nzc, nfzsym, fltype
   real(dp) :: re, xlf, zl, t, xs, dstar
@@ -34,6 +38,10 @@ program blagpu
   ! time stepping (rkcoeff.f:21-32)
   real(dp) :: anrk(3), bnrk(3), dtn, dtnp1, an, bn, anp1, bnp1, cflp, cfl, px
   real(dp), allocatable :: amp(:,:), wint(:)
+  ! x-y statistics 1-15 of boxxys.f, xys(x, y, i), and the sum of weights
+  integer, parameter :: nxys = 15
+  real(dp), allocatable :: xys(:,:,:)
+  real(dp) :: sumw, txys, zs, u0low, w0low
   type(simson_params_t) :: p
   type(simson_les_t) :: les
   integer :: sgsver
@@ -89,6 +97,25 @@ program blagpu
   call nxt(line); read(line, *) iext
   if (iext > 0) call nxt(line)
   call nxt(line); read(line, *) ixys
+  ixys = max(0, ixys)
+  txys = 0._dp
+  ixyss = 0
+  if (ixys > 0) then
+     ! rparambl.f:1494-1660 with mxys = 1 (bla.f:200)
+     ixys = max(ixys, nst) / nst * nst
+     call nxt(line); namxys = word(line)
+     call nxt(line); read(line, *) ixyss
+     call nxt(line); read(line, *) txys
+     ixyss = max(0, ixyss)
+     if (ixyss > 0) ixyss = max(ixyss, nst) / nst * nst
+     call nxt(line); read(line, *) corrf
+     call nxt(line); read(line, *) corrf_x
+     call nxt(line); read(line, *) serf
+     if (corrf .or. corrf_x .or. serf) then
+        write(*,*) 'blagpu: two-point correlations and time series are not supported'
+        call stopnow(13)
+     end if
+  end if
   call nxt(line); read(line, *) msave
   if (msave < 0 .or. msave > 99) call stopnow(13)
   do i = 1, msave
@@ -100,7 +127,7 @@ program blagpu
   call nxt(line); read(line, *) npl
   close(10)
   if (pert .or. cim .or. suction .or. spat .or. isfd /= 0 .or. imhd /= 0 .or. &
-       loctyp /= 0 .or. tripf .or. wbci /= 0 .or. ixys > 0 .or. ibc /= 0 .or. nst /= 3 &
+       loctyp /= 0 .or. tripf .or. wbci /= 0 .or. ibc /= 0 .or. nst /= 3 &
        .or. mwave > 0 .or. npl > 0 &
        .or. dt <= 0._dp) then
      write(*,*) 'blagpu: bla.i requests a feature outside the channel step library'
@@ -151,6 +178,15 @@ program blagpu
      end do
   end do
   close(12)
+  ! lower-wall velocity of the mean mode (rdiscbl.f:316-323, bla.f:1041-1044)
+  u0low = ur(1, nyp, 1, 1)
+  w0low = ur(1, nyp, 1, 3)
+  if (abs(u0low) < 1.e-8_dp) u0low = 0._dp
+  if (abs(w0low) < 1.e-8_dp) w0low = 0._dp
+  zs = 0._dp
+  sumw = 0._dp
+  allocate(xys(nx, nyp, nxys))
+  xys = 0._dp
 
   ! ---------------- device state ----------------
   p%nx = nx; p%ny = nyp; p%nz = nzc
@@ -194,6 +230,17 @@ program blagpu
   cfl = 0._dp
   call system_clock(c0, crate)
   do
+     ! x-y statistics at the start of every ixys-th step from txys on
+     ! (bla.f:1830, 2089-2106), written every ixyss-th (bla.f:2116-2142)
+     if (ixys > 0) then
+        if (mod(it - 1 - nst, ixys) == 0 .and. t >= txys) then
+           call chk(simson_gpu_xys(g, dtn, xs, zs, int(nxys, c_int), xys))
+           sumw = sumw + dtn
+        end if
+        if (ixyss > 0) then
+           if (mod(it - 1, ixyss) == 0 .and. t >= txys .and. it > 1) call write_xys()
+        end if
+     end if
      ! plane statistics at the start of every iamp-th step (nonlinbl.f:430)
      ! and the amp.stat line (bla.f:2326-2331)
      ! (nested: Fortran's .and. need not short-circuit, and mod(., 0) traps)
@@ -225,10 +272,19 @@ program blagpu
      it = it + 1
      if (mod(it - 1, nst) == 0) then
         t = t + dtn
+        xs = xs - dtn * u0low                        ! bla.f:2606-2609
+        zs = zs - dtn * w0low
         if (t + .5_dp * dtn > tsave(isave)) then
            call save_field(trim(nmsave(isave)))
            isave = isave + 1
         end if
+        ! with statistics on, the stop criteria wait for a statistics
+        ! step (bla.f:2789)
+        chkstop = .true.
+        if (ixys > 0) then
+           if (t >= txys .and. mod(it - 1, ixys) /= 0) chkstop = .false.
+        end if
+        if (.not. chkstop) cycle
         inquire(file='stop.now', exist=stopf)
         if (stopf) write(*,*) 'STOP DUE TO stop.now FILE'
         if (maxit > 0 .and. it - 1 >= maxit) then
@@ -244,6 +300,9 @@ program blagpu
   end do
   call system_clock(c1)
   if (iamp > 0) close(15)
+  if (ixys > 0) then
+     if (t >= txys) call write_xys()                  ! bla.f:2907-2930
+  end if
   call save_field(trim(namnut))
   write(*,*) 'Average time step    : ', real(c1 - c0, dp) / real(crate, dp) / &
        real((it - 1) / nst, dp)
@@ -326,6 +385,23 @@ contains
     write(15, '(19(e25.16e3,1x))') tt / dstar, (sm(j), j = 1, 3), (sm(j) * dstar, j = 4, 6), &
          sm(7), sm(8), e0, hplus, pxx, sm(13) * re / 2._dp, tau1, tau2, (delta(j), j = 1, 3), wb
   end subroutine write_amp
+
+  ! wxys.f:83-135: the velocity statistics divided by the sum of weights
+  ! (no scalars, correlations or time series)
+  subroutine write_xys()
+    integer :: j
+    write(*,*) '** Write statistics ** ', t / dstar, sumw / dstar
+    open(19, file=trim(namxys), form='unformatted', status='replace')
+    write(19) re, .false., xl, zl, t, 0._dp
+    write(19) 'A', 0._dp, 0._dp, 0._dp, 0._dp
+    write(19) nx, nyp, nzc, nfzsym
+    write(19) fltype, dstar
+    write(19) sumw, nxys, 0, 0
+    do j = 1, nxys
+       write(19) xys(:, :, j) * (1._dp / sumw)
+    end do
+    close(19)
+  end subroutine write_xys
 
   ! wdiscbl.f:50-136 (channel: no shift, no extra header records)
   subroutine save_field(name)

@@ -81,6 +81,15 @@ module simson_gpu_mod
        real(c_double), intent(out) :: amp(*)
      end function simson_gpu_amp
 
+     integer(c_int) function simson_gpu_xys(g, dtn, xs, zs, nxys, xys) &
+          bind(c, name='simson_gpu_xys')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: g
+       real(c_double), value :: dtn, xs, zs
+       integer(c_int), value :: nxys
+       real(c_double), intent(inout) :: xys(*)
+     end function simson_gpu_xys
+
      integer(c_int) function simson_gpu_les(g, les) bind(c, name='simson_gpu_les')
        import :: c_int, c_ptr, simson_les_t
        type(c_ptr), value :: g

@@ -26,7 +26,8 @@ NSLOT = 7
 EXPORTS = (
     "simson_gpu_init", "simson_gpu_upload", "simson_gpu_prhs", "simson_gpu_nonlin",
     "simson_gpu_linear", "simson_gpu_substep",
-    "simson_gpu_run", "simson_gpu_download", "simson_gpu_amp", "simson_gpu_pressure",
+    "simson_gpu_run", "simson_gpu_download", "simson_gpu_amp", "simson_gpu_xys",
+    "simson_gpu_pressure",
     "simson_gpu_les", "simson_les_filter", "simson_gpu_closure", "simson_gpu_sync",
     "simson_mlp_create", "simson_mlp_nparams", "simson_mlp_params", "simson_mlp_set_data",
     "simson_mlp_train", "simson_mlp_grad", "simson_mlp_infer", "simson_mlp_last_error",
@@ -81,6 +82,7 @@ def lib():
         L.simson_gpu_download.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                           C.POINTER(C.c_double)]
         L.simson_gpu_amp.argtypes = [vp, _dp]
+        L.simson_gpu_xys.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int, _dp]
         L.simson_gpu_pressure.argtypes = [vp, _dp, _dp]
         L.simson_gpu_les.argtypes = [vp, C.POINTER(SimsonLes)]
         L.simson_les_filter.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
@@ -230,6 +232,20 @@ class BlaGPU:
         a = np.zeros((20, self.ny))
         self._check(lib().simson_gpu_amp(self.h, a))
         return a
+
+    def xys(self, dtn: float, out: np.ndarray | None = None, xs: float = 0.0,
+            zs: float = 0.0) -> np.ndarray:
+        """Add the x-y statistics 1-15 of boxxys.f for the current state,
+        weighted by dtn, to out ([15][ny][nx], row i = xys(:, :, i+1));
+        returns out (a new zero array when None)."""
+        if out is None:
+            out = np.zeros((15, self.ny, self.nx))
+        if out.shape[1:] != (self.ny, self.nx) or out.shape[0] not in (12, 13, 14, 15):
+            raise ValueError("out must be [nxys][ny][nx] with 12 <= nxys <= 15")
+        if not (out.flags.c_contiguous and out.dtype == np.float64):
+            raise ValueError("out must be a C-contiguous float64 array")
+        self._check(lib().simson_gpu_xys(self.h, dtn, xs, zs, out.shape[0], out))
+        return out
 
     def pressure(self) -> np.ndarray:
         """Pressure of the current state, complex [z][y][kx] at physical y

@@ -1,13 +1,19 @@
-"""Host side of bla's amplitude statistics (amp.stat).
+"""Host side of bla's statistics: amplitudes (amp.stat) and x-y statistics.
 
 The per-y plane sums (boxamp.f) come from the GPU (simson_gpu_amp); this
 module does what bla's host does with them: the Clenshaw-Curtis weights of
 preprbl.f:78-87, the integration over y and the line of wampbl.f for the
 channel flow types (fltype 1), wampbl.f:124-278.
+
+The x-y statistics sums (boxxys.f, simson_gpu_xys) are written and read in
+the unformatted file layout of wxys.f:83-135 (velocity statistics only:
+scalar = 0, no correlations, no time series).
 """
 from __future__ import annotations
 
 import math
+import struct
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -85,3 +91,69 @@ def fortran_e(v: float, d: int = 16, w: int = 25, ed: int = 3) -> str:
 def format_row(row) -> str:
     """amp.stat line format of wampbl.f:276 (19(e25.16e3,tr1))."""
     return "".join(fortran_e(float(v)) + " " for v in row)
+
+
+XYS_NAMES = ("u", "v", "w", "u2", "v2", "w2", "omx", "omy", "omz", "omx2", "omy2", "omz2",
+             "uv", "uw", "vw")
+
+
+@dataclass
+class XysFile:
+    re: float
+    xl: float
+    zl: float
+    t: float
+    nx: int
+    ny: int
+    nz: int
+    fltype: int
+    dstar: float
+    sumw: float
+    stats: np.ndarray      # [nxys][ny][nx], already divided by sumw
+
+
+def _rec(f, payload: bytes) -> None:
+    f.write(struct.pack("<i", len(payload)))
+    f.write(payload)
+    f.write(struct.pack("<i", len(payload)))
+
+
+def _read_rec(f) -> bytes:
+    (n,) = struct.unpack("<i", f.read(4))
+    data = f.read(n)
+    (n2,) = struct.unpack("<i", f.read(4))
+    if n2 != n or len(data) != n:
+        raise ValueError("corrupt Fortran record marker in xy-statistics file")
+    return data
+
+
+def write_xys(path: str, sums: np.ndarray, sumw: float, re: float, xl: float, zl: float,
+              t: float, nz: int, fltype: int = 1, dstar: float = 1.0) -> None:
+    """Write accumulated x-y statistics sums[nxys][ny][nx] divided by the
+    sum of the weights, in the record layout of wxys.f:83-135."""
+    nxys, ny, nx = sums.shape
+    with open(path, "wb") as f:
+        _rec(f, struct.pack("<di4d", re, 0, xl, zl, t, 0.0))
+        _rec(f, b"A" + struct.pack("<4d", 0.0, 0.0, 0.0, 0.0))
+        _rec(f, struct.pack("<4i", nx, ny, nz, 0))
+        _rec(f, struct.pack("<id", fltype, dstar))
+        _rec(f, struct.pack("<d3i", sumw, nxys, 0, 0))
+        for i in range(nxys):
+            _rec(f, (sums[i] * (1.0 / sumw)).astype("<f8").tobytes())
+
+
+def read_xys(path: str) -> XysFile:
+    """Read a velocity x-y statistics file written by wxys.f (fltype 1-5)."""
+    with open(path, "rb") as f:
+        re_, _pou, xl, zl, t, _ = struct.unpack("<di4d", _read_rec(f)[:44])
+        _read_rec(f)                                   # 'A', mhd_n, b0
+        nx, ny, nz, _nfzsym = struct.unpack("<4i", _read_rec(f))
+        fltype, dstar = struct.unpack("<id", _read_rec(f)[:12])
+        if fltype < 0 or fltype >= 6:
+            _read_rec(f)
+        sumw, nxys, _nxysth, _scalar = struct.unpack("<d3i", _read_rec(f)[:20])
+        stats = np.empty((nxys, ny, nx))
+        for i in range(nxys):
+            stats[i] = np.frombuffer(_read_rec(f), dtype="<f8").reshape(ny, nx)
+    return XysFile(re=re_, xl=xl, zl=zl, t=t, nx=nx, ny=ny, nz=nz, fltype=fltype,
+                   dstar=dstar, sumw=sumw, stats=stats)

@@ -5,6 +5,7 @@ Runs in the build container only (it needs /root/reference and amdflang):
     python tests/golden/make_golden.py            # c1_32x33x32.npz
     python tests/golden/make_golden.py pressure   # p_32x33x32.npz
     python tests/golden/make_golden.py les        # les_32x33x32.npz
+    python tests/golden/make_golden.py xys        # xys_32x33x32.npz
 
 1. builds the reference `bla` for 32x33x32 (serial, pressure off) with
    oracle/build_ref.sh into oracle/_ref/;
@@ -38,7 +39,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, REPO)
 
-from simson_amd import blai, fields, sgsi, ufile  # noqa: E402
+from simson_amd import blai, fields, sgsi, stats, ufile  # noqa: E402
 
 NX, NY, NZ = 32, 33, 32
 BIN = os.path.join(REPO, "oracle", "_ref", "bla_32x33x32_serial")
@@ -174,10 +175,49 @@ def les():
     print("wrote", os.path.join(HERE, "les_32x33x32.npz"), sorted(res))
 
 
+def xys():
+    """x-y statistics fixture: the les_field start (every resolved mode
+    excited, so that the Reynolds stresses are well above round-off), no
+    model, maxit = 30 at dt = 0.01 with ixys = 9 and ixyss = 15.  boxxys runs
+    at the start of substeps 4, 13, 22 and 31 (bla.f:1830), i.e. on the
+    fields after 1, 4, 7 and 10 full steps; with statistics on, the stop
+    criteria are only checked when mod(it-1, ixys) = 0 (bla.f:2789), so the
+    run ends after 12 steps, and the file written then (bla.f:2907-2930)
+    holds the dt-weighted mean of the four samples (sumw = 0.04).  Keys: xys
+    (statistics 1-15, [15][ny][nx] as stored, divided by sumw), sumw, t, uend
+    (the end field)."""
+    if not os.path.exists(BIN):
+        subprocess.check_call([os.path.join(REPO, "oracle", "build_ref.sh"),
+                               str(NX), str(NY), str(NZ), "1", "no", "0",
+                               os.path.basename(BIN)])
+    work = tempfile.mkdtemp(prefix="simson_golden_xys.")
+    try:
+        ufile.write_u(os.path.join(work, "init.u"), les_field())
+        p = blai.BlaParams(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=30,
+                           dt=0.01, nst=3, xl=4 * math.pi, cflux=True, icfl=3,
+                           ixys=9, namxys="xy.stat", ixyss=15, txys=0.0)
+        with open(os.path.join(work, "bla.i"), "w") as f:
+            f.write(blai.format_channel(p))
+        with open(os.path.join(work, "out.txt"), "w") as out:
+            subprocess.check_call([BIN], cwd=work, stdout=out, stderr=subprocess.STDOUT)
+        log = open(os.path.join(work, "out.txt")).read()
+        if "bla terminated normally" not in log:
+            raise RuntimeError("reference bla (xys) did not terminate normally")
+        xf = stats.read_xys(os.path.join(work, "xy.stat"))
+        uend = ufile.read_u(os.path.join(work, "end.u")).u
+        np.savez_compressed(os.path.join(HERE, "xys_32x33x32.npz"), xys=xf.stats[:15],
+                            sumw=xf.sumw, t=xf.t, uend=uend)
+        print("wrote", os.path.join(HERE, "xys_32x33x32.npz"), xf.stats.shape, xf.sumw, xf.t)
+    finally:
+        shutil.rmtree(work)
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "pressure":
         pressure()
     elif len(sys.argv) > 1 and sys.argv[1] == "les":
         les()
+    elif len(sys.argv) > 1 and sys.argv[1] == "xys":
+        xys()
     else:
         main()

@@ -173,3 +173,66 @@ def test_fortran_host_matches_reference_bla(golden_c1):
         np.testing.assert_allclose(amp[:, 18], ref[:, 18], atol=1e-14)
     finally:
         shutil.rmtree(work)
+
+
+def _xys_dir():
+    """The x-y statistics run of tests/golden/xys_32x33x32.npz: the LES
+    start field, no model, maxit 30, ixys 9, ixyss 15."""
+    work = tempfile.mkdtemp(prefix="blagpu_xys.")
+    f = fields.channel_field(32, 33, 32, 4 * math.pi, 2 * math.pi, 3000.0,
+                             kmax=15, rms=1e-2, seed=5)
+    ufile.write_u(os.path.join(work, "init.u"), f)
+    p = blai.BlaParams(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=30, dt=0.01, nst=3,
+                       xl=4 * math.pi, cflux=True, icfl=3, ixys=9, namxys="xy.stat",
+                       ixyss=15, txys=0.0)
+    with open(os.path.join(work, "bla.i"), "w") as fh:
+        fh.write(blai.format_channel(p))
+    return work
+
+
+def test_rejects_correlations():
+    """corrf (two-point correlations) is read and refused before any device
+    call."""
+    _need_binary()
+    work = _xys_dir()
+    try:
+        path = os.path.join(work, "bla.i")
+        text = open(path).read().replace(".false.         Two-point correlations in z",
+                                         ".true.          Two-point correlations in z")
+        open(path, "w").write(text)
+        r = _run(work)
+        assert r.returncode != 0
+        assert "correlations" in r.stdout
+    finally:
+        shutil.rmtree(work)
+
+
+@pytest.mark.gpu
+def test_fortran_host_xys_matches_reference_bla():
+    """The host's statistics schedule (samples after 1, 4, 7, 10 steps, the
+    stop deferred to a statistics step, i.e. after 12 steps) and its wxys.f
+    file match the reference's: statistics 1-15 to the bounds of
+    tests/test_xys.py, the end field to 1e-12."""
+    from simson_amd import stats
+    _need_binary()
+    print("[gpu] blagpu with ixys 32x33x32", flush=True)
+    golden = np.load(os.path.join(ROOT, "tests", "golden", "xys_32x33x32.npz"))
+    work = _xys_dir()
+    try:
+        r = _run(work)
+        assert r.returncode == 0, r.stdout + r.stderr
+        xf = stats.read_xys(os.path.join(work, "xy.stat"))
+        assert (xf.nx, xf.ny, xf.nz) == (32, 33, 32)
+        assert abs(xf.sumw - float(golden["sumw"])) < 1e-15
+        assert abs(xf.t - float(golden["t"])) < 1e-12
+        ref = golden["xys"]
+        for i in range(15):
+            tol = 1e-10 if i in (6, 8, 9, 11) else 1e-12
+            err = np.abs(xf.stats[i] - ref[i]).max() / np.abs(ref[i]).max()
+            assert err < tol, (stats.XYS_NAMES[i], err)
+        u = np.delete(ufile.read_u(os.path.join(work, "end.u")).u, 16, axis=1)
+        rr = np.delete(golden["uend"], 16, axis=1)
+        for c in range(3):
+            assert _rel(u[c], rr[c]) <= 1e-12, c
+    finally:
+        shutil.rmtree(work)
