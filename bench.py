@@ -134,7 +134,8 @@ def time_form(precision: int, steps: int, warmup: int, model: str = ""):
     """Run one form of the step on the bench workload; returns steps/s,
     seconds, per-family (total ms, launches) and the last CFL*dt.
     model: "" (DNS), "adm-rt" (the relaxation model of sgs.i iles = 1 with
-    the default sgs.i) or "learned" (the closure network evaluated at every
+    the default sgs.i), "smag" (the dynamic Smagorinsky model of sgs.i
+    iles = 2, cs = 0, ineg = 1) or "learned" (the closure network evaluated at every
     point of the dealiased grid every substep; its cost does not depend on
     the weights, so the network keeps its seeded initial weights)."""
     import numpy as np
@@ -147,6 +148,8 @@ def time_form(precision: int, steps: int, warmup: int, model: str = ""):
     try:
         if model == "adm-rt":
             g.les(iles=1, cutoff=2.0 / 3.0, iord=5, chi=0.2)
+        elif model == "smag":
+            g.les(iles=2, cs=0.0, ineg=1)
         elif model == "learned":
             from simson_amd.sgsnet import SgsNet
             with SgsNet(1) as net:
@@ -218,9 +221,9 @@ def main():
     ap.add_argument("--form", choices=["both", "mixed", "fp64"], default="both",
                     help="which forms of the step to time (the headline value is the "
                          "mixed form; 'fp64' reports the FP64 step as the value)")
-    ap.add_argument("--models", default="adm-rt,learned",
+    ap.add_argument("--models", default="adm-rt,smag,learned",
                     help="subgrid models also timed on the mixed step (comma list of "
-                         "adm-rt, learned; empty for none), reported under forms")
+                         "adm-rt, smag, learned; empty for none), reported under forms")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy a device buffer of this size three times "
                          "(a plain copy of known bytes that scales rocprofv3 --pmc "

@@ -140,19 +140,30 @@ int simson_gpu_xys(simson_gpu_t *g, double dtn, double xs, double zs, int nxys, 
  * synchronises. */
 int simson_gpu_pressure(simson_gpu_t *g, double *pr, double *pi);
 
-/* ADM-RT large-eddy model: the relaxation term of nonlinbl.f:920-966 with
- * the high-pass filtered velocity of linearbl.f:1871-1962 (ihighorder = 2,
- * the filters of init_filter.f and bla.f:1494-1518), as bla runs it when
- * bla.i sets sgs and sgs.i selects iles = 1 (rparamles.f:62-71).
- * iles 0 turns the model off.  cutoff = 1/cutoff_inv of sgs.i (0 < cutoff
- * <= 1), 0 <= iord <= 5, and chi is the relaxation rate chi/dstar of
- * bla.f:2220 (dstar = 1 for the channel).  Call after init, before the
- * first step; needs nzp among the specialised z transform lengths. */
+/* Large-eddy models of bla when bla.i sets sgs (sgs.i, rparamles.f:62-81):
+ * - iles 1, ADM-RT: the relaxation term of nonlinbl.f:920-966 with the
+ *   high-pass filtered velocity of linearbl.f:1871-1962 (ihighorder = 2,
+ *   the filters of init_filter.f and bla.f:1494-1518).  cutoff =
+ *   1/cutoff_inv of sgs.i (0 < cutoff <= 1), 0 <= iord <= 5, and chi is the
+ *   relaxation rate chi/dstar of bla.f:2220 (dstar = 1 for the channel).
+ * - iles 2, Smagorinsky eddy viscosity: the stresses of les.f (called per
+ *   plane from bla.f:1945-1998 at the start of every substep) with a fixed
+ *   coefficient cs, or with cs = 0 the dynamic procedure (Germano/Lilly,
+ *   2-D spectral cutoff test filter, averaging in z) clipped by ineg (0:
+ *   none; 1: C >= 0; 2: nu + nu_t >= 0; rparamles.f:36-39), and their
+ *   divergence sgs_stress.f:62-131 subtracted from H (nonlinbl.f:1011-1050).
+ *   The reference compiles it in only with iiles_smag = 1 (bla.f:28).
+ *   Computed in FP64 in both precision forms.
+ * iles 0 turns the model off; the HPF eddy-viscosity model (iles 3) is
+ * refused (-1).  Call after init, before the first step; needs nzp among
+ * the specialised z transform lengths. */
 typedef struct simson_les {
-    int iles;            /* 0: no model; 1: ADM-RT */
-    int iord;            /* sgs.i iord */
-    double cutoff;       /* 1 / sgs.i cutoff_inv */
-    double chi;          /* sgs.i chi (divided by dstar) */
+    int iles;            /* 0: no model; 1: ADM-RT; 2: Smagorinsky */
+    int iord;            /* sgs.i iord (iles 1) */
+    double cutoff;       /* 1 / sgs.i cutoff_inv (iles 1) */
+    double chi;          /* sgs.i chi divided by dstar (iles 1) */
+    double cs;           /* sgs.i cs (iles 2; 0: dynamic procedure) */
+    int ineg;            /* sgs.i ineg (iles 2) */
 } simson_les_t;
 
 int simson_gpu_les(simson_gpu_t *g, const simson_les_t *p);

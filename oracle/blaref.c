@@ -348,6 +348,7 @@ struct orc {
     struct les_s *les;                 /* ADM-RT filters (orc_set_les) or NULL */
     double *gur, *gui;                 /* [3][nz][ny][nxh] high-pass velocities */
     double les_chi;                    /* chi/dstar of the relaxation term */
+    double *fr, *fi;                   /* [3][nz][ny][nxh] external force or NULL */
 };
 
 struct les_s;
@@ -412,6 +413,8 @@ void orc_destroy(orc_t *o)
     plan_free(&o->px_plan);
     plan_free(&o->pz_plan);
     les_free(o->les);
+    free(o->fr);
+    free(o->fi);
     free(o->gur);
     free(o->gui);
     free(o);
@@ -534,6 +537,15 @@ static double nonlin_plane(orc_t *o, double *ur, double *ui, int y, int do_cfl)
                     double c = o->les_chi * (double)nxp * (double)nzp;
                     ur[s] -= c * o->gur[s];
                     ui[s] -= c * o->gui[s];
+                }
+                /* eddy-viscosity SGS force sigma_i (nonlinbl.f:1011-1050):
+                 * subtracted with the factor nxp nzp of the unnormalised
+                 * transforms, not on the oddball row (getxz.f with ipad = 1
+                 * zeroes it) */
+                if (o->fr && z != nz / 2) {
+                    double c = (double)nxp * (double)nzp;
+                    ur[s] -= o->fr[s] * c;
+                    ui[s] -= o->fi[s] * c;
                 }
             }
         }
@@ -1462,6 +1474,26 @@ static void les_hpf_y(const les_t *L, double *d, double *rs)
     d[n - 2] = rs[n - 2] - od1[n - 2] * d[n - 1];
     for (int j = n - 3; j >= 0; j--)
         d[j] = rs[j] - od1[j] * d[j + 1] - od2[j] * d[j + 2];
+}
+
+/* External SGS force for the next orc_nonlin calls (sigma of the
+ * eddy-viscosity models, sgs_stress.f, at physical y): fr, fi
+ * [3][nz][ny][nxh] are copied; NULL switches it off. */
+void orc_set_force(orc_t *o, const double *fr, const double *fi)
+{
+    size_t n = (size_t)3 * o->nz * o->ny * o->nxh;
+    if (!fr || !fi) {
+        free(o->fr);
+        free(o->fi);
+        o->fr = o->fi = NULL;
+        return;
+    }
+    if (!o->fr) {
+        o->fr = malloc(n * sizeof(double));
+        o->fi = malloc(n * sizeof(double));
+    }
+    memcpy(o->fr, fr, n * sizeof(double));
+    memcpy(o->fi, fi, n * sizeof(double));
 }
 
 int orc_set_les(orc_t *o, int iles, double cutoff, int iord, double chi)

@@ -66,6 +66,11 @@ int orc_set_les(orc_t *o, int iles, double cutoff, int iord, double chi);
 /* ---- building blocks exposed for unit tests ---- */
 /* the ADM-RT filter operators: lpf[nz][nx/2] (bla.f:1503-1518) and the
  * wall-normal low-pass L = I - H_y^(iord+1) as a dense [ny][ny] matrix */
+/* External SGS force sigma [3][nz][ny][nxh] (physical y) subtracted from H
+ * (times nxp nzp, not on the z-oddball row) in later orc_nonlin calls, as
+ * nonlinbl.f:1011-1050 does for the eddy-viscosity models; NULL: off. */
+void orc_set_force(orc_t *o, const double *fr, const double *fi);
+
 int orc_les_filter(int nx, int ny, int nz, double cutoff, int iord, double *lpf, double *L);
 /* complex DFT, y_k = sum_j x_j exp(sign 2 pi i jk/n), in place on (re,im) */
 void orc_cdft(int n, double *re, double *im, int sign);

@@ -17,6 +17,9 @@
 #   bla.f     f(...,5+2*scalar,...)    -> max(7,5+2*scalar)
 #   bla.f     bis(...,6+2*scalar,...)  -> max(8,6+2*scalar)
 #   cbflow.f  bu1/bu2(nxp/2+1,nyp,3+scalar) -> assumed size (*)
+# With SMAG=1 in the environment the eddy-viscosity LES models are compiled
+# in: bla.f's compile-time flag iiles_smag = iiles*0 becomes iiles*1, the
+# switch bla.f:471 documents for allocating the dynamic-procedure arrays.
 set -euo pipefail
 
 if [ $# -ne 7 ]; then
@@ -52,6 +55,10 @@ sed -i -E 's/^( +real bu1\(nxp\/2\+1,nyp,)3\+scalar(\),bu2\(nxp\/2\+1,nyp,)3\+sc
 grep -q "max(7,5+2\*scalar)" bla.f
 grep -q "max(8,6+2\*scalar)" bla.f
 grep -q "bu1(nxp/2+1,nyp,\*),bu2(nxp/2+1,nyp,\*)" cbflow.f
+if [ "${SMAG:-0}" = "1" ]; then
+    sed -i -E 's/^( +integer,parameter :: iiles_smag = iiles)\*0$/\1*1/' bla.f
+    grep -q "iiles_smag = iiles\*1" bla.f
+fi
 
 # ppar.f includes 'latest_revision', which bla/Makefile:349-350 generates
 # before compiling; run the same generation step here.
@@ -79,4 +86,4 @@ fi
 JOBS=${REF_BUILD_JOBS:-8}
 echo $OBJS | tr ' ' '\n' | xargs -P "$JOBS" -I{} $FC $FLAGS -c {}.f -o {}.o
 $FC $FLAGS -o "$DEST/$OUT" $(for o in $OBJS; do echo "$o.o"; done)
-echo "built $DEST/$OUT (nx=$NX ny=$NY nz=$NZ nthread=$NTHREAD omp=$OMP pressure=$PRESSURE)"
+echo "built $DEST/$OUT (nx=$NX ny=$NY nz=$NZ nthread=$NTHREAD omp=$OMP pressure=$PRESSURE smag=${SMAG:-0})"

@@ -51,6 +51,7 @@ def lib():
         L.orc_pressure.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_set_les.restype = C.c_int
         L.orc_set_les.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double]
+        L.orc_set_force.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_les_filter.restype = C.c_int
         L.orc_les_filter.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _dp, _dp]
         _lib = L

@@ -17,9 +17,9 @@ const char *const kFamilies[] = {"zsyn", "xrow", "zana", "cfl_reduce", "cheb_fwd
                                  "tau", "cheb_bwd", "prhs_cheb_fwd", "prhs",
                                  "prhs_cheb_bwd", "tau_factor", "les_filter",
                                  "closure_zsyn", "closure_xrow", "closure_zana",
-                                 "closure_dy"};
+                                 "closure_dy", "smag"};
 enum Fam { F_ZSYN, F_XROW, F_ZANA, F_CFL, F_CHF, F_TAU, F_CHB, F_PCHF, F_PRHS, F_PCHB,
-           F_FAC, F_LES, F_CZS, F_CXR, F_CZA, F_CDY, F_N };
+           F_FAC, F_LES, F_CZS, F_CXR, F_CZA, F_CDY, F_SMAG, F_N };
 
 struct TimedLaunch {
     int fam;
@@ -87,6 +87,14 @@ struct simson_gpu {
     FftPlan pzs, pms;
     double2 *twzs = nullptr, *twxs = nullptr;
     double *xysd = nullptr;
+    // eddy-viscosity LES (simson_gpu_les with iles = 2): settings and work
+    // space of sg_smag.hip (SQ [12][ny][nz][nxh], SS [7][...], SW
+    // [21][ny][nz][nx/4] for the dynamic procedure), the filter width per y
+    bool smag = false;
+    SmagArgs sm{};
+    double2 *SQ = nullptr, *SS = nullptr, *SW = nullptr;
+    double *dlt2 = nullptr;
+    size_t smag_lds = 0;
     double mflux = 0., u0upp = 0., u0low = 0., w0upp = 0., w0low = 0.;
     bool timing = false;
     std::vector<TimedLaunch> launches;
@@ -179,7 +187,7 @@ void free_all(simson_gpu_t *g)
                     g->twz32, g->twx32, g->Tf32, g->Tb32, g->Q, g->Qc, g->PR, g->pscr,
                     g->Lm, g->lpf, g->Lm32, g->Wf, g->Wf32, g->clW, g->clY, g->PG, g->TAU,
                     g->DT, g->PG32, g->TAU32, g->DT32, g->Dm, g->Dm32, g->twzs,
-                    g->twxs, g->xysd};
+                    g->twxs, g->xysd, g->SQ, g->SS, g->SW, g->dlt2};
     for (void *q : ptrs)
         if (q)
             (void)hipFree(q);
@@ -385,6 +393,87 @@ void refresh_u32(simson_gpu_t *g)
     }
 }
 
+// the unpadded transforms of lengths nz and nx/2 (statistics, eddy
+// viscosity): plans and twiddles, made once
+int unpadded_plans(simson_gpu_t *g)
+{
+    if (g->twzs)
+        return 0;
+    if (!make_radices(g->nz, g->pzs) || !make_radices(g->nxh, g->pms)) {
+        g->err = "unsupported nx or nz for the unpadded transforms";
+        return SIMSON_E_ARG;
+    }
+    std::vector<double2> tz = roots(g->nz), tx = roots(g->nx);
+    if (dalloc(g, &g->twzs, tz.size()) || dalloc(g, &g->twxs, tx.size()) ||
+        h2d(g, g->twzs, tz.data(), tz.size()) || h2d(g, g->twxs, tx.data(), tx.size()))
+        return SIMSON_E_HIP;
+    g->pzs.tw = g->twzs;
+    g->pzs.tabn = g->nz;
+    g->pms.tw = g->twxs;          // length-nx/2 transform uses every other root
+    g->pms.tabn = g->nx;
+    return 0;
+}
+
+// stress spectra TAU [6][nz][ny][nxh], their y derivatives DT [3][...] and
+// the y differentiation matrix on the Gauss-Lobatto points, the
+// physical-space form of vchbf, dcheb, vchbb (sgs_stress.f:98-121); shared
+// by the learned closure and the eddy-viscosity model
+int alloc_stress(simson_gpu_t *g, bool fp64, bool fp32)
+{
+    const int ny = g->ny;
+    const size_t nt = 6 * field_size(g), nd = 3 * field_size(g);
+    if (!g->Dm) {
+        std::vector<double> D((size_t)ny * ny), y(ny), c(ny);
+        for (int j = 0; j < ny; j++) {
+            y[j] = cos(M_PI * j / (ny - 1));
+            c[j] = ((j == 0 || j == ny - 1) ? 2. : 1.) * ((j & 1) ? -1. : 1.);
+        }
+        for (int i = 0; i < ny; i++) {
+            double sum = 0.;
+            for (int j = 0; j < ny; j++)
+                if (i != j) {
+                    D[(size_t)i * ny + j] = c[i] / c[j] / (y[i] - y[j]);
+                    sum += D[(size_t)i * ny + j];
+                }
+            D[(size_t)i * ny + i] = -sum;
+        }
+        if (dalloc(g, &g->Dm, D.size()) || h2d(g, g->Dm, D.data(), D.size()))
+            return SIMSON_E_HIP;
+        if (g->mixed) {
+            std::vector<float> D32(D.begin(), D.end());
+            if (dalloc(g, &g->Dm32, D32.size()) || h2d(g, g->Dm32, D32.data(), D32.size()))
+                return SIMSON_E_HIP;
+        }
+    }
+    if (fp64 && !g->TAU && (dalloc(g, &g->TAU, nt) || dalloc(g, &g->DT, nd)))
+        return SIMSON_E_HIP;
+    if (fp32 && !g->TAU32 && (dalloc(g, &g->TAU32, nt) || dalloc(g, &g->DT32, nd)))
+        return SIMSON_E_HIP;
+    return 0;
+}
+
+// eddy-viscosity stresses of the current FP64 state into TAU (and TAU32 in
+// the mixed form) and their y derivatives into DT (DT32)
+void smag_stresses(simson_gpu_t *g)
+{
+    SmagArgs a = g->sm;
+    a.U = g->U;
+    a.u0low = g->u0low;
+    a.w0low = g->w0low;
+    timed(g, F_SMAG, [&] { launch_smag(a, g->pzs, g->pms, g->twxs, g->smag_lds, g->stream); });
+    if (g->mixed) {
+        launch_narrow(g->TAU, g->TAU32, 6 * field_size(g), g->stream);
+        GemmIO io = gemm_io(g, g->TAU32, 0, g->DT32, 0, {1, 3, 4}, {0, 1, 2}, g->nz / 2);
+        timed(g, F_CDY, [&] {
+            launch_cheb_mfma_f32(g->Dm32, (const float *)g->TAU32, (float *)g->DT32, io,
+                                 g->stream);
+        });
+    } else {
+        GemmIO io = gemm_io(g, g->TAU, 0, g->DT, 0, {1, 3, 4}, {0, 1, 2}, g->nz / 2);
+        timed(g, F_CDY, [&] { launch_cheb_gemm(g->Dm, io, g->stream); });
+    }
+}
+
 int do_nonlin(simson_gpu_t *g, int do_cfl)
 {
     const int odd = g->nz / 2;
@@ -404,6 +493,14 @@ int do_nonlin(simson_gpu_t *g, int do_cfl)
         if (g->clos) {
             closure_stresses<float2>(g, a, g->PG32, g->TAU32, g->DT32, g->twz32, g->twx32,
                                      g->Dm32, true);
+            a.clos = 1;
+            a.TAU = g->TAU32;
+            a.DT = g->DT32;
+        }
+        if (g->smag) {
+            // the model runs in FP64 on the FP64 image of the state
+            refresh_u64(g);
+            smag_stresses(g);
             a.clos = 1;
             a.TAU = g->TAU32;
             a.DT = g->DT32;
@@ -430,6 +527,12 @@ int do_nonlin(simson_gpu_t *g, int do_cfl)
     if (g->clos) {
         closure_stresses<double2>(g, a, g->PG, g->TAU, g->DT, (const double2 *)g->twz,
                                   (const double2 *)g->twx, g->Dm, false);
+        a.clos = 1;
+        a.TAU = g->TAU;
+        a.DT = g->DT;
+    }
+    if (g->smag) {
+        smag_stresses(g);
         a.clos = 1;
         a.TAU = g->TAU;
         a.DT = g->DT;
@@ -896,20 +999,14 @@ int simson_gpu_xys(simson_gpu_t *g, double dtn, double xs, double zs, int nxys, 
     }
     const int ny = g->ny, nx = g->nx;
     if (!g->xysd) {
-        if (!xys_fits(g->nz, g->nxh) || !make_radices(g->nz, g->pzs) ||
-            !make_radices(g->nxh, g->pms)) {
+        if (!xys_fits(g->nz, g->nxh)) {
             g->err = "xy statistics: unsupported nx or nz";
             return SIMSON_E_ARG;
         }
-        std::vector<double2> tz = roots(g->nz), tx = roots(nx);
-        if (dalloc(g, &g->twzs, tz.size()) || dalloc(g, &g->twxs, tx.size()) ||
-            dalloc(g, &g->xysd, (size_t)15 * ny * nx) || h2d(g, g->twzs, tz.data(), tz.size()) ||
-            h2d(g, g->twxs, tx.data(), tx.size()))
+        if (int st = unpadded_plans(g))
+            return st;
+        if (dalloc(g, &g->xysd, (size_t)15 * ny * nx))
             return SIMSON_E_HIP;
-        g->pzs.tw = g->twzs;
-        g->pzs.tabn = g->nz;
-        g->pms.tw = g->twxs;      // length-nx/2 transform uses every other root
-        g->pms.tabn = nx;
     }
     refresh_u64(g);
     XysArgs a{};
@@ -946,16 +1043,95 @@ int simson_les_filter(int nx, int ny, int nz, double cutoff, int iord, double *l
     return les_filter(nx, ny, nz, cutoff, iord, lpf, L);
 }
 
+// sgs.i iles = 2 (rparamles.f:72-81, 117-165): fixed cs or, with cs = 0,
+// the dynamic procedure with clipping ineg; Delta^2 of preprbl.f:60-71
+int setup_smag(simson_gpu_t *g, const simson_les_t *p)
+{
+    if (p->ineg < 0 || p->ineg > 2 || (p->cs != 0. && p->ineg != 1)) {
+        g->err = "Smagorinsky: 0 <= ineg <= 2, and ineg = 1 with a fixed cs";
+        return SIMSON_E_ARG;
+    }
+    int dev = 0, lmax = 0;
+    CK(hipGetDevice(&dev));
+    CK(hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    // the dynamic LDS of the x-row kernels, the static arrays of k_smag_xc
+    // (12 KiB) aside
+    const size_t lds = (size_t)std::max(65536, lmax) - 12288;
+    if (!smag_fits(g->nz, g->nxh, lds) || g->nx % 4 != 0 || !has_zfft4(g->nzp) ||
+        (g->mixed && cheb_mfma_ks(g->ny) == 0)) {
+        g->err = "Smagorinsky: unsupported grid size";
+        return SIMSON_E_ARG;
+    }
+    if (int st = unpadded_plans(g))
+        return st;
+    if (int st = alloc_stress(g, true, g->mixed))
+        return st;
+    const int ny = g->ny, nz = g->nz, nq = g->nx / 4;
+    const size_t F = field_size(g);
+    const bool dyn = p->cs == 0.;
+    if (!g->SQ) {
+        if (dalloc(g, &g->SQ, 12 * F) || dalloc(g, &g->SS, 7 * F) ||
+            dalloc(g, &g->dlt2, (size_t)ny))
+            return SIMSON_E_HIP;
+        if (smag_prepare(lds) != 0) {
+            g->err = "Smagorinsky: cannot raise the LDS limit of the x-row kernels";
+            return SIMSON_E_HIP;
+        }
+        std::vector<double> d2(ny), eta(ny);
+        for (int j = 0; j < ny; j++)
+            eta[j] = cos(M_PI * j / (ny - 1));
+        for (int j = 0; j < ny; j++) {
+            const double dy = (j == 0 || j == ny - 1)
+                                  ? 1. - eta[1]
+                                  : sqrt((eta[j + 1] - eta[j]) * (eta[j] - eta[j - 1]));
+            d2[j] = pow(g->p.xl / g->nx * g->p.zl / nz * dy, 2. / 3.);
+        }
+        if (h2d(g, g->dlt2, d2.data(), d2.size()))
+            return SIMSON_E_HIP;
+    }
+    if (dyn && !g->SW && dalloc(g, &g->SW, (size_t)21 * ny * nz * nq))
+        return SIMSON_E_HIP;
+    g->smag_lds = lds;
+    SmagArgs &a = g->sm;
+    a = SmagArgs{};
+    a.Q = g->SQ;
+    a.SS = g->SS;
+    a.W = g->SW;
+    a.TAU = g->TAU;
+    a.alfa = g->alfa;
+    a.beta = g->beta;
+    a.dlt2 = g->dlt2;
+    a.nx = g->nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.nxh = g->nxh;
+    a.nq = nq;
+    a.dynamic = dyn;
+    a.ineg = p->ineg;
+    a.cs2 = p->cs * p->cs;
+    a.tw2 = pow(4., 2. / 3.);
+    a.re = g->p.re;
+    a.hscale = (double)g->nxp * (double)g->nzp;
+    g->smag = true;
+    g->les = false;
+    g->clos = false;
+    return 0;
+}
+
 int simson_gpu_les(simson_gpu_t *g, const simson_les_t *p)
 {
     if (!g || !p)
         return SIMSON_E_ARG;
     if (p->iles == 0) {
         g->les = false;
+        g->smag = false;
         return 0;
     }
+    if (p->iles == 2)
+        return setup_smag(g, p);
     if (p->iles != 1) {
-        g->err = "only ADM-RT (iles = 1) is supported";
+        g->err = "iles must be 0, 1 (ADM-RT) or 2 (Smagorinsky); the HPF model (3) is not "
+                 "supported";
         return SIMSON_E_ARG;
     }
     if (!has_zfft4(g->nzp) || (g->mixed && cheb_mfma_ks(g->ny) == 0)) {
@@ -986,6 +1162,7 @@ int simson_gpu_les(simson_gpu_t *g, const simson_les_t *p)
     g->les_chi = p->chi;
     g->les = true;
     g->clos = false;
+    g->smag = false;
     return 0;
 }
 
@@ -1007,39 +1184,14 @@ int simson_gpu_closure(simson_gpu_t *g, const float *params, const double *in_sc
     }
     const int npar = simson_mlp_nparams(), ny = g->ny;
     if (!g->clW) {
-        const size_t npg = 9 * (size_t)ny * g->nzp * g->nxh, nt = 6 * field_size(g),
-                     nd = 3 * field_size(g);
-        if (dalloc(g, &g->clW, (size_t)npar) || dalloc(g, &g->clY, (size_t)ny) ||
-            dalloc(g, &g->Dm, (size_t)ny * ny))
+        const size_t npg = 9 * (size_t)ny * g->nzp * g->nxh;
+        if (dalloc(g, &g->clW, (size_t)npar) || dalloc(g, &g->clY, (size_t)ny))
             return SIMSON_E_HIP;
-        if (g->mixed ? (dalloc(g, &g->PG32, npg) || dalloc(g, &g->TAU32, nt) ||
-                        dalloc(g, &g->DT32, nd) || dalloc(g, &g->Dm32, (size_t)ny * ny))
-                     : (dalloc(g, &g->PG, npg) || dalloc(g, &g->TAU, nt) || dalloc(g, &g->DT, nd)))
+        if (g->mixed ? dalloc(g, &g->PG32, npg) : dalloc(g, &g->PG, npg))
             return SIMSON_E_HIP;
-        // d/dy on the Gauss-Lobatto points y_j = cos(pi j/(ny-1)): the
-        // physical-space form of vchbf, dcheb, vchbb (sgs_stress.f:98-121)
-        std::vector<double> D((size_t)ny * ny), y(ny), c(ny);
-        for (int j = 0; j < ny; j++) {
-            y[j] = cos(M_PI * j / (ny - 1));
-            c[j] = ((j == 0 || j == ny - 1) ? 2. : 1.) * ((j & 1) ? -1. : 1.);
-        }
-        for (int i = 0; i < ny; i++) {
-            double sum = 0.;
-            for (int j = 0; j < ny; j++)
-                if (i != j) {
-                    D[(size_t)i * ny + j] = c[i] / c[j] / (y[i] - y[j]);
-                    sum += D[(size_t)i * ny + j];
-                }
-            D[(size_t)i * ny + i] = -sum;
-        }
-        if (h2d(g, g->Dm, D.data(), D.size()))
-            return SIMSON_E_HIP;
-        if (g->mixed) {
-            std::vector<float> D32(D.begin(), D.end());
-            if (h2d(g, g->Dm32, D32.data(), D32.size()))
-                return SIMSON_E_HIP;
-        }
     }
+    if (int st = alloc_stress(g, !g->mixed, g->mixed))
+        return st;
     std::vector<float> yf(yfeat, yfeat + ny);
     if (h2d(g, g->clW, params, (size_t)npar) || h2d(g, g->clY, yf.data(), yf.size()))
         return SIMSON_E_HIP;
@@ -1061,6 +1213,7 @@ int simson_gpu_closure(simson_gpu_t *g, const float *params, const double *in_sc
     }
     g->clos = true;
     g->les = false;
+    g->smag = false;
     return 0;
 }
 

@@ -160,6 +160,21 @@ struct XysArgs {
     double c1, u0low, w0low, xs, zs;
     int shift;
 };
+// eddy-viscosity LES (sg_smag.hip): stresses into TAU [6][nz][ny][nxh],
+// scaled by nxp nzp like the learned closure's; Q [12][ny][nz][nxh], SS
+// [7][ny][nz][nxh] and (dynamic) W [21][ny][nz][nx/4] are work space
+struct SmagArgs {
+    const double2 *U;
+    double2 *Q, *SS, *W, *TAU;
+    const double *alfa, *beta, *dlt2;
+    int nx, ny, nz, nxh, nq;
+    int dynamic, ineg, rt;
+    double u0low, w0low, cs2, tw2, re, hscale;
+};
+bool smag_fits(int nz, int nxh, size_t lds_max);
+int smag_prepare(size_t lds_max);
+void launch_smag(SmagArgs a, const FftPlan &pz, const FftPlan &pm, const double2 *wx,
+                 size_t lds_max, hipStream_t s);
 bool xys_fits(int nz, int nxh);
 void launch_xys(const XysArgs &a, const FftPlan &pz, const FftPlan &pm, const double2 *wx,
                 hipStream_t s);

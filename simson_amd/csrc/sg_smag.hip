@@ -1,0 +1,459 @@
+// Eddy-viscosity LES of bla (sgs.i iles = 2, les.f with iwale = 0, no
+// scalar): the stresses tau_ij = -2 nu_t S_ij of the Smagorinsky model with
+// a fixed coefficient (C = cs^2) or the dynamic procedure (Germano/Lilly:
+// C(x, y) = 1/2 <M_ij L_ij>_z / <M_ij M_ij>_z with the 2-D spectral cutoff
+// test filter of filter_cutoff.f), on the unpadded nx x nz grid of every y
+// plane, as les.f does (called per plane from bla.f:1945-1998 at the start
+// of each substep).
+//
+// Kernels (all y planes at once):
+//   k_smag_z   u, v, w and the nine velocity gradients (les.f:278-350, from
+//              the stored velocities and om_x, om_z) synthesised in z
+//              (vcfftb, length nz) into Q [12][ny][nz][nx/2];
+//   k_smag_xa  per row: inverse real x transform (vrfftb), S_ij and |S|
+//              (les.f:627-645) kept at physical x-z in SS [7][ny][nz][nx/2]
+//              (even points in .x, odd in .y); for the dynamic procedure the
+//              21 fields to be test-filtered (-u_i u_j, -S_ij |S| Delta^2,
+//              S_ij, u_i: les.f:717-780), forward x transform and the x
+//              cutoff kx < nx/4, into W [21][ny][nz][nx/4];
+//   k_smag_zf  the z part of the test filter: forward z transform, cutoff
+//              |kz| <= nz/4, inverse, scaling 1/(nx nz) (filter_cutoff.f);
+//   k_smag_xc  one block per y plane: inverse x transform of the filtered
+//              fields, L_ij, M_ij, the z sums of M.L and M.M and the clipped
+//              coefficient C(x) (les.f:784-880, 1380-1510), then nu_t and
+//              tau_ij / (nx nz) with the forward x transform
+//              (les.f:1530-1620), into Q as [6][ny][nz][nx/2];
+//   k_smag_za  forward z transform (vcfftf), the oddball row zeroed
+//              (les.f:1630-1645), scaled by nxp nzp to the normalisation of
+//              the step's nonlinear term, into TAU [6][nz][ny][nx/2].
+// The force sigma_i = d tau_ij/dx_j (sgs_stress.f) and its subtraction
+// from H are the learned closure's path (the D-matrix product and k_zana4).
+#include <algorithm>
+
+#include "sg_fft.h"
+#include "sg_internal.h"
+
+namespace sg {
+
+namespace {
+
+constexpr int kT = 256;
+constexpr int kLdsZ = 2048;        // complex values per LDS buffer of the z kernels
+constexpr int kMaxNx = 512;
+
+__device__ __forceinline__ size_t qidx(int f, int y, int z, int x, int ny, int nz, int nx)
+{
+    return (((size_t)f * ny + y) * nz + z) * nx + x;
+}
+
+__device__ __forceinline__ size_t sidx(int s, int z, int y, int x, int nz, int ny, int nxh)
+{
+    return (((size_t)s * nz + z) * ny + y) * nxh + x;
+}
+
+// |S| = sqrt(2 S_ij S_ij) for the order 11 12 13 22 23 33
+__device__ __forceinline__ double sabs_of(const double *s)
+{
+    return sqrt(2. * (s[0] * s[0] + s[3] * s[3] + s[5] * s[5]) +
+                4. * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
+}
+
+__global__ void __launch_bounds__(kT) k_smag_z(SmagArgs a, FftPlan pz, int B)
+{
+    extern __shared__ double2 lds[];
+    const int nz = a.nz, nxh = a.nxh, ny = a.ny;
+    const size_t F = (size_t)nz * ny * nxh;
+    double2 *buf0 = lds, *buf1 = lds + (size_t)B * nz;
+    const int f = blockIdx.y, y = blockIdx.z, x0 = blockIdx.x * B;
+    const int nb = min(B, nxh - x0);
+    for (int idx = threadIdx.x; idx < nz * B; idx += blockDim.x) {
+        const int z = idx / B, b = idx - z * B;
+        double2 v = make_double2(0., 0.);
+        if (b < nb && z != nz / 2) {
+            const int x = x0 + b;
+            const size_t o = sidx(0, z, y, x, nz, ny, nxh);
+            const double al = a.alfa[x], be = a.beta[z];
+            if (f < 3) {
+                v = a.U[f * F + o];
+                if (x == 0 && z == 0) {
+                    if (f == 0)
+                        v.x -= a.u0low;
+                    if (f == 2)
+                        v.x -= a.w0low;
+                }
+            } else {
+                const int i = (f - 3) / 3, j = (f - 3) - 3 * i;
+                if (j != 1) {
+                    const double k = j == 0 ? al : be;
+                    const double2 w = a.U[i * F + o];
+                    v = make_double2(-k * w.y, k * w.x);
+                } else if (i == 0) {        // du/dy = i alpha v - om_z
+                    const double2 vv = a.U[F + o], oz = a.U[4 * F + o];
+                    v = make_double2(-al * vv.y - oz.x, al * vv.x - oz.y);
+                } else if (i == 1) {        // dv/dy = -i alpha u - i beta w
+                    const double2 uu = a.U[o], ww = a.U[2 * F + o];
+                    v = make_double2(al * uu.y + be * ww.y, -al * uu.x - be * ww.x);
+                } else {                    // dw/dy = om_x + i beta v
+                    const double2 ox = a.U[3 * F + o], vv = a.U[F + o];
+                    v = make_double2(ox.x - be * vv.y, ox.y + be * vv.x);
+                }
+            }
+        }
+        buf0[b * nz + z] = v;
+    }
+    __syncthreads();
+    const double2 *res = lds_fft(buf0, buf1, nz, B, pz, +1.0);
+    for (int idx = threadIdx.x; idx < nz * B; idx += blockDim.x) {
+        const int z = idx / B, b = idx - z * B;
+        if (b < nb)
+            a.Q[qidx(f, y, z, x0 + b, ny, nz, nxh)] = res[b * nz + z];
+    }
+}
+
+// half spectrum row (k < nk given, zero above) -> packed Z_k for the
+// inverse real transform of length 2M (only Re X_0, no Nyquist term)
+__device__ __forceinline__ double2 pack_row(const double2 *row, int k, int M, int nk,
+                                            const double2 *wx)
+{
+    double2 Xk = k < nk ? row[k] : make_double2(0., 0.);
+    if (k == 0)
+        Xk.y = 0.;
+    const int km = M - k;
+    double2 Xm = (k > 0 && km < nk) ? row[km] : make_double2(0., 0.);
+    Xm.y = -Xm.y;
+    return cadd(cadd(Xk, Xm), mul_is(cmul(wx[k], csub(Xk, Xm)), 1.0));
+}
+
+// packed forward transform result -> X_k, k < M
+__device__ __forceinline__ double2 unpack_row(const double2 *fin, int k, int M,
+                                              const double2 *wx)
+{
+    const double2 Zk = fin[k];
+    const double2 Zm = conjd(fin[k == 0 ? 0 : M - k]);
+    const double2 E = make_double2(0.5 * (Zk.x + Zm.x), 0.5 * (Zk.y + Zm.y));
+    const double2 dd = csub(Zk, Zm);
+    const double2 O = make_double2(0.5 * dd.y, -0.5 * dd.x);
+    return cadd(E, cmul(conjd(wx[k]), O));
+}
+
+__global__ void __launch_bounds__(kT) k_smag_xa(SmagArgs a, FftPlan pm,
+                                                const double2 *__restrict__ wx, int RW)
+{
+    extern __shared__ double2 lds[];
+    const int M = a.nxh, nz = a.nz, ny = a.ny, nq = a.nq;
+    const int NF = a.dynamic ? 21 : 12;      // capacity of each buffer in rows of fields
+    double2 *A = lds, *Bf = lds + (size_t)NF * RW * M;
+    const int z0 = blockIdx.x * RW, y = blockIdx.y;
+    const int nr = min(RW, nz - z0);
+    for (int idx = threadIdx.x; idx < 12 * RW * M; idx += blockDim.x) {
+        const int f = idx / (RW * M);
+        const int rem = idx - f * RW * M;
+        const int r = rem / M, k = rem - r * M;
+        double2 Z = make_double2(0., 0.);
+        if (r < nr)
+            Z = pack_row(a.Q + qidx(f, y, z0 + r, 0, ny, nz, M), k, M, M, wx);
+        A[(f * RW + r) * M + k] = Z;
+    }
+    __syncthreads();
+    double2 *res = lds_fft(A, Bf, M, 12 * RW, pm, +1.0);
+    double2 *oth = res == A ? Bf : A;
+    const double d2 = a.dlt2[y];
+    for (int idx = threadIdx.x; idx < RW * M; idx += blockDim.x) {
+        const int r = idx / M, n = idx - r * M;
+        double2 q[12];
+#pragma unroll
+        for (int f = 0; f < 12; f++)
+            q[f] = res[(f * RW + r) * M + n];
+        // even (e = 0) and odd (e = 1) physical points of the packed row
+        double o[2][21], sv[2][7];
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            double u[3], g[9];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                u[i] = e ? q[i].y : q[i].x;
+#pragma unroll
+            for (int i = 0; i < 9; i++)
+                g[i] = e ? q[3 + i].y : q[3 + i].x;
+            // g[3 i + j] = du_i/dx_j
+            double *sk = sv[e];
+            sk[0] = g[0];
+            sk[1] = .5 * (g[3] + g[1]);
+            sk[2] = .5 * (g[6] + g[2]);
+            sk[3] = g[4];
+            sk[4] = .5 * (g[5] + g[7]);
+            sk[5] = g[8];
+            sk[6] = sabs_of(sk);
+            const double dr = sk[6] * d2;
+            const int pa[6] = {0, 1, 2, 1, 2, 2}, pb[6] = {0, 0, 0, 1, 1, 2};
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                o[e][k] = -u[pa[k]] * u[pb[k]];
+                o[e][6 + k] = -sk[k] * dr;
+                o[e][12 + k] = sk[k];
+            }
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                o[e][18 + i] = u[i];
+        }
+        if (r < nr) {
+#pragma unroll
+            for (int k = 0; k < 7; k++)
+                a.SS[qidx(k, y, z0 + r, n, ny, nz, M)] = make_double2(sv[0][k], sv[1][k]);
+        }
+        if (a.dynamic) {
+#pragma unroll
+            for (int f = 0; f < 21; f++)
+                oth[(f * RW + r) * M + n] = make_double2(o[0][f], o[1][f]);
+        }
+    }
+    if (!a.dynamic)
+        return;
+    __syncthreads();
+    double2 *fin = lds_fft(oth, res, M, 21 * RW, pm, -1.0);
+    for (int idx = threadIdx.x; idx < 21 * RW * nq; idx += blockDim.x) {
+        const int f = idx / (RW * nq);
+        const int rem = idx - f * RW * nq;
+        const int r = rem / nq, k = rem - r * nq;
+        if (r < nr)
+            a.W[qidx(f, y, z0 + r, k, ny, nz, nq)] = unpack_row(fin + (f * RW + r) * M, k, M, wx);
+    }
+}
+
+// z part of the test filter on the nx/4 kept columns, scaled by 1/(nx nz)
+__global__ void __launch_bounds__(kT) k_smag_zf(SmagArgs a, FftPlan pz, int B)
+{
+    extern __shared__ double2 lds[];
+    const int nz = a.nz, ny = a.ny, nq = a.nq;
+    double2 *buf0 = lds, *buf1 = lds + (size_t)B * nz;
+    const int f = blockIdx.y, y = blockIdx.z, x0 = blockIdx.x * B;
+    const int nb = min(B, nq - x0);
+    double2 *W = a.W + qidx(f, y, 0, x0, ny, nz, nq);
+    for (int idx = threadIdx.x; idx < nz * B; idx += blockDim.x) {
+        const int z = idx / B, b = idx - z * B;
+        buf0[b * nz + z] = b < nb ? W[(size_t)z * nq + b] : make_double2(0., 0.);
+    }
+    __syncthreads();
+    double2 *sp = lds_fft(buf0, buf1, nz, B, pz, -1.0);
+    for (int idx = threadIdx.x; idx < nz * B; idx += blockDim.x) {
+        const int z = idx % nz;
+        if (z > nz / 4 && z < nz - nz / 4)
+            sp[idx] = make_double2(0., 0.);
+    }
+    __syncthreads();
+    double2 *oth = sp == buf0 ? buf1 : buf0;
+    const double2 *res = lds_fft(sp, oth, nz, B, pz, +1.0);
+    const double sc = (double)a.nx * (double)nz;
+    for (int idx = threadIdx.x; idx < nz * B; idx += blockDim.x) {
+        const int z = idx / B, b = idx - z * B;
+        if (b < nb) {
+            const double2 v = res[b * nz + z];
+            W[(size_t)z * nq + b] = make_double2(v.x / sc, v.y / sc);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kT) k_smag_xc(SmagArgs a, FftPlan pm,
+                                                const double2 *__restrict__ wx, int RW)
+{
+    extern __shared__ double2 lds[];
+    __shared__ double red[2 * 2 * kT];
+    __shared__ double cx[kMaxNx];
+    const int M = a.nxh, nz = a.nz, ny = a.ny, nq = a.nq, y = blockIdx.x;
+    const int nx = 2 * M, tid = threadIdx.x;
+    const double d2 = a.dlt2[y];
+    if (a.dynamic) {
+        double2 *A = lds, *Bf = lds + (size_t)21 * RW * M;
+        const int G = kT / M, n = tid % M, grp = tid / M;
+        double num[2] = {0., 0.}, den[2] = {0., 0.};
+        const double gd2 = a.tw2 * d2;
+        for (int z0 = 0; z0 < nz; z0 += RW) {
+            const int nr = min(RW, nz - z0);
+            for (int idx = tid; idx < 21 * RW * M; idx += blockDim.x) {
+                const int f = idx / (RW * M);
+                const int rem = idx - f * RW * M;
+                const int r = rem / M, k = rem - r * M;
+                double2 Z = make_double2(0., 0.);
+                if (r < nr)
+                    Z = pack_row(a.W + qidx(f, y, z0 + r, 0, ny, nz, nq), k, M, nq, wx);
+                A[(f * RW + r) * M + k] = Z;
+            }
+            __syncthreads();
+            const double2 *res = lds_fft(A, Bf, M, 21 * RW, pm, +1.0);
+            if (grp < G) {
+                for (int r = grp; r < nr; r += G) {
+                    double2 q[21];
+#pragma unroll
+                    for (int f = 0; f < 21; f++)
+                        q[f] = res[(f * RW + r) * M + n];
+#pragma unroll
+                    for (int e = 0; e < 2; e++) {
+                        double v[21];
+#pragma unroll
+                        for (int f = 0; f < 21; f++)
+                            v[f] = e ? q[f].y : q[f].x;
+                        const int pa[6] = {0, 1, 2, 1, 2, 2}, pb[6] = {0, 0, 0, 1, 1, 2};
+                        double L[6], Mm[6];
+                        const double ts = sabs_of(v + 12);
+                        const double dr = gd2 * ts;
+#pragma unroll
+                        for (int k = 0; k < 6; k++) {
+                            L[k] = v[k] + v[18 + pa[k]] * v[18 + pb[k]];
+                            Mm[k] = v[6 + k] + v[12 + k] * dr;
+                        }
+                        num[e] += Mm[0] * L[0] + Mm[3] * L[3] + Mm[5] * L[5] +
+                                  2. * (Mm[1] * L[1] + Mm[2] * L[2] + Mm[4] * L[4]);
+                        den[e] += Mm[0] * Mm[0] + Mm[3] * Mm[3] + Mm[5] * Mm[5] +
+                                  2. * (Mm[1] * Mm[1] + Mm[2] * Mm[2] + Mm[4] * Mm[4]);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (grp < G) {
+            red[(grp * 2 + 0) * 2 * M + 2 * n] = num[0];
+            red[(grp * 2 + 0) * 2 * M + 2 * n + 1] = num[1];
+            red[(grp * 2 + 1) * 2 * M + 2 * n] = den[0];
+            red[(grp * 2 + 1) * 2 * M + 2 * n + 1] = den[1];
+        }
+        __syncthreads();
+        for (int x = tid; x < nx; x += blockDim.x) {
+            double cn = 0., cd = 0.;
+            for (int gq = 0; gq < G; gq++) {
+                cn += red[(gq * 2 + 0) * nx + x];
+                cd += red[(gq * 2 + 1) * nx + x];
+            }
+            double c = .5 * cn / cd;
+            if (a.ineg == 1)
+                c = fmax(c, 0.);
+            if (a.ineg != 0 && (y == 0 || y == ny - 1))
+                c = 0.;
+            cx[x] = c;
+        }
+    } else {
+        for (int x = tid; x < nx; x += blockDim.x)
+            cx[x] = a.cs2;
+    }
+    __syncthreads();
+    // tau_ij / (nx nz) at physical x-z, then the forward x transform
+    const int RT = a.rt;
+    double2 *A = lds, *Bf = lds + (size_t)6 * RT * M;
+    const double gd = 1. / ((double)nx * (double)nz);
+    for (int z0 = 0; z0 < nz; z0 += RT) {
+        const int nr = min(RT, nz - z0);
+        for (int idx = tid; idx < RT * M; idx += blockDim.x) {
+            const int r = idx / M, n = idx - r * M;
+            double2 t[6];
+#pragma unroll
+            for (int k = 0; k < 6; k++)
+                t[k] = make_double2(0., 0.);
+            if (r < nr) {
+                const double2 sa = a.SS[qidx(6, y, z0 + r, n, ny, nz, M)];
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    double nu = cx[2 * n + e] * (e ? sa.y : sa.x) * d2;
+                    if (a.ineg == 2 && nu < -1. / a.re)
+                        nu = -1. / a.re;
+                    const double d1 = -2. * nu * gd;
+#pragma unroll
+                    for (int k = 0; k < 6; k++) {
+                        const double2 s = a.SS[qidx(k, y, z0 + r, n, ny, nz, M)];
+                        (e ? t[k].y : t[k].x) = d1 * (e ? s.y : s.x);
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 6; k++)
+                A[(k * RT + r) * M + n] = t[k];
+        }
+        __syncthreads();
+        const double2 *fin = lds_fft(A, Bf, M, 6 * RT, pm, -1.0);
+        for (int idx = tid; idx < 6 * RT * M; idx += blockDim.x) {
+            const int k = idx / (RT * M);
+            const int rem = idx - k * RT * M;
+            const int r = rem / M, kx = rem - r * M;
+            if (r < nr)
+                a.Q[qidx(k, y, z0 + r, kx, ny, nz, M)] = unpack_row(fin + (k * RT + r) * M, kx, M, wx);
+        }
+        __syncthreads();
+    }
+}
+
+// forward z transform of the six stresses, scaled to the nonlinear term's
+// normalisation, the z oddball row zeroed
+__global__ void __launch_bounds__(kT) k_smag_za(SmagArgs a, FftPlan pz, int B)
+{
+    extern __shared__ double2 lds[];
+    const int nz = a.nz, ny = a.ny, nxh = a.nxh;
+    double2 *buf0 = lds, *buf1 = lds + (size_t)B * nz;
+    const int f = blockIdx.y, y = blockIdx.z, x0 = blockIdx.x * B;
+    const int nb = min(B, nxh - x0);
+    for (int idx = threadIdx.x; idx < nz * B; idx += blockDim.x) {
+        const int z = idx / B, b = idx - z * B;
+        buf0[b * nz + z] =
+            b < nb ? a.Q[qidx(f, y, z, x0 + b, ny, nz, nxh)] : make_double2(0., 0.);
+    }
+    __syncthreads();
+    const double2 *res = lds_fft(buf0, buf1, nz, B, pz, -1.0);
+    for (int idx = threadIdx.x; idx < nz * B; idx += blockDim.x) {
+        const int z = idx / B, b = idx - z * B;
+        if (b < nb) {
+            double2 v = res[b * nz + z];
+            if (z == nz / 2)
+                v = make_double2(0., 0.);
+            a.TAU[sidx(f, z, y, x0 + b, nz, ny, nxh)] =
+                make_double2(v.x * a.hscale, v.y * a.hscale);
+        }
+    }
+}
+
+int zblock(int nz, int ncol) { return std::max(1, std::min(ncol, kLdsZ / nz)); }
+
+}  // namespace
+
+int smag_rows(int nxh, int nfield, size_t lds_max)
+{
+    const size_t per = (size_t)2 * nfield * nxh * sizeof(double2);
+    return (int)std::max<size_t>(1, lds_max / per);
+}
+
+bool smag_fits(int nz, int nxh, size_t lds_max)
+{
+    return nz <= kLdsZ && 2 * nxh <= kMaxNx && nxh <= kT &&
+           (size_t)2 * 21 * nxh * sizeof(double2) <= lds_max;
+}
+
+int smag_prepare(size_t lds_max)
+{
+    const void *fns[] = {(const void *)k_smag_xa, (const void *)k_smag_xc};
+    for (const void *fn : fns)
+        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max) !=
+            hipSuccess)
+            return -1;
+    return 0;
+}
+
+void launch_smag(SmagArgs a, const FftPlan &pz, const FftPlan &pm, const double2 *wx,
+                 size_t lds_max, hipStream_t s)
+{
+    const int B = zblock(a.nz, a.nxh);
+    const size_t lz = (size_t)2 * B * a.nz * sizeof(double2);
+    hipLaunchKernelGGL(k_smag_z, dim3((a.nxh + B - 1) / B, 12, a.ny), dim3(kT), lz, s, a, pz, B);
+    const int NF = a.dynamic ? 21 : 12;
+    const int RA = std::min(a.nz, smag_rows(a.nxh, NF, lds_max));
+    hipLaunchKernelGGL(k_smag_xa, dim3((a.nz + RA - 1) / RA, a.ny), dim3(kT),
+                       (size_t)2 * NF * RA * a.nxh * sizeof(double2), s, a, pm, wx, RA);
+    if (a.dynamic) {
+        const int Bq = zblock(a.nz, a.nq);
+        hipLaunchKernelGGL(k_smag_zf, dim3((a.nq + Bq - 1) / Bq, 21, a.ny), dim3(kT),
+                           (size_t)2 * Bq * a.nz * sizeof(double2), s, a, pz, Bq);
+    }
+    const int RC = a.dynamic ? std::min(a.nz, smag_rows(a.nxh, 21, lds_max)) : 1;
+    a.rt = std::min(a.nz, smag_rows(a.nxh, 6, lds_max));
+    const size_t lc = std::max((size_t)2 * 21 * RC * a.nxh * (a.dynamic ? 1 : 0),
+                               (size_t)2 * 6 * a.rt * a.nxh) * sizeof(double2);
+    hipLaunchKernelGGL(k_smag_xc, dim3(a.ny), dim3(kT), lc, s, a, pm, wx, RC);
+    hipLaunchKernelGGL(k_smag_za, dim3((a.nxh + B - 1) / B, 6, a.ny), dim3(kT), lz, s, a, pz, B);
+}
+
+}  // namespace sg

@@ -9,7 +9,7 @@
 ! the stop criteria of bla/bla.f:2779-2833, the amp.stat writer of
 ! bla/wampbl.f (channel), the x-y statistics schedule of bla/bla.f:1830,
 ! 2089-2142, 2907-2930 and file of bla/wxys.f, the sgs.i reader of
-! bla/rparamles.f (ADM-RT) --
+! bla/rparamles.f (ADM-RT and the Smagorinsky models) --
 ! and calls the library for prhs (bla.f:1562),
 ! the nonlinbl plane loop (bla.f:2251-2288), its plane statistics
 ! (boxamp.f, nonlinbl.f:430; boxxys.f) and the linearbl plane loop
@@ -141,6 +141,7 @@ program blagpu
   ! sgs.i (rparamles.f:23-165): only ADM-RT (iles = 1) or no model
   if (sgs) then
      les%iles = 1; cutoff_inv = 1.5_dp; les%iord = 5; les%chi = 0.2_dp
+     les%cs = 0._dp; les%ineg = 1
      open(10, file='sgs.i', status='old')
      call nxt(line); read(line, *) sgsver
      if (sgsver /= 20060909) call stopnow(345432)
@@ -149,14 +150,22 @@ program blagpu
         call nxt(line); read(line, *) cutoff_inv
         call nxt(line); read(line, *) les%iord
         call nxt(line); read(line, *) les%chi
+     else if (les%iles == 2) then
+        ! rparamles.f:72-81: cs, and ineg for the dynamic procedure (cs = 0)
+        call nxt(line); read(line, *) les%cs
+        if (les%cs == 0._dp) then
+           call nxt(line); read(line, *) les%ineg
+        end if
      else if (les%iles /= 0) then
-        write(*,*) 'blagpu: only the ADM-RT model (iles = 1) is supported'
+        write(*,*) 'blagpu: the HPF eddy-viscosity model (iles = 3) is not supported'
         call stopnow(45543)
      end if
      close(10)
      les%cutoff = 1._dp / cutoff_inv
      if (les%cutoff < 0._dp .or. les%cutoff > 1._dp .or. les%iord < 0 .or. les%iord > 5) &
           call stopnow(45543)
+     if (les%ineg < 0 .or. les%ineg > 2) call stopnow(45543)
+     if (les%cs /= 0._dp .and. les%ineg /= 1) call stopnow(45543)
   end if
 
   ! ---------------- .u file (rdiscbl.f) ----------------

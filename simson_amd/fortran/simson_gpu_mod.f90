@@ -24,6 +24,8 @@ module simson_gpu_mod
      integer(c_int) :: iord
      real(c_double) :: cutoff
      real(c_double) :: chi
+     real(c_double) :: cs
+     integer(c_int) :: ineg
   end type simson_les_t
 
   interface

@@ -48,7 +48,7 @@ class SimsonParams(C.Structure):
 
 class SimsonLes(C.Structure):
     _fields_ = [("iles", C.c_int), ("iord", C.c_int), ("cutoff", C.c_double),
-                ("chi", C.c_double)]
+                ("chi", C.c_double), ("cs", C.c_double), ("ineg", C.c_int)]
 
 
 class SimsonError(RuntimeError):
@@ -208,10 +208,16 @@ class BlaGPU:
     def timing(self, on: bool):
         self._check(lib().simson_gpu_timing_enable(self.h, int(on)))
 
-    def les(self, iles: int = 1, cutoff: float = 2.0 / 3.0, iord: int = 5, chi: float = 0.2):
-        """ADM-RT relaxation model (sgs.i iles = 1; cutoff = 1/cutoff_inv),
-        or none with iles = 0."""
-        p = SimsonLes(iles, iord, cutoff, chi)
+    def les(self, iles=1, cutoff: float = 2.0 / 3.0, iord: int = 5, chi: float = 0.2,
+            cs: float = 0.0, ineg: int = 1):
+        """Large-eddy model of sgs.i: iles = 1 ADM-RT (cutoff = 1/cutoff_inv),
+        iles = 2 Smagorinsky (cs = 0: dynamic procedure, clipping ineg), 0
+        none.  iles may also be a simson_amd.sgsi.SgsParams."""
+        if not isinstance(iles, int):
+            sp = iles
+            iles, cutoff, iord, chi = sp.iles, sp.cutoff, sp.iord, sp.chi
+            cs, ineg = sp.cs, sp.ineg
+        p = SimsonLes(iles, iord, cutoff, chi, cs, ineg)
         self._check(lib().simson_gpu_les(self.h, C.byref(p)))
 
     def closure(self, params, in_scale=None, in_shift=None, yfeat=None, out_scale=None,

@@ -3,13 +3,16 @@ sets ``sgs``).
 
 Grammar (version 20060909): comment lines start with ``#``; the values are
 ``version``, ``iles`` and, for ``iles = 1`` (ADM-RT, the relaxation-term
-model), ``cutoff_inv``, ``iord`` and ``chi`` (rparamles.f:62-71).  The
-defaults of rparamles.f:28-51 apply to values a model does not read.  The
-checks of rparamles.f:117-165 are repeated (cutoff = 1/cutoff_inv in [0, 1],
-0 <= iord <= 5); like the reference, a bad file stops with code 45543.
+model), ``cutoff_inv``, ``iord`` and ``chi`` (rparamles.f:62-71); for
+``iles = 2`` (Smagorinsky), ``cs`` and, when cs = 0 (the dynamic
+procedure), ``ineg`` (rparamles.f:72-81).  The defaults of
+rparamles.f:28-51 apply to values a model does not read.  The checks of
+rparamles.f:117-165 are repeated (cutoff = 1/cutoff_inv in [0, 1],
+0 <= iord <= 5, 0 <= ineg <= 2, ineg = 1 for a fixed cs); like the
+reference, a bad file stops with code 45543.
 
-Only iles 0 and 1 are supported by the step library: the eddy-viscosity
-models (iles 2 and 3) need the reference compiled with iiles_smag = 1.
+The step library supports iles 0, 1 and 2; the HPF eddy-viscosity model
+(iles 3) is refused.
 """
 from __future__ import annotations
 
@@ -30,6 +33,8 @@ class SgsParams:
     cutoff_inv: float = 1.5
     iord: int = 5
     chi: float = 0.2
+    cs: float = 0.0
+    ineg: int = 1
 
     @property
     def cutoff(self) -> float:
@@ -56,8 +61,12 @@ def parse(text: str) -> SgsParams:
             p.cutoff_inv = float(next(it).replace("d", "e").replace("D", "e"))
             p.iord = int(next(it))
             p.chi = float(next(it).replace("d", "e").replace("D", "e"))
-        elif p.iles in (2, 3):
-            raise SgsInputError("eddy-viscosity LES (iles 2, 3) is not supported")
+        elif p.iles == 2:
+            p.cs = float(next(it).replace("d", "e").replace("D", "e"))
+            if p.cs == 0.0:
+                p.ineg = int(next(it))
+        elif p.iles == 3:
+            raise SgsInputError("the HPF eddy-viscosity model (iles 3) is not supported")
         elif p.iles != 0:
             raise SgsInputError(f"iles must be between 0 and 3: {p.iles}")
     except StopIteration:
@@ -66,6 +75,10 @@ def parse(text: str) -> SgsParams:
         raise SgsInputError(f"cutoff wavenumber must be between 0 and 1: {p.cutoff}")
     if not 0 <= p.iord <= 5:
         raise SgsInputError(f"iord must be between 0 and 5: {p.iord}")
+    if not 0 <= p.ineg <= 2:
+        raise SgsInputError(f"ineg must be between 0 and 2: {p.ineg}")
+    if p.cs != 0.0 and p.ineg != 1:
+        raise SgsInputError(f"ineg must be 1: {p.ineg}")
     return p
 
 
@@ -79,4 +92,6 @@ def format_sgs(p: SgsParams) -> str:
     out = ["# sgs.i written by simson_amd", f"{VERSION}", f"{p.iles}"]
     if p.iles == 1:
         out += [f"{p.cutoff_inv!r}", f"{p.iord}", f"{p.chi!r}"]
+    elif p.iles == 2:
+        out += [f"{p.cs!r}"] + ([f"{p.ineg}"] if p.cs == 0.0 else [])
     return "\n".join(out) + "\n"

@@ -6,6 +6,7 @@ Runs in the build container only (it needs /root/reference and amdflang):
     python tests/golden/make_golden.py pressure   # p_32x33x32.npz
     python tests/golden/make_golden.py les        # les_32x33x32.npz
     python tests/golden/make_golden.py xys        # xys_32x33x32.npz
+    python tests/golden/make_golden.py smag       # smag_32x33x32.npz
 
 1. builds the reference `bla` for 32x33x32 (serial, pressure off) with
    oracle/build_ref.sh into oracle/_ref/;
@@ -44,6 +45,7 @@ from simson_amd import blai, fields, sgsi, stats, ufile  # noqa: E402
 NX, NY, NZ = 32, 33, 32
 BIN = os.path.join(REPO, "oracle", "_ref", "bla_32x33x32_serial")
 PBIN = os.path.join(REPO, "oracle", "_ref", "bla_32x33x32_press")
+SBIN = os.path.join(REPO, "oracle", "_ref", "bla_32x33x32_smag")
 
 
 def c1_field():
@@ -212,6 +214,52 @@ def xys():
         shutil.rmtree(work)
 
 
+SMAG_CASES = {
+    # dynamic procedure (cs = 0) with each clipping of rparamles.f:36-39, and
+    # a fixed coefficient; 10 steps, saves after 1 step for d1
+    "d1": (sgsi.SgsParams(iles=2, cs=0.0, ineg=1), [0.01]),
+    "d0": (sgsi.SgsParams(iles=2, cs=0.0, ineg=0), []),
+    "d2": (sgsi.SgsParams(iles=2, cs=0.0, ineg=2), []),
+    "c": (sgsi.SgsParams(iles=2, cs=0.15, ineg=1), []),
+}
+
+
+def smag():
+    """Eddy-viscosity fixture (sgs.i iles = 2): the reference is compiled
+    with iiles_smag = 1 (oracle/build_ref.sh with SMAG=1), started from the
+    LES field, 10 steps at dt = 0.01.  Keys <case>_u10 (and d1_u1)."""
+    if not os.path.exists(SBIN):
+        env = dict(os.environ, SMAG="1")
+        subprocess.check_call([os.path.join(REPO, "oracle", "build_ref.sh"),
+                               str(NX), str(NY), str(NZ), "1", "no", "0",
+                               os.path.basename(SBIN)], env=env)
+    res = {}
+    for key, (sp, tsave) in SMAG_CASES.items():
+        work = tempfile.mkdtemp(prefix="simson_golden_smag.")
+        try:
+            ufile.write_u(os.path.join(work, "init.u"), les_field())
+            names = [f"t{round(t / 0.01)}.u" for t in tsave]
+            p = blai.BlaParams(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=30,
+                               dt=0.01, nst=3, xl=4 * math.pi, cflux=True, icfl=3,
+                               sgs=True, tsave=tsave, nmsave=names)
+            with open(os.path.join(work, "bla.i"), "w") as f:
+                f.write(blai.format_channel(p))
+            with open(os.path.join(work, "sgs.i"), "w") as f:
+                f.write(sgsi.format_sgs(sp))
+            with open(os.path.join(work, "out.txt"), "w") as out:
+                subprocess.check_call([SBIN], cwd=work, stdout=out, stderr=subprocess.STDOUT)
+            log = open(os.path.join(work, "out.txt")).read()
+            if "bla terminated normally" not in log:
+                raise RuntimeError(f"reference bla (smag {key}) did not terminate normally")
+            for name in names:
+                res[f"{key}_u{name[1:-2]}"] = ufile.read_u(os.path.join(work, name)).u
+            res[f"{key}_u10"] = ufile.read_u(os.path.join(work, "end.u")).u
+        finally:
+            shutil.rmtree(work)
+    np.savez_compressed(os.path.join(HERE, "smag_32x33x32.npz"), **res)
+    print("wrote", os.path.join(HERE, "smag_32x33x32.npz"), sorted(res))
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "pressure":
         pressure()
@@ -219,5 +267,7 @@ if __name__ == "__main__":
         les()
     elif len(sys.argv) > 1 and sys.argv[1] == "xys":
         xys()
+    elif len(sys.argv) > 1 and sys.argv[1] == "smag":
+        smag()
     else:
         main()
