@@ -168,6 +168,37 @@ typedef struct simson_les {
 
 int simson_gpu_les(simson_gpu_t *g, const simson_les_t *p);
 
+/* Fringe region of spatial simulations (bla.i spat, rparambl.f:547-720):
+ * the forcing -lambda(x) (u_i - U_i(x, y)) toward the base flow U that
+ * fring.f adds to the nonlinear term of every nonlinbl call
+ * (nonlinbl.f:678-693) and fringp.f to that of the pressure
+ * (nonlinp.f:172-174), with the fringe function lambda of ffun.f (no
+ * forced waves, OS modes or streaks).  Lengths in the channel scaling of
+ * bla.i; xsc is the frame position xs - cnp1 u0low of bla.f:1819 (0 for a
+ * fixed frame); fstart < fend within -xl/2 .. xl/2, fmax, frise, ffall > 0
+ * (rparambl.f:1898-1913). */
+typedef struct simson_fringe {
+    double fmax;         /* bla.i fmax */
+    double fstart;       /* bla.i fstart */
+    double fend;         /* bla.i fend */
+    double frise;        /* bla.i frise */
+    double ffall;        /* bla.i ffall */
+    double xsc;          /* frame position */
+} simson_fringe_t;
+
+/* Switch the fringe on with base, a C array [3][ny][3nx/2] of the base-flow
+ * velocities u, v, w at the physical points x_j = -xl/2 + j xl/nxp of the
+ * 3/2 x grid (bu1/bu2 of bflow.f interleaved: even j from bu1, odd from
+ * bu2), or off with f = NULL.  For the spatial Poiseuille flow of fltype 4
+ * the base flow is u = 1 - y^2 (cbflow.f:117-131); the pressure gradient is
+ * zero (init with cflux = 0, retau = 0; bla.f:1657-1662).  Needs nxp/2
+ * among the specialised x-row lengths.  Synchronises. */
+int simson_gpu_fringe(simson_gpu_t *g, const simson_fringe_t *f, const double *base);
+
+/* The fringe function lambda at the 3nx/2 physical points of the 3/2 x
+ * grid (ffun.f:37-75), without a device.  Returns 0 or SIMSON_E_ARG. */
+int simson_fringe_function(int nx, double xl, const simson_fringe_t *f, double *lam);
+
 /* The ADM-RT filter operators of one grid, without a device: lpf [nz][nx/2]
  * is the x-z transfer function lpfxz of bla.f:1503-1518 and L [ny][ny] the
  * wall-normal low-pass I - H_y^(iord+1), H_y the compact high-pass filter of

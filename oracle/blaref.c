@@ -349,6 +349,7 @@ struct orc {
     double *gur, *gui;                 /* [3][nz][ny][nxh] high-pass velocities */
     double les_chi;                    /* chi/dstar of the relaxation term */
     double *fr, *fi;                   /* [3][nz][ny][nxh] external force or NULL */
+    double *flam, *fbase;              /* fringe [nxp] and base flow [3][ny][nxp] or NULL */
 };
 
 struct les_s;
@@ -415,6 +416,8 @@ void orc_destroy(orc_t *o)
     les_free(o->les);
     free(o->fr);
     free(o->fi);
+    free(o->flam);
+    free(o->fbase);
     free(o->gur);
     free(o->gui);
     free(o);
@@ -511,6 +514,16 @@ static double nonlin_plane(orc_t *o, double *ur, double *ui, int y, int do_cfl)
         double h1 = v * oz - w * oy;
         double h2 = w * ox - u * oz;
         double h3 = u * oy - v * ox;
+        /* fringe damping toward the base flow (fring.f: no forced waves) */
+        if (o->flam) {
+            int xp = (int)(p % (size_t)nxp);
+            double lam = o->flam[xp];
+            const double *b = o->fbase + (size_t)y * nxp + xp;
+            size_t bs = (size_t)o->ny * nxp;
+            h1 -= lam * (u - b[0]);
+            h2 -= lam * (v - b[bs]);
+            h3 -= lam * (w - b[2 * bs]);
+        }
         R[p] = h1;
         R[np + p] = h2;
         R[2 * np + p] = h3;
@@ -1105,6 +1118,16 @@ static void nonlinp_plane(orc_t *o, const double *ur, const double *ui, int y, d
         R[np + p] = w * ox - u * oz;
         R[2 * np + p] = u * oy - v * ox;
         R[3 * np + p] = u * u + v * v + w * w;
+        /* the same fringe term (fringp.f) */
+        if (o->flam) {
+            int xp = (int)(p % (size_t)nxp);
+            double lam = o->flam[xp];
+            const double *b = o->fbase + (size_t)y * nxp + xp;
+            size_t bs = (size_t)o->ny * nxp;
+            R[p] -= lam * (u - b[0]);
+            R[np + p] -= lam * (v - b[bs]);
+            R[2 * np + p] -= lam * (w - b[2 * bs]);
+        }
     }
     cplx *S = malloc(sizeof(cplx) * 4 * (size_t)nz * nxh);   /* [f][z][x] */
     for (int f = 0; f < 4; f++) {
@@ -1494,6 +1517,23 @@ void orc_set_force(orc_t *o, const double *fr, const double *fi)
     }
     memcpy(o->fr, fr, n * sizeof(double));
     memcpy(o->fi, fi, n * sizeof(double));
+}
+
+/* Fringe forcing of spatial simulations (fring.f, fringp.f without forced
+ * waves): H_i -= lam(x) (u_i - U_i(x, y)) at the physical points of the 3/2
+ * x grid, lam [nxp], base [3][ny][nxp]; NULL switches it off. */
+void orc_set_fringe(orc_t *o, const double *lam, const double *base)
+{
+    free(o->flam);
+    free(o->fbase);
+    o->flam = o->fbase = NULL;
+    if (!lam || !base)
+        return;
+    size_t nb = (size_t)3 * o->ny * o->nxp;
+    o->flam = malloc((size_t)o->nxp * sizeof(double));
+    o->fbase = malloc(nb * sizeof(double));
+    memcpy(o->flam, lam, (size_t)o->nxp * sizeof(double));
+    memcpy(o->fbase, base, nb * sizeof(double));
 }
 
 int orc_set_les(orc_t *o, int iles, double cutoff, int iord, double chi)

@@ -71,6 +71,10 @@ int orc_set_les(orc_t *o, int iles, double cutoff, int iord, double chi);
  * nonlinbl.f:1011-1050 does for the eddy-viscosity models; NULL: off. */
 void orc_set_force(orc_t *o, const double *fr, const double *fi);
 
+/* Fringe forcing (fring.f, fringp.f): lam [nxp] at the physical points of
+ * the 3/2 x grid, base flow [3][ny][nxp]; NULL: off. */
+void orc_set_fringe(orc_t *o, const double *lam, const double *base);
+
 int orc_les_filter(int nx, int ny, int nz, double cutoff, int iord, double *lpf, double *L);
 /* complex DFT, y_k = sum_j x_j exp(sign 2 pi i jk/n), in place on (re,im) */
 void orc_cdft(int n, double *re, double *im, int sign);

@@ -52,6 +52,7 @@ def lib():
         L.orc_set_les.restype = C.c_int
         L.orc_set_les.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double]
         L.orc_set_force.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_set_fringe.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_les_filter.restype = C.c_int
         L.orc_les_filter.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _dp, _dp]
         _lib = L
@@ -110,6 +111,15 @@ class Oracle:
         """ADM-RT relaxation model (sgs.i iles = 1) on, or off with iles = 0."""
         if lib().orc_set_les(self.h, iles, cutoff, iord, chi) != 0:
             raise ValueError("unsupported LES parameters")
+
+    def set_fringe(self, lam=None, base=None):
+        """Fringe forcing with lam [nxp] and base flow [3][ny][nxp]; None: off."""
+        if lam is None:
+            lib().orc_set_fringe(self.h, None, None)
+            return
+        self._fringe = (np.ascontiguousarray(lam, np.float64),
+                        np.ascontiguousarray(base, np.float64))
+        lib().orc_set_fringe(self.h, self._fringe[0].ctypes.data, self._fringe[1].ctypes.data)
 
     def pressure(self, ur, ui):
         """Pressure [z][y][x] (complex) of the state in ur, ui (slots 0-4)."""

@@ -56,6 +56,12 @@ class BlaParams:
     suction: bool = False
     spat: bool = False
     cdev: float = 0.0
+    # spatial simulation: fringe region (rparambl.f:547-720, ffun.f)
+    fmax: float = 0.0
+    fstart: float = 0.0
+    fend: float = 0.0
+    frise: float = 0.0
+    ffall: float = 0.0
     sgs: bool = False
     isfd: int = 0
     imhd: int = 0
@@ -177,8 +183,25 @@ def parse(text: str) -> BlaParams:
         raise BlaInputError("wall suction is not supported")
     p.spat = _logical(tok())
     if p.spat:
-        raise BlaInputError("spatial simulations are not supported")
-    p.cdev = _num(tok())
+        # rparambl.f:547-720: only the fringe damping toward the computed
+        # base flow; tabulated free stream, base flow files, forced waves,
+        # OS modes, streaks and fringe waves are not supported
+        if _logical(tok()):
+            raise BlaInputError("tabulated free-stream velocity (tabfre) is not supported")
+        if _logical(tok()):
+            raise BlaInputError("base flow from a file (rbfl) is not supported")
+        p.fmax, p.fstart, p.fend, p.frise, p.ffall = (_num(tok()) for _ in range(5))
+        if _num(tok()) != 0.0 or _num(tok()) != 0.0:
+            raise BlaInputError("forced waves in the fringe (ampob, amp2d) are not supported")
+        if _logical(tok()) or _logical(tok()) or _logical(tok()):
+            raise BlaInputError("OS modes, streaks and waves in the fringe are not supported")
+        # rparambl.f:1898-1913
+        if (p.fstart >= p.fend or p.frise <= 0 or p.ffall <= 0 or p.fmax <= 0 or
+                abs(p.fstart) > .5 * p.xl or abs(p.fend) > .5 * p.xl):
+            raise BlaInputError("fmax, frise, ffall must be positive and fstart < fend "
+                                "within -xl/2 to xl/2")
+    elif not p.suction:
+        p.cdev = _num(tok())
     p.sgs = _logical(tok())          # LES parameters then come from sgs.i (simson_amd.sgsi)
     p.isfd = int(_num(tok()))
     if p.isfd != 0:
@@ -270,8 +293,24 @@ def format_channel(p: BlaParams) -> str:
         ".false.         Chebyshev integration method (cim)",
         f"{L(p.gall)}    Galilei transformation (gall)",
         ".false.         Constant wall-suction (suction)",
-        ".false.         Spatial simulation (spat)",
-        f"{p.cdev!r}      Temporal development vel. (cdev)",
+        f"{L(p.spat)}    Spatial simulation (spat)",
+    ]
+    if p.spat:
+        out += [".false.         Tabulated free-stream velocity (tabfre)",
+                ".false.         Read base flow from file (rbfl)",
+                f"{p.fmax!r}      Fringe strength (fmax)",
+                f"{p.fstart!r}    Fringe start (fstart)",
+                f"{p.fend!r}      Fringe end (fend)",
+                f"{p.frise!r}     Fringe rise distance (frise)",
+                f"{p.ffall!r}     Fringe fall distance (ffall)",
+                "0.0             Amplitude oblique waves (ampob)",
+                "0.0             Amplitude 2D TS waves (amp2d)",
+                ".false.         Forcing with Orr-Sommerfeld modes (osmod)",
+                ".false.         Forcing with streaks (streak)",
+                ".false.         Forcing with waves (waves)"]
+    else:
+        out += [f"{p.cdev!r}      Temporal development vel. (cdev)"]
+    out += [
         f"{L(p.sgs)}     Read sgs.i (sgs)",
         "0               SFD (isfd)",
         "0               MHD (imhd)",

@@ -95,6 +95,11 @@ struct simson_gpu {
     double2 *SQ = nullptr, *SS = nullptr, *SW = nullptr;
     double *dlt2 = nullptr;
     size_t smag_lds = 0;
+    // fringe of spatial simulations (simson_gpu_fringe): lambda [nxp/2] and
+    // the base flow [3][ny][nxp/2] packed as the x rows, FP64 and FP32
+    int fringe = 0;
+    double2 *flam = nullptr, *fbase = nullptr;
+    float2 *flam32 = nullptr, *fbase32 = nullptr;
     double mflux = 0., u0upp = 0., u0low = 0., w0upp = 0., w0low = 0.;
     bool timing = false;
     std::vector<TimedLaunch> launches;
@@ -187,7 +192,8 @@ void free_all(simson_gpu_t *g)
                     g->twz32, g->twx32, g->Tf32, g->Tb32, g->Q, g->Qc, g->PR, g->pscr,
                     g->Lm, g->lpf, g->Lm32, g->Wf, g->Wf32, g->clW, g->clY, g->PG, g->TAU,
                     g->DT, g->PG32, g->TAU32, g->DT32, g->Dm, g->Dm32, g->twzs,
-                    g->twxs, g->xysd, g->SQ, g->SS, g->SW, g->dlt2};
+                    g->twxs, g->xysd, g->SQ, g->SS, g->SW, g->dlt2, g->flam, g->fbase,
+                    g->flam32, g->fbase32};
     for (void *q : ptrs)
         if (q)
             (void)hipFree(q);
@@ -244,6 +250,9 @@ NonlinArgs nonlin_args(simson_gpu_t *g, int do_cfl)
     a.clos = 0;
     a.TAU = nullptr;
     a.DT = nullptr;
+    a.fringe = g->fringe;
+    a.flam = g->flam;
+    a.fbase = g->fbase;
     return a;
 }
 
@@ -345,6 +354,9 @@ Step2Args<float2> nonlin_args32(simson_gpu_t *g, int do_cfl)
     a.clos = 0;
     a.TAU = nullptr;
     a.DT = nullptr;
+    a.fringe = d.fringe;
+    a.flam = g->flam32;
+    a.fbase = g->fbase32;
     return a;
 }
 
@@ -1033,6 +1045,64 @@ int simson_gpu_xys(simson_gpu_t *g, double dtn, double xs, double zs, int nxys, 
     CK(hipStreamSynchronize(g->stream));
     for (size_t i = 0; i < h.size(); i++)
         xys[i] += h[i];
+    return 0;
+}
+
+int simson_fringe_function(int nx, double xl, const simson_fringe_t *f, double *lam)
+{
+    if (!f || !lam)
+        return SIMSON_E_ARG;
+    return fringe_function(nx, xl, f, lam);
+}
+
+int simson_gpu_fringe(simson_gpu_t *g, const simson_fringe_t *f, const double *base)
+{
+    if (!g)
+        return SIMSON_E_ARG;
+    if (!f) {
+        g->fringe = 0;
+        return 0;
+    }
+    if (!base)
+        return SIMSON_E_ARG;
+    if (!has_xrow4(g->nxp / 2)) {
+        g->err = "fringe: nxp/2 needs a specialised x-row length";
+        return SIMSON_E_ARG;
+    }
+    const int nxp = g->nxp, M = nxp / 2, ny = g->ny;
+    std::vector<double> lam(nxp);
+    if (fringe_function(g->nx, g->p.xl, f, lam.data()) != 0) {
+        g->err = "fringe: fmax, frise, ffall > 0 and fstart < fend within -xl/2 to xl/2";
+        return SIMSON_E_ARG;
+    }
+    std::vector<double2> hl(M), hb((size_t)3 * ny * M);
+    for (int n = 0; n < M; n++)
+        hl[n] = make_double2(lam[2 * n], lam[2 * n + 1]);
+    for (int c = 0; c < 3; c++)
+        for (int y = 0; y < ny; y++)
+            for (int n = 0; n < M; n++) {
+                const double *b = base + ((size_t)c * ny + y) * nxp;
+                hb[((size_t)c * ny + y) * M + n] = make_double2(b[2 * n], b[2 * n + 1]);
+            }
+    if (!g->flam && (dalloc(g, &g->flam, hl.size()) || dalloc(g, &g->fbase, hb.size())))
+        return SIMSON_E_HIP;
+    if (h2d(g, g->flam, hl.data(), hl.size()) || h2d(g, g->fbase, hb.data(), hb.size()))
+        return SIMSON_E_HIP;
+    if (g->mixed) {
+        std::vector<float2> l32(M), b32(hb.size());
+        for (int n = 0; n < M; n++)
+            l32[n] = make_float2((float)hl[n].x, (float)hl[n].y);
+        for (size_t i = 0; i < hb.size(); i++)
+            b32[i] = make_float2((float)hb[i].x, (float)hb[i].y);
+        if (!g->flam32 &&
+            (dalloc(g, &g->flam32, l32.size()) || dalloc(g, &g->fbase32, b32.size())))
+            return SIMSON_E_HIP;
+        if (h2d(g, g->flam32, l32.data(), l32.size()) ||
+            h2d(g, g->fbase32, b32.data(), b32.size()))
+            return SIMSON_E_HIP;
+    }
+    CK(hipStreamSynchronize(g->stream));
+    g->fringe = 1;
     return 0;
 }
 

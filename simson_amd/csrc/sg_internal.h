@@ -101,6 +101,12 @@ struct Step2Args {
     // and DT = d/dy of tau_12, tau_22, tau_23 [3][nz][ny][nxh]
     int clos;
     const V *TAU, *DT;
+    // fringe forcing of spatial simulations (fring.f, fringp.f): H_i -=
+    // lambda(x) (u_i - U_i(x, y)) at the physical points of the 3/2 x grid,
+    // packed as the x rows (even points in .x, odd in .y): flam [nxp/2],
+    // fbase [3][ny][nxp/2]; applied by the four-step x-row kernels
+    int fringe;
+    const V *flam, *fbase;
 };
 typedef Step2Args<double2> NonlinArgs;
 
@@ -121,6 +127,10 @@ bool launch_xclos(const Step2Args<V> &a, const V *wx, const ClosArgs &c, hipStre
 // synthesised in z, into a.P as [9][ny][nzp][nx/2]
 template <typename V>
 bool launch_gsyn4(const Step2Args<V> &a, const V *tw, hipStream_t s);
+
+// fringe function of ffun.f at the nxp physical points of the 3/2 x grid
+// (host); 0 or SIMSON_E_ARG
+int fringe_function(int nx, double xl, const simson_fringe_t *f, double *lam);
 
 // ADM-RT filter operators (host): lpf [nz][nx/2] and L = I - H_y^(iord+1)
 // [ny][ny]; 0 or SIMSON_E_ARG

@@ -251,6 +251,24 @@ __global__ void __launch_bounds__((XBlock<V, M>::NT)) k_xrow4(Step2Args<V> a, co
             h2[i] = vmk<V>(h2[i].x - (Um * o[i].x + u[i].x * zx),
                            h2[i].y - (Um * o[i].y + u[i].y * zy));
         }
+        // fringe damping toward the base flow (fring.f, fringp.f): point
+        // n = k2 + M2 i of the row (the order the two inverse passes leave)
+        if (a.fringe) {
+#pragma unroll
+            for (int i = 0; i < M1; i++) {
+                const int n = k2 + M2 * i;
+                const V lam = a.flam[n];
+                const V b0 = a.fbase[(size_t)y * M + n];
+                const V b1 = a.fbase[((size_t)ny + y) * M + n];
+                const V b2 = a.fbase[((size_t)2 * ny + y) * M + n];
+                h1[i] = vmk<V>(h1[i].x - lam.x * (Um + u[i].x - b0.x),
+                               h1[i].y - lam.y * (Um + u[i].y - b0.y));
+                h2[i] = vmk<V>(h2[i].x - lam.x * (v[i].x - b1.x),
+                               h2[i].y - lam.y * (v[i].y - b1.y));
+                h3[i] = vmk<V>(h3[i].x - lam.x * (Wm + w[i].x - b2.x),
+                               h3[i].y - lam.y * (Wm + w[i].y - b2.y));
+            }
+        }
         // forward pass A
         reg_dft<V, M1, -1, TABN>(h1, tw);
         reg_dft<V, M1, -1, TABN>(h2, tw);

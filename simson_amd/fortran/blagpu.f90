@@ -30,7 +30,11 @@ program blagpu
   real(dp) :: tmax, cpumax, wallmax, dt, cflmaxin, xl, rot, retau, cdev
   logical :: write_inter, varsiz, cflux, pert, cim, gall, suction, spat
   logical :: sgs, tripf, longli, fileurms, pou, stopf, docfl, corrf, corrf_x, serf
-  logical :: chkstop
+  logical :: chkstop, tabfre, rbfl, osmod, streak, waves
+  ! spatial simulations: fringe (ffun.f) and base flow [3][nyp][nxp]
+  type(simson_fringe_t) :: fr
+  real(dp) :: ampob, amp2d, bstart, blength
+  real(dp), allocatable :: base(:,:,:)
   ! field and header
   integer :: nx, nyp, nzc, nfzsym, fltype
   real(dp) :: re, xlf, zl, t, xs, dstar
@@ -79,8 +83,36 @@ program blagpu
   call nxt(line); read(line, *) cim
   call nxt(line); read(line, *) gall
   call nxt(line); read(line, *) suction
+  if (suction) then
+     ! the suction parameters follow in bla.i (rparambl.f); not supported
+     write(*,*) 'blagpu: bla.i requests a feature outside the channel step library'
+     call stopnow(13)
+  end if
   call nxt(line); read(line, *) spat
-  call nxt(line); read(line, *) cdev
+  cdev = 0._dp
+  if (spat) then
+     ! rparambl.f:547-720: the fringe toward the computed base flow only
+     call nxt(line); read(line, *) tabfre
+     call nxt(line); read(line, *) rbfl
+     call nxt(line); read(line, *) fr%fmax
+     call nxt(line); read(line, *) fr%fstart
+     call nxt(line); read(line, *) fr%fend
+     call nxt(line); read(line, *) fr%frise
+     call nxt(line); read(line, *) fr%ffall
+     call nxt(line); read(line, *) ampob
+     call nxt(line); read(line, *) amp2d
+     call nxt(line); read(line, *) osmod
+     call nxt(line); read(line, *) streak
+     call nxt(line); read(line, *) waves
+     fr%xsc = 0._dp
+     if (tabfre .or. rbfl .or. ampob /= 0._dp .or. amp2d /= 0._dp .or. osmod .or. streak &
+          .or. waves) then
+        write(*,*) 'blagpu: only the fringe damping toward the computed base flow is supported'
+        call stopnow(13)
+     end if
+  else if (.not. suction) then
+     call nxt(line); read(line, *) cdev
+  end if
   call nxt(line); read(line, *) sgs
   call nxt(line); read(line, *) isfd
   call nxt(line); read(line, *) imhd
@@ -126,7 +158,7 @@ program blagpu
   call nxt(line); read(line, *) mwave
   call nxt(line); read(line, *) npl
   close(10)
-  if (pert .or. cim .or. suction .or. spat .or. isfd /= 0 .or. imhd /= 0 .or. &
+  if (pert .or. cim .or. suction .or. isfd /= 0 .or. imhd /= 0 .or. &
        loctyp /= 0 .or. tripf .or. wbci /= 0 .or. ibc /= 0 .or. nst /= 3 &
        .or. mwave > 0 .or. npl > 0 &
        .or. dt <= 0._dp) then
@@ -173,7 +205,11 @@ program blagpu
   read(12) re, pou, xlf, zl, t, xs
   read(12) nx, nyp, nzc, nfzsym
   read(12) fltype, dstar
-  if (fltype /= 1 .or. nfzsym /= 0) call stopnow(13)
+  ! temporal (fltype 1) or spatial (fltype 4) Poiseuille flow
+  if (nfzsym /= 0 .or. .not. (fltype == 1 .or. (spat .and. fltype == 4))) call stopnow(13)
+  bstart = 0._dp
+  blength = 0._dp
+  if (fltype == 4 .or. fltype == 5) read(12) bstart, blength
   allocate(ur(nx/2, nyp, nzc, 7), ui(nx/2, nyp, nzc, 7), urx(nx))
   ur = 0._dp
   ui = 0._dp
@@ -200,6 +236,11 @@ program blagpu
   ! ---------------- device state ----------------
   p%nx = nx; p%ny = nyp; p%nz = nzc
   p%precision = SIMSON_FP64; p%ibc = 0
+  ! no pressure gradient unless temporal Poiseuille flow (bla.f:1622-1662)
+  if (fltype /= 1) then
+     cflux = .false.
+     retau = 0._dp
+  end if
   p%cflux = merge(1, 0, cflux)
   p%re = re; p%xl = xl; p%zl = zl; p%rot = rot; p%retau = retau
   call chk(simson_gpu_init(p, g))
@@ -210,7 +251,16 @@ program blagpu
   if (cflux) then
      px = -2._dp / re                             ! bla.f:1629-1630
   else
-     px = -(retau / re)**2                        ! bla.f:1645
+     px = -(retau / re)**2                        ! bla.f:1645 (0 for fltype 4)
+  end if
+  if (spat) then
+     ! the Poiseuille base flow of fltype 4 at the 3/2 x grid (cbflow.f:117-131)
+     allocate(base(3 * nx / 2, nyp, 3))
+     base = 0._dp
+     do it = 1, nyp
+        base(:, it, 1) = 1._dp - cos(pi * real(it - 1, dp) / real(nyp - 1, dp))**2
+     end do
+     call chk(simson_gpu_fringe(g, fr, base))
   end if
   call chk(simson_gpu_upload(g, ur, ui, 3_c_int, c_null_ptr, px))
   ! Clenshaw-Curtis weights (preprbl.f:78-87) for the amplitude file
@@ -422,6 +472,7 @@ contains
     write(11) re, .false., xl, zl, t, 0._dp
     write(11) nx, nyp, nzc, nfzsym
     write(11) fltype, dstar
+    if (fltype == 4 .or. fltype == 5) write(11) bstart, blength
     do c = 1, 3
        do z = 1, nzc
           do y = 1, nyp

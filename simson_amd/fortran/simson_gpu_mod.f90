@@ -19,6 +19,15 @@ module simson_gpu_mod
   end type simson_params_t
 
   ! ADM-RT LES parameters (sgs.i, rparamles.f)
+  type, bind(c) :: simson_fringe_t
+     real(c_double) :: fmax
+     real(c_double) :: fstart
+     real(c_double) :: fend
+     real(c_double) :: frise
+     real(c_double) :: ffall
+     real(c_double) :: xsc
+  end type simson_fringe_t
+
   type, bind(c) :: simson_les_t
      integer(c_int) :: iles
      integer(c_int) :: iord
@@ -91,6 +100,13 @@ module simson_gpu_mod
        integer(c_int), value :: nxys
        real(c_double), intent(inout) :: xys(*)
      end function simson_gpu_xys
+
+     integer(c_int) function simson_gpu_fringe(g, f, base) bind(c, name='simson_gpu_fringe')
+       import :: c_int, c_ptr, c_double, simson_fringe_t
+       type(c_ptr), value :: g
+       type(simson_fringe_t), intent(in) :: f
+       real(c_double), intent(in) :: base(*)
+     end function simson_gpu_fringe
 
      integer(c_int) function simson_gpu_les(g, les) bind(c, name='simson_gpu_les')
        import :: c_int, c_ptr, simson_les_t

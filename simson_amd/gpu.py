@@ -29,6 +29,7 @@ EXPORTS = (
     "simson_gpu_run", "simson_gpu_download", "simson_gpu_amp", "simson_gpu_xys",
     "simson_gpu_pressure",
     "simson_gpu_les", "simson_les_filter", "simson_gpu_closure", "simson_gpu_sync",
+    "simson_gpu_fringe", "simson_fringe_function",
     "simson_mlp_create", "simson_mlp_nparams", "simson_mlp_params", "simson_mlp_set_data",
     "simson_mlp_train", "simson_mlp_grad", "simson_mlp_infer", "simson_mlp_last_error",
     "simson_mlp_destroy",
@@ -49,6 +50,11 @@ class SimsonParams(C.Structure):
 class SimsonLes(C.Structure):
     _fields_ = [("iles", C.c_int), ("iord", C.c_int), ("cutoff", C.c_double),
                 ("chi", C.c_double), ("cs", C.c_double), ("ineg", C.c_int)]
+
+
+class SimsonFringe(C.Structure):
+    _fields_ = [("fmax", C.c_double), ("fstart", C.c_double), ("fend", C.c_double),
+                ("frise", C.c_double), ("ffall", C.c_double), ("xsc", C.c_double)]
 
 
 class SimsonError(RuntimeError):
@@ -90,6 +96,8 @@ def lib():
         L.simson_gpu_closure.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p]
         L.simson_gpu_sync.argtypes = [vp]
+        L.simson_gpu_fringe.argtypes = [vp, C.POINTER(SimsonFringe), C.c_void_p]
+        L.simson_fringe_function.argtypes = [C.c_int, C.c_double, C.POINTER(SimsonFringe), _dp]
         L.simson_gpu_timing_enable.argtypes = [vp, C.c_int]
         L.simson_gpu_timing_read.argtypes = [vp, C.POINTER(C.c_char_p),
                                              C.POINTER(C.c_double), C.POINTER(C.c_long),
@@ -220,6 +228,17 @@ class BlaGPU:
         p = SimsonLes(iles, iord, cutoff, chi, cs, ineg)
         self._check(lib().simson_gpu_les(self.h, C.byref(p)))
 
+    def fringe(self, fmax=None, fstart=0.0, fend=0.0, frise=0.0, ffall=0.0, base=None,
+               xsc=0.0):
+        """Fringe forcing toward the base flow base [3][ny][3nx/2] (physical
+        points of the 3/2 x grid); fmax None switches it off."""
+        if fmax is None:
+            self._check(lib().simson_gpu_fringe(self.h, None, None))
+            return
+        f = SimsonFringe(fmax, fstart, fend, frise, ffall, xsc)
+        b = np.ascontiguousarray(base, np.float64)
+        self._check(lib().simson_gpu_fringe(self.h, C.byref(f), b.ctypes.data_as(C.c_void_p)))
+
     def closure(self, params, in_scale=None, in_shift=None, yfeat=None, out_scale=None,
                 out_shift=None):
         """Learned closure in the step (params None turns it off); see
@@ -281,6 +300,18 @@ def les_filter(nx: int, ny: int, nz: int, cutoff: float, iord: int):
     if st != 0:
         raise SimsonError(st, "invalid ADM-RT filter parameters")
     return lpf, L
+
+
+def fringe_function(nx: int, xl: float, fmax: float, fstart: float, fend: float,
+                    frise: float, ffall: float, xsc: float = 0.0) -> np.ndarray:
+    """The library's fringe function lambda at the 3nx/2 physical points of
+    the 3/2 x grid; host computation, no device needed."""
+    lam = np.zeros(3 * nx // 2)
+    f = SimsonFringe(fmax, fstart, fend, frise, ffall, xsc)
+    st = lib().simson_fringe_function(nx, xl, C.byref(f), lam)
+    if st != 0:
+        raise SimsonError(st, "invalid fringe parameters")
+    return lam
 
 
 def rk3_weights(dt: float):

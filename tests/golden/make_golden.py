@@ -7,6 +7,7 @@ Runs in the build container only (it needs /root/reference and amdflang):
     python tests/golden/make_golden.py les        # les_32x33x32.npz
     python tests/golden/make_golden.py xys        # xys_32x33x32.npz
     python tests/golden/make_golden.py smag       # smag_32x33x32.npz
+    python tests/golden/make_golden.py fringe     # fringe_32x33x32.npz
 
 1. builds the reference `bla` for 32x33x32 (serial, pressure off) with
    oracle/build_ref.sh into oracle/_ref/;
@@ -260,6 +261,53 @@ def smag():
     print("wrote", os.path.join(HERE, "smag_32x33x32.npz"), sorted(res))
 
 
+FRINGE_CASES = {
+    # (fmax, fstart, fend, frise, ffall) in the channel scaling of bla.i
+    "a": (1.0, -4.0, 0.0, 2.0, 1.0),
+    "b": (2.5, -6.0, 3.0, 3.0, 2.0),
+}
+
+
+def fringe():
+    """Fringe fixture: the spatial Poiseuille channel (fltype 4, spat, ibc 0,
+    no pressure gradient: bla.f:1657-1662) from the LES start field, the
+    fringe damping toward U = 1 - y^2 (cbflow.f:117-131), 10 steps at dt =
+    0.01, saves after 1 and 10 steps with their pressure (the pressure = 1
+    binary: fringp.f adds the same forcing to the pressure's nonlinear
+    term).  Keys <case>_u1, <case>_u10, <case>_p10."""
+    import dataclasses
+    if not os.path.exists(PBIN):
+        subprocess.check_call([os.path.join(REPO, "oracle", "build_ref.sh"),
+                               str(NX), str(NY), str(NZ), "1", "no", "1",
+                               os.path.basename(PBIN)])
+    res = {}
+    for key, (fmax, fstart, fend, frise, ffall) in FRINGE_CASES.items():
+        work = tempfile.mkdtemp(prefix="simson_golden_fringe.")
+        try:
+            f = dataclasses.replace(les_field(), fltype=4,
+                                    extra={"bstart": 0.0, "blength": 0.0})
+            ufile.write_u(os.path.join(work, "init.u"), f)
+            p = blai.BlaParams(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=30,
+                               dt=0.01, nst=3, xl=4 * math.pi, cflux=False, retau=0.0,
+                               icfl=3, spat=True, fmax=fmax, fstart=fstart, fend=fend,
+                               frise=frise, ffall=ffall, tsave=[0.01, 0.1],
+                               nmsave=["t1.u", "t10.u"])
+            with open(os.path.join(work, "bla.i"), "w") as fh:
+                fh.write(blai.format_channel(p))
+            with open(os.path.join(work, "out.txt"), "w") as out:
+                subprocess.check_call([PBIN], cwd=work, stdout=out, stderr=subprocess.STDOUT)
+            log = open(os.path.join(work, "out.txt")).read()
+            if "bla terminated normally" not in log:
+                raise RuntimeError(f"reference bla (fringe {key}) did not terminate normally")
+            res[f"{key}_u1"] = ufile.read_u(os.path.join(work, "t1.u")).u
+            res[f"{key}_u10"] = ufile.read_u(os.path.join(work, "t10.u")).u
+            res[f"{key}_p10"] = ufile.read_u(os.path.join(work, "t10.u.p"), ncomp=1).u[0]
+        finally:
+            shutil.rmtree(work)
+    np.savez_compressed(os.path.join(HERE, "fringe_32x33x32.npz"), **res)
+    print("wrote", os.path.join(HERE, "fringe_32x33x32.npz"), sorted(res))
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "pressure":
         pressure()
@@ -269,5 +317,7 @@ if __name__ == "__main__":
         xys()
     elif len(sys.argv) > 1 and sys.argv[1] == "smag":
         smag()
+    elif len(sys.argv) > 1 and sys.argv[1] == "fringe":
+        fringe()
     else:
         main()

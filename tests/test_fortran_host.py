@@ -260,3 +260,39 @@ def test_fortran_host_smag_matches_reference_bla():
                 assert err <= 1e-12, (name, c, err)
     finally:
         shutil.rmtree(work)
+
+
+@pytest.mark.gpu
+def test_fortran_host_fringe_matches_reference_bla():
+    """The host on a spatial Poiseuille channel (fltype 4, bla.i spat with
+    the fringe of case a of tests/golden/fringe_32x33x32.npz) writes the
+    fields the reference bla wrote, fltype 4 header included."""
+    import dataclasses
+    _need_binary()
+    print("[gpu] blagpu spatial channel with fringe 32x33x32", flush=True)
+    golden = np.load(os.path.join(ROOT, "tests", "golden", "fringe_32x33x32.npz"))
+    work = tempfile.mkdtemp(prefix="blagpu_fringe.")
+    try:
+        f = fields.channel_field(32, 33, 32, 4 * math.pi, 2 * math.pi, 3000.0,
+                                 kmax=15, rms=1e-2, seed=5)
+        f = dataclasses.replace(f, fltype=4, extra={"bstart": 0.0, "blength": 0.0})
+        ufile.write_u(os.path.join(work, "init.u"), f)
+        p = blai.BlaParams(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=30, dt=0.01,
+                           nst=3, xl=4 * math.pi, cflux=False, retau=0.0, icfl=3, spat=True,
+                           fmax=1.0, fstart=-4.0, fend=0.0, frise=2.0, ffall=1.0,
+                           tsave=[0.01], nmsave=["t1.u"])
+        with open(os.path.join(work, "bla.i"), "w") as fh:
+            fh.write(blai.format_channel(p))
+        r = _run(work)
+        assert r.returncode == 0, r.stdout + r.stderr
+        for name, key in (("t1.u", "a_u1"), ("end.u", "a_u10")):
+            uf = ufile.read_u(os.path.join(work, name))
+            assert uf.fltype == 4
+            u = np.delete(uf.u, 16, axis=1)
+            rr = np.delete(golden[key], 16, axis=1)
+            for c in range(3):
+                err = _rel(u[c], rr[c])
+                print(f"    {name} comp {c}: rel L2 {err:.2e}", flush=True)
+                assert err <= 1e-12, (name, c, err)
+    finally:
+        shutil.rmtree(work)
