@@ -195,6 +195,44 @@ typedef struct simson_fringe {
  * among the specialised x-row lengths.  Synchronises. */
 int simson_gpu_fringe(simson_gpu_t *g, const simson_fringe_t *f, const double *base);
 
+/* Blowing and suction at the lower wall, bla.i wbci = 1 (rparambl.f:
+ * 1206-1232): the vertical velocity v(x, z, y = -1) = max(s P, s P damp),
+ * P = amp f(x) cos(zbet z), s = sin(tomeg tc) (1 when tomeg = 0), f the
+ * smooth strip of cwallbc.f:53-115 between xstart and xend, imposed on the
+ * v solve of every linear step (linearbl.f:458-469) at the time tc of the
+ * start of the substep (bla.f:1797, 2405), with the mean v set to the
+ * mean wall velocity (linearbl.f:1663-1670).  Coordinates in the channel
+ * scaling, x and z measured from the box centre (xsc, zsc the frame
+ * position, 0 for a fixed frame); the CFL estimate then keeps the full 1/dy
+ * near the walls (boxcfl.f:38-43). */
+typedef struct simson_wall {
+    double amp;          /* wp1 */
+    double damp;         /* wp2: factor on the suction part */
+    double xstart;       /* wpds1 */
+    double xend;         /* wpds2 */
+    double xrise;        /* wpds3 */
+    double xfall;        /* wpds4 */
+    double zbet;         /* wpdds1: spanwise wavenumber */
+    double tomeg;        /* wpdds2: angular frequency */
+    double xsc, zsc;     /* frame position */
+} simson_wall_t;
+
+/* Switch the wall forcing on (w) or off (NULL).  Synchronises. */
+int simson_gpu_wall(simson_gpu_t *g, const simson_wall_t *w);
+
+/* The spectral wall velocity the library imposes at time tc, vr, vi
+ * [nz][nx/2] (cwallbc.f:53-115, 380-407), without a device.  Returns 0 or
+ * SIMSON_E_ARG. */
+int simson_wall_velocity(int nx, int nz, double xl, double zl, const simson_wall_t *w,
+                         double tc, double *vr, double *vi);
+
+/* The time t of the start of the next substep, the tc of bla.f:1797 that
+ * time-dependent boundary forcing uses (0 after init).  simson_gpu_run
+ * advances it by c_k dt per substep and leaves it at the start of the next
+ * step; a host that calls simson_gpu_linear itself sets it before each
+ * call. */
+int simson_gpu_set_time(simson_gpu_t *g, double t);
+
 /* The fringe function lambda at the 3nx/2 physical points of the 3/2 x
  * grid (ffun.f:37-75), without a device.  Returns 0 or SIMSON_E_ARG. */
 int simson_fringe_function(int nx, double xl, const simson_fringe_t *f, double *lam);

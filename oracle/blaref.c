@@ -350,6 +350,7 @@ struct orc {
     double les_chi;                    /* chi/dstar of the relaxation term */
     double *fr, *fi;                   /* [3][nz][ny][nxh] external force or NULL */
     double *flam, *fbase;              /* fringe [nxp] and base flow [3][ny][nxp] or NULL */
+    double *wvr, *wvi;                 /* lower-wall v [nz][nxh] (spectral) or NULL */
 };
 
 struct les_s;
@@ -418,6 +419,8 @@ void orc_destroy(orc_t *o)
     free(o->fi);
     free(o->flam);
     free(o->fbase);
+    free(o->wvr);
+    free(o->wvi);
     free(o->gur);
     free(o->gui);
     free(o);
@@ -708,9 +711,17 @@ static void linear_plane(orc_t *o, double *ur, double *ui, double *puw, int z,
         tau_solve(n, lam2, zero, 1., 1., phh, W->gam, W->rho);
         tau_solve(n, lam2, fwr, 0., 0., omr, W->gam, W->rho);
         tau_solve(n, lam2, fwi, 0., 0., omi, W->gam, W->rho);
-        /* v from phi with k^2 (linearbl.f:1000-1001) */
-        tau_solve(n, k2, phr, 0., 0., vr, W->gam, W->rho);
-        tau_solve(n, k2, phi, 0., 0., vi, W->gam, W->rho);
+        /* v from phi with k^2 (linearbl.f:1000-1001); an imposed lower-wall
+         * v_w (wbci = 1) enters as the Dirichlet values v(1) = 0,
+         * v(-1) = v_w: (v_w/2, -v_w/2) for the even and odd classes
+         * (linearbl.f:458-469, 944-945) */
+        double bwr = 0., bwi = 0.;
+        if (o->wvr && !k0) {
+            bwr = .5 * o->wvr[(size_t)z * nxh + x];
+            bwi = .5 * o->wvi[(size_t)z * nxh + x];
+        }
+        tau_solve(n, k2, phr, bwr, -bwr, vr, W->gam, W->rho);
+        tau_solve(n, k2, phi, bwi, -bwi, vi, W->gam, W->rho);
         tau_solve(n, k2, phh, 0., 0., vh, W->gam, W->rho);
         /* superpose the homogeneous solution per parity so that dv/dy = 0
          * at both walls (linearbl.f:1023-1070) */
@@ -786,6 +797,10 @@ static void linear_plane(orc_t *o, double *ur, double *ui, double *puw, int z,
                 vi[y] = 0.;
                 ui_[y] = wi_[y] = oxi[y] = ozi[y] = 0.;
             }
+            /* the mean v is the mean wall velocity, constant in y
+             * (linearbl.f:1663-1670) */
+            if (o->wvr)
+                vr[0] = o->wvr[0];
             /* next partial RHS for the mean flow (linearbl.f:1734-1746) */
             dcheb(n, du0, tmp);
             for (int y = 0; y < n; y++)
@@ -1517,6 +1532,26 @@ void orc_set_force(orc_t *o, const double *fr, const double *fi)
     }
     memcpy(o->fr, fr, n * sizeof(double));
     memcpy(o->fi, fi, n * sizeof(double));
+}
+
+/* Lower-wall vertical velocity of wbci = 1 for the next orc_linear calls
+ * (the spectral v_w of cwallbc.f, [nz][nxh] real and imaginary parts);
+ * NULL switches it off. */
+void orc_set_wall(orc_t *o, const double *vr, const double *vi)
+{
+    size_t n = (size_t)o->nz * o->nxh;
+    if (!vr || !vi) {
+        free(o->wvr);
+        free(o->wvi);
+        o->wvr = o->wvi = NULL;
+        return;
+    }
+    if (!o->wvr) {
+        o->wvr = malloc(n * sizeof(double));
+        o->wvi = malloc(n * sizeof(double));
+    }
+    memcpy(o->wvr, vr, n * sizeof(double));
+    memcpy(o->wvi, vi, n * sizeof(double));
 }
 
 /* Fringe forcing of spatial simulations (fring.f, fringp.f without forced

@@ -75,6 +75,10 @@ void orc_set_force(orc_t *o, const double *fr, const double *fi);
  * the 3/2 x grid, base flow [3][ny][nxp]; NULL: off. */
 void orc_set_fringe(orc_t *o, const double *lam, const double *base);
 
+/* Lower-wall v_w [nz][nxh] (spectral, wbci = 1) for the v solves of later
+ * orc_linear calls; NULL: off. */
+void orc_set_wall(orc_t *o, const double *vr, const double *vi);
+
 int orc_les_filter(int nx, int ny, int nz, double cutoff, int iord, double *lpf, double *L);
 /* complex DFT, y_k = sum_j x_j exp(sign 2 pi i jk/n), in place on (re,im) */
 void orc_cdft(int n, double *re, double *im, int sign);

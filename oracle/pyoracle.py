@@ -53,6 +53,7 @@ def lib():
         L.orc_set_les.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double]
         L.orc_set_force.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_set_fringe.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_set_wall.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_les_filter.restype = C.c_int
         L.orc_les_filter.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _dp, _dp]
         _lib = L
@@ -120,6 +121,15 @@ class Oracle:
         self._fringe = (np.ascontiguousarray(lam, np.float64),
                         np.ascontiguousarray(base, np.float64))
         lib().orc_set_fringe(self.h, self._fringe[0].ctypes.data, self._fringe[1].ctypes.data)
+
+    def set_wall(self, vw=None):
+        """Lower-wall v_w [nz][nx/2] (complex spectral) for later linear
+        steps; None: off."""
+        if vw is None:
+            lib().orc_set_wall(self.h, None, None)
+            return
+        self._wall = (np.ascontiguousarray(vw.real), np.ascontiguousarray(vw.imag))
+        lib().orc_set_wall(self.h, self._wall[0].ctypes.data, self._wall[1].ctypes.data)
 
     def pressure(self, ur, ui):
         """Pressure [z][y][x] (complex) of the state in ur, ui (slots 0-4)."""

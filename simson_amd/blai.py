@@ -68,6 +68,17 @@ class BlaParams:
     loctyp: int = 0
     tripf: bool = False
     wbci: int = 0
+    # wbci = 1: blowing and suction strip v(x, z, t) = amp f(x) cos(zbet z)
+    # sin(tomeg t) at the lower wall, negative part scaled by damp
+    # (rparambl.f:1206-1232, cwallbc.f:53-115)
+    wamp: float = 0.0
+    wdamp: float = 1.0
+    wxstart: float = 0.0
+    wxend: float = 0.0
+    wxrise: float = 1.0
+    wxfall: float = 1.0
+    wzbet: float = 0.0
+    wtomeg: float = 0.0
     icfl: int = 0
     iamp: int = 0
     namamp: str = "amp.stat"
@@ -216,8 +227,11 @@ def parse(text: str) -> BlaParams:
     if p.tripf:
         raise BlaInputError("trip forcing is not supported")
     p.wbci = int(_num(tok()))
-    if p.wbci != 0:
-        raise BlaInputError("wall boundary forcing (wbci) is not supported")
+    if p.wbci == 1:
+        (p.wamp, p.wdamp, p.wxstart, p.wxend, p.wxrise, p.wxfall, p.wzbet,
+         p.wtomeg) = (_num(tok()) for _ in range(8))
+    elif p.wbci != 0:
+        raise BlaInputError("only the blowing/suction strip wbci = 1 is supported")
     p.icfl = int(_num(tok()))
     p.iamp = int(_num(tok()))
     if p.iamp > 0:
@@ -316,7 +330,14 @@ def format_channel(p: BlaParams) -> str:
         "0               MHD (imhd)",
         "0               Type of localized disturbance (loctyp)",
         ".false.         Trip forcing (tripf)",
-        "0               Boundary condition at lower wall (wbci)",
+        f"{p.wbci}       Boundary condition at lower wall (wbci)",
+    ]
+    if p.wbci == 1:
+        out += [f"{p.wamp!r}  amplitude (wp1)", f"{p.wdamp!r}  damping of suction (wp2)",
+                f"{p.wxstart!r}  xstart", f"{p.wxend!r}  xend", f"{p.wxrise!r}  xrise",
+                f"{p.wxfall!r}  xfall", f"{p.wzbet!r}  spanwise wavenumber (zbet)",
+                f"{p.wtomeg!r}  angular frequency (tomeg)"]
+    out += [
         f"{p.icfl}        CFL calculation interval (icfl)",
         f"{p.iamp}        Amplitude calculation interval (iamp)",
     ]

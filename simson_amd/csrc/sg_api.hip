@@ -93,13 +93,19 @@ struct simson_gpu {
     bool smag = false;
     SmagArgs sm{};
     double2 *SQ = nullptr, *SS = nullptr, *SW = nullptr;
-    double *dlt2 = nullptr;
+    double *dlt2 = nullptr, *SP = nullptr;
     size_t smag_lds = 0;
     // fringe of spatial simulations (simson_gpu_fringe): lambda [nxp/2] and
     // the base flow [3][ny][nxp/2] packed as the x rows, FP64 and FP32
     int fringe = 0;
     double2 *flam = nullptr, *fbase = nullptr;
     float2 *flam32 = nullptr, *fbase32 = nullptr;
+    // lower-wall blowing and suction (simson_gpu_wall): the spectra of
+    // max(P, P damp) and min(P, P damp), their (0,0) values, and the time
+    // at the start of the next substep (simson_gpu_set_time, advanced by run)
+    int wall = 0;
+    double wall_tomeg = 0., wall_p0 = 0., wall_m0 = 0., tnow = 0.;
+    double2 *wallP = nullptr, *wallM = nullptr;
     double mflux = 0., u0upp = 0., u0low = 0., w0upp = 0., w0low = 0.;
     bool timing = false;
     std::vector<TimedLaunch> launches;
@@ -192,8 +198,8 @@ void free_all(simson_gpu_t *g)
                     g->twz32, g->twx32, g->Tf32, g->Tb32, g->Q, g->Qc, g->PR, g->pscr,
                     g->Lm, g->lpf, g->Lm32, g->Wf, g->Wf32, g->clW, g->clY, g->PG, g->TAU,
                     g->DT, g->PG32, g->TAU32, g->DT32, g->Dm, g->Dm32, g->twzs,
-                    g->twxs, g->xysd, g->SQ, g->SS, g->SW, g->dlt2, g->flam, g->fbase,
-                    g->flam32, g->fbase32};
+                    g->twxs, g->xysd, g->SQ, g->SS, g->SW, g->dlt2, g->SP, g->flam, g->fbase,
+                    g->flam32, g->fbase32, g->wallP, g->wallM};
     for (void *q : ptrs)
         if (q)
             (void)hipFree(q);
@@ -253,6 +259,7 @@ NonlinArgs nonlin_args(simson_gpu_t *g, int do_cfl)
     a.fringe = g->fringe;
     a.flam = g->flam;
     a.fbase = g->fbase;
+    a.nocap = g->wall;
     return a;
 }
 
@@ -278,6 +285,9 @@ ColArgs col_args(simson_gpu_t *g)
     a.u0low = g->u0low;
     a.w0upp = g->w0upp;
     a.w0low = g->w0low;
+    a.vwall = nullptr;
+    a.vscale = 0.;
+    a.v0 = 0.;
     return a;
 }
 
@@ -357,6 +367,7 @@ Step2Args<float2> nonlin_args32(simson_gpu_t *g, int do_cfl)
     a.fringe = d.fringe;
     a.flam = g->flam32;
     a.fbase = g->fbase32;
+    a.nocap = d.nocap;
     return a;
 }
 
@@ -584,6 +595,13 @@ int do_linear(simson_gpu_t *g, double an, double bn, double anp1, double bnp1)
             launch_cheb_gemm(g->Tf, fw, g->stream);
     });
     ColArgs ca = col_args(g);
+    if (g->wall) {
+        // the wall velocity at the start of the substep (bla.f:1797, 2405)
+        const double sv = g->wall_tomeg == 0. ? 1. : std::sin(g->wall_tomeg * g->tnow);
+        ca.vwall = sv >= 0. ? g->wallP : g->wallM;
+        ca.vscale = sv;
+        ca.v0 = sv * (sv >= 0. ? g->wall_p0 : g->wall_m0);
+    }
     int st = 0;
     timed(g, F_TAU, [&] {
         st = launch_tau(ca, g->lam[slot], g->kfac, an, bn, anp1, bnp1, g->mixed, g->stream);
@@ -916,6 +934,8 @@ int simson_gpu_run(simson_gpu_t *g, int nsteps, double dt, int icfl, int first, 
         if (st)
             return st;
     }
+    static const double cnrk[3] = {0., 8. / 15., 2. / 3.};
+    const double tstep = g->tnow;
     std::vector<double> cfls;
     for (int it = 1; it <= nst * nsteps; it++) {
         const int do_cfl = icfl > 0 && ((it - 1) % icfl) == 0;
@@ -926,10 +946,13 @@ int simson_gpu_run(simson_gpu_t *g, int nsteps, double dt, int icfl, int first, 
         bn = bnp1;
         anp1 = dtnp1 * anrk[it % nst];
         bnp1 = dtnp1 * bnrk[it % nst];
+        // substep time t + c dt (bla.f:1797)
+        g->tnow = tstep + dt * (double)((it - 1) / nst) + dt * cnrk[(it - 1) % nst];
         st = do_linear(g, an, bn, anp1, bnp1);
         if (st)
             return st;
     }
+    g->tnow = tstep + dt * (double)nsteps;
     double c[2] = {0., 0.};
     CK(hipMemcpyAsync(c, g->cfl, 2 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
     CK(hipStreamSynchronize(g->stream));
@@ -1048,6 +1071,58 @@ int simson_gpu_xys(simson_gpu_t *g, double dtn, double xs, double zs, int nxys, 
     return 0;
 }
 
+int simson_gpu_set_time(simson_gpu_t *g, double t)
+{
+    if (!g)
+        return SIMSON_E_ARG;
+    g->tnow = t;
+    return 0;
+}
+
+int simson_wall_velocity(int nx, int nz, double xl, double zl, const simson_wall_t *w,
+                         double tc, double *vr, double *vi)
+{
+    if (!w || !vr || !vi)
+        return SIMSON_E_ARG;
+    const size_t nc = (size_t)nz * (nx / 2);
+    std::vector<double2> pm(nc), mn(nc);
+    if (wall_spectra(nx, nz, xl, zl, w, pm.data(), mn.data()) != 0)
+        return SIMSON_E_ARG;
+    const double sv = w->tomeg == 0. ? 1. : std::sin(w->tomeg * tc);
+    const std::vector<double2> &q = sv >= 0. ? pm : mn;
+    for (size_t i = 0; i < nc; i++) {
+        vr[i] = sv * q[i].x;
+        vi[i] = sv * q[i].y;
+    }
+    return 0;
+}
+
+int simson_gpu_wall(simson_gpu_t *g, const simson_wall_t *w)
+{
+    if (!g)
+        return SIMSON_E_ARG;
+    if (!w) {
+        g->wall = 0;
+        return 0;
+    }
+    const size_t nc = (size_t)g->nz * g->nxh;
+    std::vector<double2> pm(nc), mn(nc);
+    if (wall_spectra(g->nx, g->nz, g->p.xl, g->p.zl, w, pm.data(), mn.data()) != 0) {
+        g->err = "wall: xrise, xfall > 0 needed";
+        return SIMSON_E_ARG;
+    }
+    if (!g->wallP && (dalloc(g, &g->wallP, nc) || dalloc(g, &g->wallM, nc)))
+        return SIMSON_E_HIP;
+    if (h2d(g, g->wallP, pm.data(), nc) || h2d(g, g->wallM, mn.data(), nc))
+        return SIMSON_E_HIP;
+    CK(hipStreamSynchronize(g->stream));
+    g->wall_tomeg = w->tomeg;
+    g->wall_p0 = pm[0].x;
+    g->wall_m0 = mn[0].x;
+    g->wall = 1;
+    return 0;
+}
+
 int simson_fringe_function(int nx, double xl, const simson_fringe_t *f, double *lam)
 {
     if (!f || !lam)
@@ -1159,7 +1234,9 @@ int setup_smag(simson_gpu_t *g, const simson_les_t *p)
         if (h2d(g, g->dlt2, d2.data(), d2.size()))
             return SIMSON_E_HIP;
     }
-    if (dyn && !g->SW && dalloc(g, &g->SW, (size_t)21 * ny * nz * nq))
+    if (dyn && !g->SW &&
+        (dalloc(g, &g->SW, (size_t)21 * ny * nz * nq) ||
+         dalloc(g, &g->SP, (size_t)ny * smag_zblocks(nz, g->nxh, lds) * 2 * g->nx)))
         return SIMSON_E_HIP;
     g->smag_lds = lds;
     SmagArgs &a = g->sm;
@@ -1167,6 +1244,7 @@ int setup_smag(simson_gpu_t *g, const simson_les_t *p)
     a.Q = g->SQ;
     a.SS = g->SS;
     a.W = g->SW;
+    a.part = g->SP;
     a.TAU = g->TAU;
     a.alfa = g->alfa;
     a.beta = g->beta;

@@ -67,6 +67,11 @@ struct ColArgs {
     double mflux;
     int cflux;
     double u0upp, u0low, w0upp, w0low;
+    // imposed lower-wall v (wbci = 1): per column v(-1) = vscale vwall[t]
+    // (vwall [nz][nxh] or null), and the mean v = v0 (linearbl.f:458-469,
+    // 1663-1670)
+    const double2 *vwall;
+    double vscale, v0;
 };
 
 // Step-2 launch arguments.
@@ -107,6 +112,9 @@ struct Step2Args {
     // fbase [3][ny][nxp/2]; applied by the four-step x-row kernels
     int fringe;
     const V *flam, *fbase;
+    // no wall cap on the CFL's 1/dy (boxcfl.f:38-43 cap it only when no
+    // wall velocity is imposed)
+    int nocap;
 };
 typedef Step2Args<double2> NonlinArgs;
 
@@ -127,6 +135,11 @@ bool launch_xclos(const Step2Args<V> &a, const V *wx, const ClosArgs &c, hipStre
 // synthesised in z, into a.P as [9][ny][nzp][nx/2]
 template <typename V>
 bool launch_gsyn4(const Step2Args<V> &a, const V *tw, hipStream_t s);
+
+// spectra [nz][nx/2] of the wbci = 1 wall velocity patterns max(P, P damp)
+// and min(P, P damp), P = amp f(x) cos(zbet z) on the 3/2 grid (host)
+int wall_spectra(int nx, int nz, double xl, double zl, const simson_wall_t *w, double2 *pmax,
+                 double2 *pmin);
 
 // fringe function of ffun.f at the nxp physical points of the 3/2 x grid
 // (host); 0 or SIMSON_E_ARG
@@ -176,12 +189,14 @@ struct XysArgs {
 struct SmagArgs {
     const double2 *U;
     double2 *Q, *SS, *W, *TAU;
+    double *part;                 // [ny][nzb][2][nx] partial z sums (dynamic)
     const double *alfa, *beta, *dlt2;
     int nx, ny, nz, nxh, nq;
     int dynamic, ineg, rt;
     double u0low, w0low, cs2, tw2, re, hscale;
 };
 bool smag_fits(int nz, int nxh, size_t lds_max);
+int smag_zblocks(int nz, int nxh, size_t lds_max);
 int smag_prepare(size_t lds_max);
 void launch_smag(SmagArgs a, const FftPlan &pz, const FftPlan &pm, const double2 *wx,
                  size_t lds_max, hipStream_t s);

@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(256) k_xrow(NonlinArgs a, FftPlan pm, const do
     // products on the physical row (even points in .x, odd points in .y)
     double cflmax = 0.;
     double dyinv = 1.0 / a.deta[y];
-    if (a.ny >= 17) {
+    if (a.ny >= 17 && !a.nocap) {
         const double cap = (double)(a.ny - 1) * (double)(a.ny - 1) / 10. / 3.14;
         dyinv = fmin(dyinv, cap);
     }
@@ -339,7 +339,7 @@ __global__ void __launch_bounds__(256) k_xrow_t(NonlinArgs a, const double2 *__r
     double2 *oth = (res == A) ? Bf : A;
     double cflmax = 0.;
     double dyinv = 1.0 / a.deta[y];
-    if (a.ny >= 17)
+    if (a.ny >= 17 && !a.nocap)
         dyinv = fmin(dyinv, (double)(a.ny - 1) * (double)(a.ny - 1) / 10. / 3.14);
     const double dxi = (double)(a.nx - 2) / a.xl;
     const double dzi = (double)(a.nz - 2) / a.zl;

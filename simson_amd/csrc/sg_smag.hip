@@ -18,10 +18,12 @@
 //              cutoff kx < nx/4, into W [21][ny][nz][nx/4];
 //   k_smag_zf  the z part of the test filter: forward z transform, cutoff
 //              |kz| <= nz/4, inverse, scaling 1/(nx nz) (filter_cutoff.f);
-//   k_smag_xc  one block per y plane: inverse x transform of the filtered
-//              fields, L_ij, M_ij, the z sums of M.L and M.M and the clipped
-//              coefficient C(x) (les.f:784-880, 1380-1510), then nu_t and
-//              tau_ij / (nx nz) with the forward x transform
+//   k_smag_xc1 per block of z rows: inverse x transform of the filtered
+//              fields, L_ij, M_ij and the partial z sums of M.L and M.M
+//              (les.f:784-880, 1380-1416);
+//   k_smag_xc2 per block of z rows: the clipped coefficient C(x) from the
+//              partial sums, added in a fixed order (les.f:1418-1510), then
+//              nu_t and tau_ij / (nx nz) with the forward x transform
 //              (les.f:1530-1620), into Q as [6][ny][nz][nx/2];
 //   k_smag_za  forward z transform (vcfftf), the oddball row zeroed
 //              (les.f:1630-1645), scaled by nxp nzp to the normalisation of
@@ -253,129 +255,141 @@ __global__ void __launch_bounds__(kT) k_smag_zf(SmagArgs a, FftPlan pz, int B)
     }
 }
 
-__global__ void __launch_bounds__(kT) k_smag_xc(SmagArgs a, FftPlan pm,
-                                                const double2 *__restrict__ wx, int RW)
+// dynamic procedure, pass 1: the RW rows z0.. of plane y (block (bz, y),
+// z0 = RW bz): inverse x transform of the filtered fields, L_ij, M_ij, and
+// the partial z sums of M.L and M.M per x point into part [ny][nzb][2][nx]
+__global__ void __launch_bounds__(kT) k_smag_xc1(SmagArgs a, FftPlan pm,
+                                                 const double2 *__restrict__ wx, int RW)
 {
     extern __shared__ double2 lds[];
     __shared__ double red[2 * 2 * kT];
-    __shared__ double cx[kMaxNx];
-    const int M = a.nxh, nz = a.nz, ny = a.ny, nq = a.nq, y = blockIdx.x;
+    const int M = a.nxh, nz = a.nz, ny = a.ny, nq = a.nq;
+    const int bz = blockIdx.x, y = blockIdx.y, nzb = gridDim.x;
     const int nx = 2 * M, tid = threadIdx.x;
-    const double d2 = a.dlt2[y];
-    if (a.dynamic) {
-        double2 *A = lds, *Bf = lds + (size_t)21 * RW * M;
-        const int G = kT / M, n = tid % M, grp = tid / M;
-        double num[2] = {0., 0.}, den[2] = {0., 0.};
-        const double gd2 = a.tw2 * d2;
-        for (int z0 = 0; z0 < nz; z0 += RW) {
-            const int nr = min(RW, nz - z0);
-            for (int idx = tid; idx < 21 * RW * M; idx += blockDim.x) {
-                const int f = idx / (RW * M);
-                const int rem = idx - f * RW * M;
-                const int r = rem / M, k = rem - r * M;
-                double2 Z = make_double2(0., 0.);
-                if (r < nr)
-                    Z = pack_row(a.W + qidx(f, y, z0 + r, 0, ny, nz, nq), k, M, nq, wx);
-                A[(f * RW + r) * M + k] = Z;
-            }
-            __syncthreads();
-            const double2 *res = lds_fft(A, Bf, M, 21 * RW, pm, +1.0);
-            if (grp < G) {
-                for (int r = grp; r < nr; r += G) {
-                    double2 q[21];
+    const int z0 = bz * RW, nr = min(RW, nz - z0);
+    double2 *A = lds, *Bf = lds + (size_t)21 * RW * M;
+    const int G = kT / M, n = tid % M, grp = tid / M;
+    double num[2] = {0., 0.}, den[2] = {0., 0.};
+    const double gd2 = a.tw2 * a.dlt2[y];
+    for (int idx = tid; idx < 21 * RW * M; idx += blockDim.x) {
+        const int f = idx / (RW * M);
+        const int rem = idx - f * RW * M;
+        const int r = rem / M, k = rem - r * M;
+        double2 Z = make_double2(0., 0.);
+        if (r < nr)
+            Z = pack_row(a.W + qidx(f, y, z0 + r, 0, ny, nz, nq), k, M, nq, wx);
+        A[(f * RW + r) * M + k] = Z;
+    }
+    __syncthreads();
+    const double2 *res = lds_fft(A, Bf, M, 21 * RW, pm, +1.0);
+    if (grp < G) {
+        for (int r = grp; r < nr; r += G) {
+            double2 q[21];
 #pragma unroll
-                    for (int f = 0; f < 21; f++)
-                        q[f] = res[(f * RW + r) * M + n];
+            for (int f = 0; f < 21; f++)
+                q[f] = res[(f * RW + r) * M + n];
 #pragma unroll
-                    for (int e = 0; e < 2; e++) {
-                        double v[21];
+            for (int e = 0; e < 2; e++) {
+                double v[21];
 #pragma unroll
-                        for (int f = 0; f < 21; f++)
-                            v[f] = e ? q[f].y : q[f].x;
-                        const int pa[6] = {0, 1, 2, 1, 2, 2}, pb[6] = {0, 0, 0, 1, 1, 2};
-                        double L[6], Mm[6];
-                        const double ts = sabs_of(v + 12);
-                        const double dr = gd2 * ts;
+                for (int f = 0; f < 21; f++)
+                    v[f] = e ? q[f].y : q[f].x;
+                const int pa[6] = {0, 1, 2, 1, 2, 2}, pb[6] = {0, 0, 0, 1, 1, 2};
+                double L[6], Mm[6];
+                const double dr = gd2 * sabs_of(v + 12);
 #pragma unroll
-                        for (int k = 0; k < 6; k++) {
-                            L[k] = v[k] + v[18 + pa[k]] * v[18 + pb[k]];
-                            Mm[k] = v[6 + k] + v[12 + k] * dr;
-                        }
-                        num[e] += Mm[0] * L[0] + Mm[3] * L[3] + Mm[5] * L[5] +
-                                  2. * (Mm[1] * L[1] + Mm[2] * L[2] + Mm[4] * L[4]);
-                        den[e] += Mm[0] * Mm[0] + Mm[3] * Mm[3] + Mm[5] * Mm[5] +
-                                  2. * (Mm[1] * Mm[1] + Mm[2] * Mm[2] + Mm[4] * Mm[4]);
-                    }
+                for (int k = 0; k < 6; k++) {
+                    L[k] = v[k] + v[18 + pa[k]] * v[18 + pb[k]];
+                    Mm[k] = v[6 + k] + v[12 + k] * dr;
                 }
+                num[e] += Mm[0] * L[0] + Mm[3] * L[3] + Mm[5] * L[5] +
+                          2. * (Mm[1] * L[1] + Mm[2] * L[2] + Mm[4] * L[4]);
+                den[e] += Mm[0] * Mm[0] + Mm[3] * Mm[3] + Mm[5] * Mm[5] +
+                          2. * (Mm[1] * Mm[1] + Mm[2] * Mm[2] + Mm[4] * Mm[4]);
             }
-            __syncthreads();
         }
-        if (grp < G) {
-            red[(grp * 2 + 0) * 2 * M + 2 * n] = num[0];
-            red[(grp * 2 + 0) * 2 * M + 2 * n + 1] = num[1];
-            red[(grp * 2 + 1) * 2 * M + 2 * n] = den[0];
-            red[(grp * 2 + 1) * 2 * M + 2 * n + 1] = den[1];
-        }
-        __syncthreads();
-        for (int x = tid; x < nx; x += blockDim.x) {
+        red[(grp * 2 + 0) * 2 * M + 2 * n] = num[0];
+        red[(grp * 2 + 0) * 2 * M + 2 * n + 1] = num[1];
+        red[(grp * 2 + 1) * 2 * M + 2 * n] = den[0];
+        red[(grp * 2 + 1) * 2 * M + 2 * n + 1] = den[1];
+    }
+    __syncthreads();
+    double *out = a.part + ((size_t)y * nzb + bz) * 2 * nx;
+    for (int idx = tid; idx < 2 * nx; idx += blockDim.x) {
+        const int c = idx / nx, x = idx - c * nx;
+        double t = 0.;
+        for (int gq = 0; gq < G; gq++)
+            t += red[(gq * 2 + c) * nx + x];
+        out[idx] = t;
+    }
+}
+
+// pass 2 (block (bz, y), rows RT bz..): the coefficient C(x, y) from the
+// partial sums (dynamic, clipped by ineg) or cs^2, then nu_t, tau_ij /
+// (nx nz) and the forward x transform into Q as [6][ny][nz][nx/2]
+__global__ void __launch_bounds__(kT) k_smag_xc2(SmagArgs a, FftPlan pm,
+                                                 const double2 *__restrict__ wx, int RT,
+                                                 int nzb)
+{
+    extern __shared__ double2 lds[];
+    __shared__ double cx[kMaxNx];
+    const int M = a.nxh, nz = a.nz, ny = a.ny, y = blockIdx.y;
+    const int nx = 2 * M, tid = threadIdx.x;
+    const int z0 = blockIdx.x * RT, nr = min(RT, nz - z0);
+    const double d2 = a.dlt2[y];
+    for (int x = tid; x < nx; x += blockDim.x) {
+        double c = a.cs2;
+        if (a.dynamic) {
+            const double *pp = a.part + (size_t)y * nzb * 2 * nx;
             double cn = 0., cd = 0.;
-            for (int gq = 0; gq < G; gq++) {
-                cn += red[(gq * 2 + 0) * nx + x];
-                cd += red[(gq * 2 + 1) * nx + x];
+            for (int b = 0; b < nzb; b++) {
+                cn += pp[(size_t)b * 2 * nx + x];
+                cd += pp[(size_t)b * 2 * nx + nx + x];
             }
-            double c = .5 * cn / cd;
+            c = .5 * cn / cd;
             if (a.ineg == 1)
                 c = fmax(c, 0.);
             if (a.ineg != 0 && (y == 0 || y == ny - 1))
                 c = 0.;
-            cx[x] = c;
         }
-    } else {
-        for (int x = tid; x < nx; x += blockDim.x)
-            cx[x] = a.cs2;
+        cx[x] = c;
     }
     __syncthreads();
-    // tau_ij / (nx nz) at physical x-z, then the forward x transform
-    const int RT = a.rt;
     double2 *A = lds, *Bf = lds + (size_t)6 * RT * M;
     const double gd = 1. / ((double)nx * (double)nz);
-    for (int z0 = 0; z0 < nz; z0 += RT) {
-        const int nr = min(RT, nz - z0);
-        for (int idx = tid; idx < RT * M; idx += blockDim.x) {
-            const int r = idx / M, n = idx - r * M;
-            double2 t[6];
+    for (int idx = tid; idx < RT * M; idx += blockDim.x) {
+        const int r = idx / M, n = idx - r * M;
+        double2 t[6];
 #pragma unroll
-            for (int k = 0; k < 6; k++)
-                t[k] = make_double2(0., 0.);
-            if (r < nr) {
-                const double2 sa = a.SS[qidx(6, y, z0 + r, n, ny, nz, M)];
+        for (int k = 0; k < 6; k++)
+            t[k] = make_double2(0., 0.);
+        if (r < nr) {
+            const double2 sa = a.SS[qidx(6, y, z0 + r, n, ny, nz, M)];
 #pragma unroll
-                for (int e = 0; e < 2; e++) {
-                    double nu = cx[2 * n + e] * (e ? sa.y : sa.x) * d2;
-                    if (a.ineg == 2 && nu < -1. / a.re)
-                        nu = -1. / a.re;
-                    const double d1 = -2. * nu * gd;
+            for (int e = 0; e < 2; e++) {
+                double nu = cx[2 * n + e] * (e ? sa.y : sa.x) * d2;
+                if (a.ineg == 2 && nu < -1. / a.re)
+                    nu = -1. / a.re;
+                const double d1 = -2. * nu * gd;
 #pragma unroll
-                    for (int k = 0; k < 6; k++) {
-                        const double2 s = a.SS[qidx(k, y, z0 + r, n, ny, nz, M)];
-                        (e ? t[k].y : t[k].x) = d1 * (e ? s.y : s.x);
-                    }
+                for (int k = 0; k < 6; k++) {
+                    const double2 s = a.SS[qidx(k, y, z0 + r, n, ny, nz, M)];
+                    (e ? t[k].y : t[k].x) = d1 * (e ? s.y : s.x);
                 }
             }
+        }
 #pragma unroll
-            for (int k = 0; k < 6; k++)
-                A[(k * RT + r) * M + n] = t[k];
-        }
-        __syncthreads();
-        const double2 *fin = lds_fft(A, Bf, M, 6 * RT, pm, -1.0);
-        for (int idx = tid; idx < 6 * RT * M; idx += blockDim.x) {
-            const int k = idx / (RT * M);
-            const int rem = idx - k * RT * M;
-            const int r = rem / M, kx = rem - r * M;
-            if (r < nr)
-                a.Q[qidx(k, y, z0 + r, kx, ny, nz, M)] = unpack_row(fin + (k * RT + r) * M, kx, M, wx);
-        }
-        __syncthreads();
+        for (int k = 0; k < 6; k++)
+            A[(k * RT + r) * M + n] = t[k];
+    }
+    __syncthreads();
+    const double2 *fin = lds_fft(A, Bf, M, 6 * RT, pm, -1.0);
+    for (int idx = tid; idx < 6 * RT * M; idx += blockDim.x) {
+        const int k = idx / (RT * M);
+        const int rem = idx - k * RT * M;
+        const int r = rem / M, kx = rem - r * M;
+        if (r < nr)
+            a.Q[qidx(k, y, z0 + r, kx, ny, nz, M)] = unpack_row(fin + (k * RT + r) * M, kx, M, wx);
     }
 }
 
@@ -417,6 +431,12 @@ int smag_rows(int nxh, int nfield, size_t lds_max)
     return (int)std::max<size_t>(1, lds_max / per);
 }
 
+int smag_zblocks(int nz, int nxh, size_t lds_max)
+{
+    const int RC = std::min(nz, smag_rows(nxh, 21, lds_max));
+    return (nz + RC - 1) / RC;
+}
+
 bool smag_fits(int nz, int nxh, size_t lds_max)
 {
     return nz <= kLdsZ && 2 * nxh <= kMaxNx && nxh <= kT &&
@@ -425,7 +445,8 @@ bool smag_fits(int nz, int nxh, size_t lds_max)
 
 int smag_prepare(size_t lds_max)
 {
-    const void *fns[] = {(const void *)k_smag_xa, (const void *)k_smag_xc};
+    const void *fns[] = {(const void *)k_smag_xa, (const void *)k_smag_xc1,
+                         (const void *)k_smag_xc2};
     for (const void *fn : fns)
         if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max) !=
             hipSuccess)
@@ -448,11 +469,15 @@ void launch_smag(SmagArgs a, const FftPlan &pz, const FftPlan &pm, const double2
         hipLaunchKernelGGL(k_smag_zf, dim3((a.nq + Bq - 1) / Bq, 21, a.ny), dim3(kT),
                            (size_t)2 * Bq * a.nz * sizeof(double2), s, a, pz, Bq);
     }
-    const int RC = a.dynamic ? std::min(a.nz, smag_rows(a.nxh, 21, lds_max)) : 1;
-    a.rt = std::min(a.nz, smag_rows(a.nxh, 6, lds_max));
-    const size_t lc = std::max((size_t)2 * 21 * RC * a.nxh * (a.dynamic ? 1 : 0),
-                               (size_t)2 * 6 * a.rt * a.nxh) * sizeof(double2);
-    hipLaunchKernelGGL(k_smag_xc, dim3(a.ny), dim3(kT), lc, s, a, pm, wx, RC);
+    if (a.dynamic) {
+        const int RC = std::min(a.nz, smag_rows(a.nxh, 21, lds_max));
+        hipLaunchKernelGGL(k_smag_xc1, dim3((a.nz + RC - 1) / RC, a.ny), dim3(kT),
+                           (size_t)2 * 21 * RC * a.nxh * sizeof(double2), s, a, pm, wx, RC);
+    }
+    const int RT = std::min(a.nz, smag_rows(a.nxh, 6, lds_max));
+    hipLaunchKernelGGL(k_smag_xc2, dim3((a.nz + RT - 1) / RT, a.ny), dim3(kT),
+                       (size_t)2 * 6 * RT * a.nxh * sizeof(double2), s, a, pm, wx, RT,
+                       smag_zblocks(a.nz, a.nxh, lds_max));
     hipLaunchKernelGGL(k_smag_za, dim3((a.nxh + B - 1) / B, 6, a.ny), dim3(kT), lz, s, a, pz, B);
 }
 

@@ -516,7 +516,15 @@ __global__ void __launch_bounds__(256, (tau_waves_per_eu<SW, E>()))
     // v from phi with k^2 (linearbl.f:1000-1001)
     {
         io.load(kf.gp, tmp);
-        const double bcv[2][2] = {{0., 0.}, {0., 0.}};
+        // v(+1) = 0 and v(-1) = the imposed wall velocity: (w/2, -w/2) for
+        // the even and odd classes (linearbl.f:458-469, 944-945)
+        double2 wv = make_double2(0., 0.);
+        if (a.vwall && !k0) {
+            wv = a.vwall[t];
+            wv.x *= .5 * a.vscale;
+            wv.y *= .5 * a.vscale;
+        }
+        const double bcv[2][2] = {{wv.x, -wv.x}, {wv.y, -wv.y}};
         solve_w<SW, 2, E, KT>(n, k2, tmp, tw, v, bcv, l);
     }
     // superpose v_h per parity so that dv/dy = 0 at the walls
@@ -720,7 +728,7 @@ __global__ void __launch_bounds__(256, (tau_waves_per_eu<SW, E>()))
             a.puw[n + k] = -d2uw[1][c][e] - c1n * uw[1][c][e] - c2 * h3r[c][e];
             CV *Ck = Cw + (size_t)t * n;
             put(Ck + k, uw[0][c][e], 0.);
-            put(Ck + fs + k, 0., 0.);
+            put(Ck + fs + k, k == 0 ? a.v0 : 0., 0.);   // mean v: the mean wall v
             put(Ck + 2 * fs + k, uw[1][c][e], 0.);
             put(Ck + 3 * fs + k, duw[1][c][e], 0.);
             put(Ck + 4 * fs + k, -duw[0][c][e], 0.);
@@ -729,7 +737,7 @@ __global__ void __launch_bounds__(256, (tau_waves_per_eu<SW, E>()))
                 // that downloads and the pressure see the FP64 mean profiles
                 double2 *C64 = a.C + (size_t)t * n;
                 C64[k] = make_double2(uw[0][c][e], 0.);
-                C64[fs + k] = make_double2(0., 0.);
+                C64[fs + k] = make_double2(k == 0 ? a.v0 : 0., 0.);
                 C64[2 * fs + k] = make_double2(uw[1][c][e], 0.);
                 C64[3 * fs + k] = make_double2(duw[1][c][e], 0.);
                 C64[4 * fs + k] = make_double2(-duw[0][c][e], 0.);

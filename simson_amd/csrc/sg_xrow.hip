@@ -163,13 +163,14 @@ __global__ void __launch_bounds__((XBlock<V, M>::NT)) k_xrow4(Step2Args<V> a, co
     // inverse pass 2, products, forward pass A (one block per task)
     T cflmax = 0;
     T dyinv = (T)(1.0 / a.deta[y]);
-    if (a.ny >= 17)
+    if (a.ny >= 17 && !a.nocap)
         dyinv = fmin(dyinv, (T)((double)(a.ny - 1) * (double)(a.ny - 1) / 10. / 3.14));
     const T dxi = (T)((double)(a.nx - 2) / a.xl);
     const T dzi = (T)((double)(a.nz - 2) / a.zl);
     const T r2 = (T)(2.0 * a.rot);
     // mean profiles of this plane when the fields hold only fluctuations
-    // (storage slots 0 u, 2 w, 3 om_x, 4 om_z at kx = kz = 0; V = Om_y = 0)
+    // (storage slots 0 u, 2 w, 3 om_x, 4 om_z at kx = kz = 0; Om_y = 0, and
+    // the mean v, nonzero only with an imposed wall velocity, stays in v)
     T Um = 0, Wm = 0, Oxm = 0, Ozm = 0;
     if (a.split_mean) {
         const size_t F = (size_t)a.nz * ny * nxh, yo = (size_t)y * nxh;

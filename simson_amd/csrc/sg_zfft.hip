@@ -148,8 +148,10 @@ __global__ void __launch_bounds__((ZPlan<V, N>::NT)) k_zsyn4(Step2Args<V> a, con
                 v = vmk<V>(-bt[q] * u[q].y + al * w[q].y, bt[q] * u[q].x - al * w[q].x);
             }
             // with split_mean the kx = kz = 0 mode enters the products
-            // directly (step-2 fields only)
-            if (b >= nb || z < 0 || (MODE == 0 && a.split_mean && z == 0 && x0 + b == 0))
+            // directly (step-2 fields only); the mean v (nonzero only with
+            // an imposed wall velocity) stays in the field
+            if (b >= nb || z < 0 ||
+                (MODE == 0 && a.split_mean && f != 1 && z == 0 && x0 + b == 0))
                 v = vmk<V>(0, 0);
             A[b * SCOL + zp + zp / N1] = v;
         }

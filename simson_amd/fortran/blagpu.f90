@@ -33,6 +33,8 @@ program blagpu
   logical :: chkstop, tabfre, rbfl, osmod, streak, waves
   ! spatial simulations: fringe (ffun.f) and base flow [3][nyp][nxp]
   type(simson_fringe_t) :: fr
+  type(simson_wall_t) :: wl
+  real(dp) :: cnrk(3)
   real(dp) :: ampob, amp2d, bstart, blength
   real(dp), allocatable :: base(:,:,:)
   ! field and header
@@ -54,6 +56,7 @@ program blagpu
   integer(8) :: c0, c1, crate
 
   anrk = [8._dp/15._dp, 5._dp/12._dp, 3._dp/4._dp]
+  cnrk = [0._dp, 8._dp/15._dp, 2._dp/3._dp]
   bnrk = [0._dp, -17._dp/60._dp, -5._dp/12._dp]
 
   ! ---------------- bla.i (rparambl.f) ----------------
@@ -119,6 +122,22 @@ program blagpu
   call nxt(line); read(line, *) loctyp
   call nxt(line); read(line, *) tripf
   call nxt(line); read(line, *) wbci
+  if (wbci == 1) then
+     ! rparambl.f:1206-1232: the blowing/suction strip of cwallbc.f
+     call nxt(line); read(line, *) wl%amp
+     call nxt(line); read(line, *) wl%damp
+     call nxt(line); read(line, *) wl%xstart
+     call nxt(line); read(line, *) wl%xend
+     call nxt(line); read(line, *) wl%xrise
+     call nxt(line); read(line, *) wl%xfall
+     call nxt(line); read(line, *) wl%zbet
+     call nxt(line); read(line, *) wl%tomeg
+     wl%xsc = 0._dp
+     wl%zsc = 0._dp
+  else if (wbci /= 0) then
+     write(*,*) 'blagpu: bla.i requests a feature outside the channel step library'
+     call stopnow(13)
+  end if
   call nxt(line); read(line, *) icfl
   call nxt(line); read(line, *) iamp
   if (iamp > 0) then
@@ -159,7 +178,7 @@ program blagpu
   call nxt(line); read(line, *) npl
   close(10)
   if (pert .or. cim .or. suction .or. isfd /= 0 .or. imhd /= 0 .or. &
-       loctyp /= 0 .or. tripf .or. wbci /= 0 .or. ibc /= 0 .or. nst /= 3 &
+       loctyp /= 0 .or. tripf .or. (wbci /= 0 .and. wbci /= 1) .or. ibc /= 0 .or. nst /= 3 &
        .or. mwave > 0 .or. npl > 0 &
        .or. dt <= 0._dp) then
      write(*,*) 'blagpu: bla.i requests a feature outside the channel step library'
@@ -262,6 +281,7 @@ program blagpu
      end do
      call chk(simson_gpu_fringe(g, fr, base))
   end if
+  if (wbci == 1) call chk(simson_gpu_wall(g, wl))
   call chk(simson_gpu_upload(g, ur, ui, 3_c_int, c_null_ptr, px))
   ! Clenshaw-Curtis weights (preprbl.f:78-87) for the amplitude file
   allocate(amp(nyp, 20), wint(nyp))
@@ -327,6 +347,8 @@ program blagpu
      bn = bnp1
      anp1 = dtnp1 * anrk(mod(it, nst) + 1)
      bnp1 = dtnp1 * bnrk(mod(it, nst) + 1)
+     ! time of the start of this substep for the wall forcing (bla.f:1797)
+     call chk(simson_gpu_set_time(g, t + dtn * cnrk(mod(it - 1, nst) + 1)))
      call chk(simson_gpu_linear(g, an, bn, anp1, bnp1, c_null_ptr))
      it = it + 1
      if (mod(it - 1, nst) == 0) then

@@ -28,6 +28,10 @@ module simson_gpu_mod
      real(c_double) :: xsc
   end type simson_fringe_t
 
+  type, bind(c) :: simson_wall_t
+     real(c_double) :: amp, damp, xstart, xend, xrise, xfall, zbet, tomeg, xsc, zsc
+  end type simson_wall_t
+
   type, bind(c) :: simson_les_t
      integer(c_int) :: iles
      integer(c_int) :: iord
@@ -107,6 +111,18 @@ module simson_gpu_mod
        type(simson_fringe_t), intent(in) :: f
        real(c_double), intent(in) :: base(*)
      end function simson_gpu_fringe
+
+     integer(c_int) function simson_gpu_wall(g, w) bind(c, name='simson_gpu_wall')
+       import :: c_int, c_ptr, simson_wall_t
+       type(c_ptr), value :: g
+       type(simson_wall_t), intent(in) :: w
+     end function simson_gpu_wall
+
+     integer(c_int) function simson_gpu_set_time(g, t) bind(c, name='simson_gpu_set_time')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: g
+       real(c_double), value :: t
+     end function simson_gpu_set_time
 
      integer(c_int) function simson_gpu_les(g, les) bind(c, name='simson_gpu_les')
        import :: c_int, c_ptr, simson_les_t

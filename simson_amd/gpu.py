@@ -29,7 +29,8 @@ EXPORTS = (
     "simson_gpu_run", "simson_gpu_download", "simson_gpu_amp", "simson_gpu_xys",
     "simson_gpu_pressure",
     "simson_gpu_les", "simson_les_filter", "simson_gpu_closure", "simson_gpu_sync",
-    "simson_gpu_fringe", "simson_fringe_function",
+    "simson_gpu_fringe", "simson_fringe_function", "simson_gpu_wall", "simson_gpu_set_time",
+    "simson_wall_velocity",
     "simson_mlp_create", "simson_mlp_nparams", "simson_mlp_params", "simson_mlp_set_data",
     "simson_mlp_train", "simson_mlp_grad", "simson_mlp_infer", "simson_mlp_last_error",
     "simson_mlp_destroy",
@@ -55,6 +56,11 @@ class SimsonLes(C.Structure):
 class SimsonFringe(C.Structure):
     _fields_ = [("fmax", C.c_double), ("fstart", C.c_double), ("fend", C.c_double),
                 ("frise", C.c_double), ("ffall", C.c_double), ("xsc", C.c_double)]
+
+
+class SimsonWall(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("amp", "damp", "xstart", "xend", "xrise", "xfall",
+                                         "zbet", "tomeg", "xsc", "zsc")]
 
 
 class SimsonError(RuntimeError):
@@ -97,6 +103,10 @@ def lib():
                                          C.c_void_p, C.c_void_p]
         L.simson_gpu_sync.argtypes = [vp]
         L.simson_gpu_fringe.argtypes = [vp, C.POINTER(SimsonFringe), C.c_void_p]
+        L.simson_gpu_wall.argtypes = [vp, C.POINTER(SimsonWall)]
+        L.simson_gpu_set_time.argtypes = [vp, C.c_double]
+        L.simson_wall_velocity.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double,
+                                           C.POINTER(SimsonWall), C.c_double, _dp, _dp]
         L.simson_fringe_function.argtypes = [C.c_int, C.c_double, C.POINTER(SimsonFringe), _dp]
         L.simson_gpu_timing_enable.argtypes = [vp, C.c_int]
         L.simson_gpu_timing_read.argtypes = [vp, C.POINTER(C.c_char_p),
@@ -239,6 +249,20 @@ class BlaGPU:
         b = np.ascontiguousarray(base, np.float64)
         self._check(lib().simson_gpu_fringe(self.h, C.byref(f), b.ctypes.data_as(C.c_void_p)))
 
+    def wall(self, amp=None, damp=1.0, xstart=0.0, xend=0.0, xrise=1.0, xfall=1.0, zbet=0.0,
+             tomeg=0.0, xsc=0.0, zsc=0.0):
+        """Blowing and suction at the lower wall (bla.i wbci = 1); amp None
+        switches it off."""
+        if amp is None:
+            self._check(lib().simson_gpu_wall(self.h, None))
+            return
+        w = SimsonWall(amp, damp, xstart, xend, xrise, xfall, zbet, tomeg, xsc, zsc)
+        self._check(lib().simson_gpu_wall(self.h, C.byref(w)))
+
+    def set_time(self, t: float):
+        """Time of the start of the next substep (time-dependent forcing)."""
+        self._check(lib().simson_gpu_set_time(self.h, t))
+
     def closure(self, params, in_scale=None, in_shift=None, yfeat=None, out_scale=None,
                 out_shift=None):
         """Learned closure in the step (params None turns it off); see
@@ -312,6 +336,19 @@ def fringe_function(nx: int, xl: float, fmax: float, fstart: float, fend: float,
     if st != 0:
         raise SimsonError(st, "invalid fringe parameters")
     return lam
+
+
+def wall_velocity(nx: int, nz: int, xl: float, zl: float, tc: float, amp: float,
+                  damp: float = 1.0, xstart: float = 0.0, xend: float = 0.0, xrise: float = 1.0,
+                  xfall: float = 1.0, zbet: float = 0.0, tomeg: float = 0.0) -> np.ndarray:
+    """The library's spectral lower-wall v [nz][nx/2] at time tc (host)."""
+    vr = np.zeros((nz, nx // 2))
+    vi = np.zeros_like(vr)
+    w = SimsonWall(amp, damp, xstart, xend, xrise, xfall, zbet, tomeg, 0.0, 0.0)
+    st = lib().simson_wall_velocity(nx, nz, xl, zl, C.byref(w), tc, vr, vi)
+    if st != 0:
+        raise SimsonError(st, "invalid wall parameters")
+    return vr + 1j * vi
 
 
 def rk3_weights(dt: float):

@@ -8,6 +8,7 @@ Runs in the build container only (it needs /root/reference and amdflang):
     python tests/golden/make_golden.py xys        # xys_32x33x32.npz
     python tests/golden/make_golden.py smag       # smag_32x33x32.npz
     python tests/golden/make_golden.py fringe     # fringe_32x33x32.npz
+    python tests/golden/make_golden.py wall       # wall_32x33x32.npz
 
 1. builds the reference `bla` for 32x33x32 (serial, pressure off) with
    oracle/build_ref.sh into oracle/_ref/;
@@ -308,6 +309,49 @@ def fringe():
     print("wrote", os.path.join(HERE, "fringe_32x33x32.npz"), sorted(res))
 
 
+WALL_CASES = {
+    # (amp, damp, xstart, xend, xrise, xfall, zbet, tomeg) of bla.i wbci = 1
+    "a": (0.05, 0.5, -3.0, 2.0, 1.0, 1.5, 2.0, 20.0),
+    "b": (0.02, 1.0, -5.0, -1.0, 2.0, 0.5, 1.0, 0.0),
+}
+
+
+def wall():
+    """Blowing and suction fixture: the temporal channel from the LES start
+    field with the lower-wall strip of bla.i wbci = 1 (cwallbc.f), 10 steps
+    at dt = 0.01 under constant flux; case a oscillates in time with the
+    suction part damped, case b is steady.  Keys <case>_u1, <case>_u10 and
+    the CFL*dt log lines <case>_cfl (boxcfl.f without the wall cap)."""
+    if not os.path.exists(BIN):
+        subprocess.check_call([os.path.join(REPO, "oracle", "build_ref.sh"),
+                               str(NX), str(NY), str(NZ), "1", "no", "0",
+                               os.path.basename(BIN)])
+    res = {}
+    for key, (amp, damp, x0, x1, xr, xf, zb, om) in WALL_CASES.items():
+        work = tempfile.mkdtemp(prefix="simson_golden_wall.")
+        try:
+            ufile.write_u(os.path.join(work, "init.u"), les_field())
+            p = blai.BlaParams(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=30,
+                               dt=0.01, nst=3, xl=4 * math.pi, cflux=True, icfl=3, wbci=1,
+                               wamp=amp, wdamp=damp, wxstart=x0, wxend=x1, wxrise=xr,
+                               wxfall=xf, wzbet=zb, wtomeg=om, tsave=[0.01], nmsave=["t1.u"])
+            with open(os.path.join(work, "bla.i"), "w") as fh:
+                fh.write(blai.format_channel(p))
+            with open(os.path.join(work, "out.txt"), "w") as out:
+                subprocess.check_call([BIN], cwd=work, stdout=out, stderr=subprocess.STDOUT)
+            log = open(os.path.join(work, "out.txt")).read()
+            if "bla terminated normally" not in log:
+                raise RuntimeError(f"reference bla (wall {key}) did not terminate normally")
+            res[f"{key}_u1"] = ufile.read_u(os.path.join(work, "t1.u")).u
+            res[f"{key}_u10"] = ufile.read_u(os.path.join(work, "end.u")).u
+            res[f"{key}_cfl"] = np.array(
+                [float(m) for m in re.findall(r"CFL\*dt\s+([0-9.Ee+-]+)", log)])
+        finally:
+            shutil.rmtree(work)
+    np.savez_compressed(os.path.join(HERE, "wall_32x33x32.npz"), **res)
+    print("wrote", os.path.join(HERE, "wall_32x33x32.npz"), sorted(res))
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "pressure":
         pressure()
@@ -319,5 +363,7 @@ if __name__ == "__main__":
         smag()
     elif len(sys.argv) > 1 and sys.argv[1] == "fringe":
         fringe()
+    elif len(sys.argv) > 1 and sys.argv[1] == "wall":
+        wall()
     else:
         main()

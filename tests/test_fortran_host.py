@@ -296,3 +296,37 @@ def test_fortran_host_fringe_matches_reference_bla():
                 assert err <= 1e-12, (name, c, err)
     finally:
         shutil.rmtree(work)
+
+
+@pytest.mark.gpu
+def test_fortran_host_wall_matches_reference_bla():
+    """The host with the lower-wall strip (bla.i wbci = 1, case a of
+    tests/golden/wall_32x33x32.npz: oscillating, suction damped) writes the
+    fields and CFL*dt lines the reference bla wrote."""
+    _need_binary()
+    print("[gpu] blagpu with wbci = 1 32x33x32", flush=True)
+    golden = np.load(os.path.join(ROOT, "tests", "golden", "wall_32x33x32.npz"))
+    work = tempfile.mkdtemp(prefix="blagpu_wall.")
+    try:
+        f = fields.channel_field(32, 33, 32, 4 * math.pi, 2 * math.pi, 3000.0,
+                                 kmax=15, rms=1e-2, seed=5)
+        ufile.write_u(os.path.join(work, "init.u"), f)
+        p = blai.BlaParams(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=30, dt=0.01,
+                           nst=3, xl=4 * math.pi, cflux=True, icfl=3, wbci=1, wamp=0.05,
+                           wdamp=0.5, wxstart=-3.0, wxend=2.0, wxrise=1.0, wxfall=1.5,
+                           wzbet=2.0, wtomeg=20.0, tsave=[0.01], nmsave=["t1.u"])
+        with open(os.path.join(work, "bla.i"), "w") as fh:
+            fh.write(blai.format_channel(p))
+        r = _run(work)
+        assert r.returncode == 0, r.stdout + r.stderr
+        for name, key in (("t1.u", "a_u1"), ("end.u", "a_u10")):
+            u = np.delete(ufile.read_u(os.path.join(work, name)).u, 16, axis=1)
+            rr = np.delete(golden[key], 16, axis=1)
+            for c in range(3):
+                err = _rel(u[c], rr[c])
+                print(f"    {name} comp {c}: rel L2 {err:.2e}", flush=True)
+                assert err <= 1e-12, (name, c, err)
+        cfl = [float(m) for m in re.findall(r"CFL\*dt\s+([0-9.Ee+-]+)", r.stdout)]
+        np.testing.assert_allclose(cfl, golden["a_cfl"], rtol=1e-10)
+    finally:
+        shutil.rmtree(work)
