@@ -147,6 +147,42 @@ __device__ __forceinline__ double2 *fft_t(double2 *src, double2 *dst, int nb, co
     }
 }
 
+// One Stockham pass of radix R (a compile-time constant, so that the R
+// values stay in registers) over nb transforms of length n ([b][n]);
+// twiddles exp(+2 pi i t/tabn) from pl.tw, t = r k tstep < tabn.
+template <int R>
+__device__ __forceinline__ void lds_pass(const double2 *src, double2 *dst, int n, int nb,
+                                         int Ns, const FftPlan &pl, double sign)
+{
+    const int nbf = n / R;
+    const int tstep = pl.tabn / (Ns * R);
+    for (int idx = threadIdx.x; idx < nb * nbf; idx += blockDim.x) {
+        const int b = idx / nbf;
+        const int j = idx - b * nbf;
+        const int k = j % Ns;
+        const double2 *s = src + b * n;
+        double2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            v[r] = s[j + r * nbf];
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                double2 w = pl.tw[r * k * tstep];
+                w.y *= sign;
+                v[r] = cmul(v[r], w);
+            }
+        }
+        dft_small(v, R, sign);
+        double2 *d = dst + b * n;
+        const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            d[base + r * Ns] = v[r];
+    }
+    __syncthreads();
+}
+
 // nb transforms of length n stored contiguously ([b][n]) in src.  Ping-pongs
 // between src and dst; returns the buffer holding the result.  All threads of
 // the block must call it; it ends with a barrier.
@@ -156,30 +192,12 @@ __device__ inline double2 *lds_fft(double2 *src, double2 *dst, int n, int nb,
     int Ns = 1;
     for (int p = 0; p < pl.nrad; p++) {
         const int R = pl.rad[p];
-        const int nbf = n / R;
-        const int tstep = pl.tabn / (Ns * R);
-        for (int idx = threadIdx.x; idx < nb * nbf; idx += blockDim.x) {
-            const int b = idx / nbf;
-            const int j = idx - b * nbf;
-            const int k = j % Ns;
-            const double2 *s = src + b * n;
-            double2 v[5];
-            for (int r = 0; r < R; r++)
-                v[r] = s[j + r * nbf];
-            if (Ns > 1) {
-                for (int r = 1; r < R; r++) {
-                    double2 w = pl.tw[(r * k * tstep) % pl.tabn];
-                    w.y *= sign;
-                    v[r] = cmul(v[r], w);
-                }
-            }
-            dft_small(v, R, sign);
-            double2 *d = dst + b * n;
-            const int base = (j / Ns) * Ns * R + k;
-            for (int r = 0; r < R; r++)
-                d[base + r * Ns] = v[r];
+        switch (R) {
+        case 2: lds_pass<2>(src, dst, n, nb, Ns, pl, sign); break;
+        case 3: lds_pass<3>(src, dst, n, nb, Ns, pl, sign); break;
+        case 4: lds_pass<4>(src, dst, n, nb, Ns, pl, sign); break;
+        default: lds_pass<5>(src, dst, n, nb, Ns, pl, sign); break;
         }
-        __syncthreads();
         double2 *t = src;
         src = dst;
         dst = t;

@@ -60,12 +60,24 @@ __device__ __forceinline__ double sabs_of(const double *s)
                 4. * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
 }
 
+// the plan with its twiddle table copied to LDS at dst (tabn values)
+__device__ FftPlan plan_in_lds(const FftPlan &p, double2 *dst)
+{
+    for (int i = threadIdx.x; i < p.tabn; i += blockDim.x)
+        dst[i] = p.tw[i];
+    __syncthreads();
+    FftPlan q = p;
+    q.tw = dst;
+    return q;
+}
+
 __global__ void __launch_bounds__(kT) k_smag_z(SmagArgs a, FftPlan pz, int B)
 {
     extern __shared__ double2 lds[];
     const int nz = a.nz, nxh = a.nxh, ny = a.ny;
     const size_t F = (size_t)nz * ny * nxh;
     double2 *buf0 = lds, *buf1 = lds + (size_t)B * nz;
+    const FftPlan pl = plan_in_lds(pz, lds + (size_t)2 * B * nz);
     const int f = blockIdx.y, y = blockIdx.z, x0 = blockIdx.x * B;
     const int nb = min(B, nxh - x0);
     for (int idx = threadIdx.x; idx < nz * B; idx += blockDim.x) {
@@ -104,7 +116,7 @@ __global__ void __launch_bounds__(kT) k_smag_z(SmagArgs a, FftPlan pz, int B)
         buf0[b * nz + z] = v;
     }
     __syncthreads();
-    const double2 *res = lds_fft(buf0, buf1, nz, B, pz, +1.0);
+    const double2 *res = lds_fft(buf0, buf1, nz, B, pl, +1.0);
     for (int idx = threadIdx.x; idx < nz * B; idx += blockDim.x) {
         const int z = idx / B, b = idx - z * B;
         if (b < nb)
@@ -145,6 +157,7 @@ __global__ void __launch_bounds__(kT) k_smag_xa(SmagArgs a, FftPlan pm,
     const int M = a.nxh, nz = a.nz, ny = a.ny, nq = a.nq;
     const int NF = a.dynamic ? 21 : 12;      // capacity of each buffer in rows of fields
     double2 *A = lds, *Bf = lds + (size_t)NF * RW * M;
+    const FftPlan pl = plan_in_lds(pm, lds + (size_t)2 * NF * RW * M);
     const int z0 = blockIdx.x * RW, y = blockIdx.y;
     const int nr = min(RW, nz - z0);
     for (int idx = threadIdx.x; idx < 12 * RW * M; idx += blockDim.x) {
@@ -157,7 +170,7 @@ __global__ void __launch_bounds__(kT) k_smag_xa(SmagArgs a, FftPlan pm,
         A[(f * RW + r) * M + k] = Z;
     }
     __syncthreads();
-    double2 *res = lds_fft(A, Bf, M, 12 * RW, pm, +1.0);
+    double2 *res = lds_fft(A, Bf, M, 12 * RW, pl, +1.0);
     double2 *oth = res == A ? Bf : A;
     const double d2 = a.dlt2[y];
     for (int idx = threadIdx.x; idx < RW * M; idx += blockDim.x) {
@@ -212,7 +225,7 @@ __global__ void __launch_bounds__(kT) k_smag_xa(SmagArgs a, FftPlan pm,
     if (!a.dynamic)
         return;
     __syncthreads();
-    double2 *fin = lds_fft(oth, res, M, 21 * RW, pm, -1.0);
+    double2 *fin = lds_fft(oth, res, M, 21 * RW, pl, -1.0);
     for (int idx = threadIdx.x; idx < 21 * RW * nq; idx += blockDim.x) {
         const int f = idx / (RW * nq);
         const int rem = idx - f * RW * nq;
@@ -228,6 +241,7 @@ __global__ void __launch_bounds__(kT) k_smag_zf(SmagArgs a, FftPlan pz, int B)
     extern __shared__ double2 lds[];
     const int nz = a.nz, ny = a.ny, nq = a.nq;
     double2 *buf0 = lds, *buf1 = lds + (size_t)B * nz;
+    const FftPlan pl = plan_in_lds(pz, lds + (size_t)2 * B * nz);
     const int f = blockIdx.y, y = blockIdx.z, x0 = blockIdx.x * B;
     const int nb = min(B, nq - x0);
     double2 *W = a.W + qidx(f, y, 0, x0, ny, nz, nq);
@@ -236,7 +250,7 @@ __global__ void __launch_bounds__(kT) k_smag_zf(SmagArgs a, FftPlan pz, int B)
         buf0[b * nz + z] = b < nb ? W[(size_t)z * nq + b] : make_double2(0., 0.);
     }
     __syncthreads();
-    double2 *sp = lds_fft(buf0, buf1, nz, B, pz, -1.0);
+    double2 *sp = lds_fft(buf0, buf1, nz, B, pl, -1.0);
     for (int idx = threadIdx.x; idx < nz * B; idx += blockDim.x) {
         const int z = idx % nz;
         if (z > nz / 4 && z < nz - nz / 4)
@@ -244,7 +258,7 @@ __global__ void __launch_bounds__(kT) k_smag_zf(SmagArgs a, FftPlan pz, int B)
     }
     __syncthreads();
     double2 *oth = sp == buf0 ? buf1 : buf0;
-    const double2 *res = lds_fft(sp, oth, nz, B, pz, +1.0);
+    const double2 *res = lds_fft(sp, oth, nz, B, pl, +1.0);
     const double sc = (double)a.nx * (double)nz;
     for (int idx = threadIdx.x; idx < nz * B; idx += blockDim.x) {
         const int z = idx / B, b = idx - z * B;
@@ -268,6 +282,7 @@ __global__ void __launch_bounds__(kT) k_smag_xc1(SmagArgs a, FftPlan pm,
     const int nx = 2 * M, tid = threadIdx.x;
     const int z0 = bz * RW, nr = min(RW, nz - z0);
     double2 *A = lds, *Bf = lds + (size_t)21 * RW * M;
+    const FftPlan pl = plan_in_lds(pm, lds + (size_t)2 * 21 * RW * M);
     const int G = kT / M, n = tid % M, grp = tid / M;
     double num[2] = {0., 0.}, den[2] = {0., 0.};
     const double gd2 = a.tw2 * a.dlt2[y];
@@ -281,7 +296,7 @@ __global__ void __launch_bounds__(kT) k_smag_xc1(SmagArgs a, FftPlan pm,
         A[(f * RW + r) * M + k] = Z;
     }
     __syncthreads();
-    const double2 *res = lds_fft(A, Bf, M, 21 * RW, pm, +1.0);
+    const double2 *res = lds_fft(A, Bf, M, 21 * RW, pl, +1.0);
     if (grp < G) {
         for (int r = grp; r < nr; r += G) {
             double2 q[21];
@@ -356,6 +371,7 @@ __global__ void __launch_bounds__(kT) k_smag_xc2(SmagArgs a, FftPlan pm,
     }
     __syncthreads();
     double2 *A = lds, *Bf = lds + (size_t)6 * RT * M;
+    const FftPlan pl = plan_in_lds(pm, lds + (size_t)2 * 6 * RT * M);
     const double gd = 1. / ((double)nx * (double)nz);
     for (int idx = tid; idx < RT * M; idx += blockDim.x) {
         const int r = idx / M, n = idx - r * M;
@@ -383,7 +399,7 @@ __global__ void __launch_bounds__(kT) k_smag_xc2(SmagArgs a, FftPlan pm,
             A[(k * RT + r) * M + n] = t[k];
     }
     __syncthreads();
-    const double2 *fin = lds_fft(A, Bf, M, 6 * RT, pm, -1.0);
+    const double2 *fin = lds_fft(A, Bf, M, 6 * RT, pl, -1.0);
     for (int idx = tid; idx < 6 * RT * M; idx += blockDim.x) {
         const int k = idx / (RT * M);
         const int rem = idx - k * RT * M;
@@ -400,6 +416,7 @@ __global__ void __launch_bounds__(kT) k_smag_za(SmagArgs a, FftPlan pz, int B)
     extern __shared__ double2 lds[];
     const int nz = a.nz, ny = a.ny, nxh = a.nxh;
     double2 *buf0 = lds, *buf1 = lds + (size_t)B * nz;
+    const FftPlan pl = plan_in_lds(pz, lds + (size_t)2 * B * nz);
     const int f = blockIdx.y, y = blockIdx.z, x0 = blockIdx.x * B;
     const int nb = min(B, nxh - x0);
     for (int idx = threadIdx.x; idx < nz * B; idx += blockDim.x) {
@@ -408,7 +425,7 @@ __global__ void __launch_bounds__(kT) k_smag_za(SmagArgs a, FftPlan pz, int B)
             b < nb ? a.Q[qidx(f, y, z, x0 + b, ny, nz, nxh)] : make_double2(0., 0.);
     }
     __syncthreads();
-    const double2 *res = lds_fft(buf0, buf1, nz, B, pz, -1.0);
+    const double2 *res = lds_fft(buf0, buf1, nz, B, pl, -1.0);
     for (int idx = threadIdx.x; idx < nz * B; idx += blockDim.x) {
         const int z = idx / B, b = idx - z * B;
         if (b < nb) {
@@ -425,10 +442,14 @@ int zblock(int nz, int ncol) { return std::max(1, std::min(ncol, kLdsZ / nz)); }
 
 }  // namespace
 
+// rows of z per block: as many as fit in kRowLds of LDS (several blocks
+// per CU), at least one (up to lds_max)
+constexpr size_t kRowLds = 48 * 1024;
+
 int smag_rows(int nxh, int nfield, size_t lds_max)
 {
     const size_t per = (size_t)2 * nfield * nxh * sizeof(double2);
-    return (int)std::max<size_t>(1, lds_max / per);
+    return (int)std::max<size_t>(1, std::min(lds_max, kRowLds) / per);
 }
 
 int smag_zblocks(int nz, int nxh, size_t lds_max)
@@ -440,13 +461,15 @@ int smag_zblocks(int nz, int nxh, size_t lds_max)
 bool smag_fits(int nz, int nxh, size_t lds_max)
 {
     return nz <= kLdsZ && 2 * nxh <= kMaxNx && nxh <= kT &&
-           (size_t)2 * 21 * nxh * sizeof(double2) <= lds_max;
+           (size_t)(2 * 21 * nxh + 2 * nxh) * sizeof(double2) <= lds_max &&
+           (size_t)(2 * kLdsZ + nz) * sizeof(double2) <= lds_max;
 }
 
 int smag_prepare(size_t lds_max)
 {
-    const void *fns[] = {(const void *)k_smag_xa, (const void *)k_smag_xc1,
-                         (const void *)k_smag_xc2};
+    const void *fns[] = {(const void *)k_smag_z,   (const void *)k_smag_xa,
+                         (const void *)k_smag_zf,  (const void *)k_smag_xc1,
+                         (const void *)k_smag_xc2, (const void *)k_smag_za};
     for (const void *fn : fns)
         if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max) !=
             hipSuccess)
@@ -458,25 +481,27 @@ void launch_smag(SmagArgs a, const FftPlan &pz, const FftPlan &pm, const double2
                  size_t lds_max, hipStream_t s)
 {
     const int B = zblock(a.nz, a.nxh);
-    const size_t lz = (size_t)2 * B * a.nz * sizeof(double2);
+    const size_t lz = ((size_t)2 * B * a.nz + a.nz) * sizeof(double2);
+    const size_t twx = (size_t)a.nx * sizeof(double2);
     hipLaunchKernelGGL(k_smag_z, dim3((a.nxh + B - 1) / B, 12, a.ny), dim3(kT), lz, s, a, pz, B);
     const int NF = a.dynamic ? 21 : 12;
     const int RA = std::min(a.nz, smag_rows(a.nxh, NF, lds_max));
     hipLaunchKernelGGL(k_smag_xa, dim3((a.nz + RA - 1) / RA, a.ny), dim3(kT),
-                       (size_t)2 * NF * RA * a.nxh * sizeof(double2), s, a, pm, wx, RA);
+                       (size_t)2 * NF * RA * a.nxh * sizeof(double2) + twx, s, a, pm, wx, RA);
     if (a.dynamic) {
         const int Bq = zblock(a.nz, a.nq);
         hipLaunchKernelGGL(k_smag_zf, dim3((a.nq + Bq - 1) / Bq, 21, a.ny), dim3(kT),
-                           (size_t)2 * Bq * a.nz * sizeof(double2), s, a, pz, Bq);
+                           ((size_t)2 * Bq * a.nz + a.nz) * sizeof(double2), s, a, pz, Bq);
     }
     if (a.dynamic) {
         const int RC = std::min(a.nz, smag_rows(a.nxh, 21, lds_max));
         hipLaunchKernelGGL(k_smag_xc1, dim3((a.nz + RC - 1) / RC, a.ny), dim3(kT),
-                           (size_t)2 * 21 * RC * a.nxh * sizeof(double2), s, a, pm, wx, RC);
+                           (size_t)2 * 21 * RC * a.nxh * sizeof(double2) + twx, s, a, pm, wx,
+                           RC);
     }
     const int RT = std::min(a.nz, smag_rows(a.nxh, 6, lds_max));
     hipLaunchKernelGGL(k_smag_xc2, dim3((a.nz + RT - 1) / RT, a.ny), dim3(kT),
-                       (size_t)2 * 6 * RT * a.nxh * sizeof(double2), s, a, pm, wx, RT,
+                       (size_t)2 * 6 * RT * a.nxh * sizeof(double2) + twx, s, a, pm, wx, RT,
                        smag_zblocks(a.nz, a.nxh, lds_max));
     hipLaunchKernelGGL(k_smag_za, dim3((a.nxh + B - 1) / B, 6, a.ny), dim3(kT), lz, s, a, pz, B);
 }
