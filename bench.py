@@ -174,6 +174,36 @@ def time_form(precision: int, steps: int, warmup: int, model: str = ""):
     return steps / (t1 - t0), t1 - t0, kern, cfl
 
 
+def time_entry_points(reps: int = 5):
+    """Milliseconds per call of the FP64 statistics and pressure entry points
+    on the bench workload (boxamp, boxxys 1-15, nonlinp + linearp), each
+    call synchronising; not part of the step."""
+    import numpy as np
+
+    from simson_amd import fields
+    from simson_amd.gpu import SIMSON_FP64, BlaGPU
+
+    f = fields.channel_field(NX, NY, NZ, XL, ZL, RE, kmax=16, rms=0.1, seed=2)
+    g = BlaGPU(NX, NY, NZ, RE, XL, ZL, cflux=True, precision=SIMSON_FP64)
+    out = {}
+    try:
+        g.upload_velocity(f.u, -2.0 / RE)
+        g.run(1, DT, icfl=0, first=True)
+        acc = np.zeros((15, NY, NX))
+        for name, call in (("amp", g.amp), ("xys", lambda: g.xys(DT, acc)),
+                           ("pressure", g.pressure)):
+            call()                          # first call allocates
+            g.sync()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                call()
+            g.sync()
+            out[name] = round(1e3 * (time.perf_counter() - t0) / reps, 4)
+    finally:
+        g.close()
+    return out
+
+
 def kernel_table(kern: dict, mixed: bool):
     alg = alg_per_launch(mixed)
     flop_peak = FP32_PEAK_TFLOPS if mixed else FP64_PEAK_TFLOPS
@@ -224,6 +254,8 @@ def main():
     ap.add_argument("--models", default="adm-rt,smag,learned",
                     help="subgrid models also timed on the mixed step (comma list of "
                          "adm-rt, smag, learned; empty for none), reported under forms")
+    ap.add_argument("--no-entry-points", dest="entry_points", action="store_false",
+                    help="skip timing the statistics and pressure entry points")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy a device buffer of this size three times "
                          "(a plain copy of known bytes that scales rocprofv3 --pmc "
@@ -246,6 +278,7 @@ def main():
         res["mixed"] = time_form(SIMSON_MIXED, args.steps, args.warmup)
         for m in [m for m in args.models.split(",") if m]:
             res["mixed+" + m] = time_form(SIMSON_MIXED, max(1, args.steps // 2), args.warmup, m)
+    entry = time_entry_points() if args.entry_points else None
     if args.calib_copy_mib > 0:
         import ctypes
         from simson_amd import build as _b
@@ -287,6 +320,7 @@ def main():
         "cfl_dt": cfl,
         "roofline": roofline_of(kernels, kern, mixed),
         "forms": forms,
+        "entry_points_ms": entry,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -21,10 +21,11 @@
 //   k_smag_xc1 per block of z rows: inverse x transform of the filtered
 //              fields, L_ij, M_ij and the partial z sums of M.L and M.M
 //              (les.f:784-880, 1380-1416);
-//   k_smag_xc2 per block of z rows: the clipped coefficient C(x) from the
-//              partial sums, added in a fixed order (les.f:1418-1510), then
-//              nu_t and tau_ij / (nx nz) with the forward x transform
-//              (les.f:1530-1620), into Q as [6][ny][nz][nx/2];
+//   k_smag_c   per y plane: the clipped coefficient C(x) from the partial
+//              sums, added in a fixed order (les.f:1418-1510);
+//   k_smag_xc2 per block of z rows: nu_t and tau_ij / (nx nz) with the
+//              forward x transform (les.f:1530-1620), into Q as
+//              [6][ny][nz][nx/2];
 //   k_smag_za  forward z transform (vcfftf), the oddball row zeroed
 //              (les.f:1630-1645), scaled by nxp nzp to the normalisation of
 //              the step's nonlinear term, into TAU [6][nz][ny][nx/2].
@@ -339,9 +340,31 @@ __global__ void __launch_bounds__(kT) k_smag_xc1(SmagArgs a, FftPlan pm,
     }
 }
 
-// pass 2 (block (bz, y), rows RT bz..): the coefficient C(x, y) from the
-// partial sums (dynamic, clipped by ineg) or cs^2, then nu_t, tau_ij /
-// (nx nz) and the forward x transform into Q as [6][ny][nz][nx/2]
+// the coefficient C(x, y) of the dynamic procedure from the partial sums,
+// added in a fixed order and clipped by ineg (les.f:1418-1510), one block
+// per y plane; written over the first partial row part[y][0][0][x]
+__global__ void __launch_bounds__(kT) k_smag_c(SmagArgs a, int nzb)
+{
+    const int ny = a.ny, nx = 2 * a.nxh, y = blockIdx.x;
+    double *pp = a.part + (size_t)y * nzb * 2 * nx;
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+        double cn = 0., cd = 0.;
+        for (int b = 0; b < nzb; b++) {
+            cn += pp[(size_t)b * 2 * nx + x];
+            cd += pp[(size_t)b * 2 * nx + nx + x];
+        }
+        double c = .5 * cn / cd;
+        if (a.ineg == 1)
+            c = fmax(c, 0.);
+        if (a.ineg != 0 && (y == 0 || y == ny - 1))
+            c = 0.;
+        pp[x] = c;
+    }
+}
+
+// pass 2 (block (bz, y), rows RT bz..): C(x, y) (dynamic, from k_smag_c)
+// or cs^2, then nu_t, tau_ij / (nx nz) and the forward x transform into Q
+// as [6][ny][nz][nx/2]
 __global__ void __launch_bounds__(kT) k_smag_xc2(SmagArgs a, FftPlan pm,
                                                  const double2 *__restrict__ wx, int RT,
                                                  int nzb)
@@ -352,23 +375,8 @@ __global__ void __launch_bounds__(kT) k_smag_xc2(SmagArgs a, FftPlan pm,
     const int nx = 2 * M, tid = threadIdx.x;
     const int z0 = blockIdx.x * RT, nr = min(RT, nz - z0);
     const double d2 = a.dlt2[y];
-    for (int x = tid; x < nx; x += blockDim.x) {
-        double c = a.cs2;
-        if (a.dynamic) {
-            const double *pp = a.part + (size_t)y * nzb * 2 * nx;
-            double cn = 0., cd = 0.;
-            for (int b = 0; b < nzb; b++) {
-                cn += pp[(size_t)b * 2 * nx + x];
-                cd += pp[(size_t)b * 2 * nx + nx + x];
-            }
-            c = .5 * cn / cd;
-            if (a.ineg == 1)
-                c = fmax(c, 0.);
-            if (a.ineg != 0 && (y == 0 || y == ny - 1))
-                c = 0.;
-        }
-        cx[x] = c;
-    }
+    for (int x = tid; x < nx; x += blockDim.x)
+        cx[x] = a.dynamic ? a.part[(size_t)y * nzb * 2 * nx + x] : a.cs2;
     __syncthreads();
     double2 *A = lds, *Bf = lds + (size_t)6 * RT * M;
     const FftPlan pl = plan_in_lds(pm, lds + (size_t)2 * 6 * RT * M);
@@ -499,6 +507,9 @@ void launch_smag(SmagArgs a, const FftPlan &pz, const FftPlan &pm, const double2
                            (size_t)2 * 21 * RC * a.nxh * sizeof(double2) + twx, s, a, pm, wx,
                            RC);
     }
+    if (a.dynamic)
+        hipLaunchKernelGGL(k_smag_c, dim3(a.ny), dim3(kT), 0, s, a,
+                           smag_zblocks(a.nz, a.nxh, lds_max));
     const int RT = std::min(a.nz, smag_rows(a.nxh, 6, lds_max));
     hipLaunchKernelGGL(k_smag_xc2, dim3((a.nz + RT - 1) / RT, a.ny), dim3(kT),
                        (size_t)2 * 6 * RT * a.nxh * sizeof(double2) + twx, s, a, pm, wx, RT,
