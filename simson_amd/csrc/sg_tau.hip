@@ -252,14 +252,16 @@ __device__ __forceinline__ void solve_w(int n, double lam, const double2 (&gp)[2
     }
     wave_sum<SW, (NC + 1) * 2>(sums, l);
 #pragma unroll
-    for (int c = 0; c < 2; c++)
+    for (int c = 0; c < 2; c++) {
+        const double rq = 1.0 / sums[2 * NC + c];
 #pragma unroll
         for (int q = 0; q < NC; q++) {
-            const double a0 = (bc[q][c] - sums[q * 2 + c]) / sums[2 * NC + c];
+            const double a0 = (bc[q][c] - sums[q * 2 + c]) * rq;
 #pragma unroll
             for (int e = 0; e < E; e++)
                 u[q][c][e] = Ap[q][c][e] + Bp[c][e] * a0;
         }
+    }
 }
 
 // Chebyshev-column storage: FP64 (C) or FP32 (C32, the mixed form, whose

@@ -53,6 +53,17 @@ __device__ __forceinline__ double dpp0(double x)
     return dbl(lo, hi);
 }
 
+// value of lane src (any lane of the wave) through the LDS crossbar
+// (ds_bpermute, byte address 4 src): an LDS instruction, not a VALU one.
+// Every lane of the wave must be active.
+__device__ __forceinline__ double bperm(double x, int src)
+{
+    const long long v = bits(x);
+    const int lo = __builtin_amdgcn_ds_bpermute(4 * src, (int)v);
+    const int hi = __builtin_amdgcn_ds_bpermute(4 * src, (int)(v >> 32));
+    return dbl(lo, hi);
+}
+
 __device__ __forceinline__ double readlane(double x, int lane)
 {
     const long long v = bits(x);
@@ -162,15 +173,17 @@ __device__ __forceinline__ void scan_suffix(double (&X)[NX], double &Y, int l)
             combine<NX, HASY>(X, Y, Xo, row == 0 ? Ya : Yb);
         }
     } else if (SW == 64) {
-        // rows 2, 1, 0 in turn take the total of the rows above them
+        // rows 0-2 take the head of the next row (lane 16(r+1)), then rows
+        // 0-1 the head two rows on, which by then composes rows r+2..3
 #pragma unroll
-        for (int r = 2; r >= 0; r--) {
+        for (int d = 1; d <= 2; d *= 2) {
+            const int src = 16 * ((row + d) & 3);
             double Xo[NX];
 #pragma unroll
             for (int q = 0; q < NX; q++)
-                Xo[q] = readlane(X[q], 16 * (r + 1));
-            const double Yo = HASY ? readlane(Y, 16 * (r + 1)) : 1.0;
-            if (row == r)
+                Xo[q] = bperm(X[q], src);
+            const double Yo = HASY ? bperm(Y, src) : 1.0;
+            if (row + d < 4)
                 combine<NX, HASY>(X, Y, Xo, Yo);
         }
     }
@@ -202,6 +215,18 @@ __device__ __forceinline__ void sum_all(double (&X)[NX], int l)
     dpp_step<0x118, NX, false>(X, one);
     // lane 16 r + 15 holds the total of row r
     const int row = l >> 4;
+    if (SW == 64) {
+        // every lane fetches its row's total, adds the partner row's (lane
+        // xor 16), then the other half's (lane xor 32): (r0 + r1) + (r2 + r3)
+        // on every lane, the operands of each addition only commuted
+#pragma unroll
+        for (int q = 0; q < NX; q++) {
+            const double t = bperm(X[q], 16 * row + 15);
+            const double p = t + bperm(t, l ^ 16);
+            X[q] = p + bperm(p, l ^ 32);
+        }
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < NX; q++) {
         const double r0 = readlane(X[q], 15), r1 = readlane(X[q], 31);
