@@ -13,8 +13,10 @@
 //     of affine maps, the forward substitution a_k = rho_k - gamma_k a_{k-2}
 //     a prefix scan of affine maps, and the boundary row a sum;
 //   * parity sums (dv/dy at the wall, the mass flux) are wave reductions.
-// The scans run on DPP lane moves within 16-lane rows and row broadcasts or
-// lane reads across rows (sg_wave.h), so no step waits on an LDS permute.
+// The scans run on DPP lane moves within 16-lane rows.  Across the four rows
+// of a 64-lane segment, prefix scans use DPP row broadcasts; suffix scans
+// and sums take ds_bpermute lane reads, which go through the LDS pipe and so
+// leave the VALU, this kernel's bound, to the arithmetic (sg_wave.h).
 #include <cstdlib>
 
 #include "sg_internal.h"

@@ -1,6 +1,10 @@
 // Lane-segment data movement for 64-lane waves on gfx950, built on DPP
 // (data-parallel primitives: a VALU operand modifier that reads another
-// lane's register in the same instruction) instead of LDS permutes.
+// lane's register in the same instruction) within 16-lane rows and, for
+// prefix scans, the row broadcasts.  Suffix scans and sums across the rows
+// of a 64-lane segment read other lanes with ds_bpermute, an LDS-pipe
+// instruction, rather than with v_readlane, which occupies the VALU like an
+// arithmetic instruction.
 //
 // A wave is split into segments of SW = 16, 32 or 64 lanes (one column of
 // the column solver per segment); every scan, shift and sum below stays
