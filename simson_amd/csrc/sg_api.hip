@@ -483,7 +483,7 @@ void smag_stresses(simson_gpu_t *g)
     a.U = g->U;
     a.u0low = g->u0low;
     a.w0low = g->w0low;
-    timed(g, F_SMAG, [&] { launch_smag(a, g->pzs, g->pms, g->twxs, g->smag_lds, g->stream); });
+    timed(g, F_SMAG, [&] { launch_smag(a, g->pzs, g->pms, g->smag_lds, g->stream); });
     if (g->mixed) {
         launch_narrow(g->TAU, g->TAU32, 6 * field_size(g), g->stream);
         GemmIO io = gemm_io(g, g->TAU32, 0, g->DT32, 0, {1, 3, 4}, {0, 1, 2}, g->nz / 2);

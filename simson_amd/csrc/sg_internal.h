@@ -198,8 +198,10 @@ struct SmagArgs {
 bool smag_fits(int nz, int nxh, size_t lds_max);
 int smag_zblocks(int nz, int nxh, size_t lds_max);
 int smag_prepare(size_t lds_max);
-void launch_smag(SmagArgs a, const FftPlan &pz, const FftPlan &pm, const double2 *wx,
-                 size_t lds_max, hipStream_t s);
+// pm's twiddle table exp(+2 pi i t / nx), t < nx, also serves the packing of
+// the real x transforms
+void launch_smag(SmagArgs a, const FftPlan &pz, const FftPlan &pm, size_t lds_max,
+                 hipStream_t s);
 bool xys_fits(int nz, int nxh);
 void launch_xys(const XysArgs &a, const FftPlan &pz, const FftPlan &pm, const double2 *wx,
                 hipStream_t s);

@@ -43,6 +43,7 @@ namespace {
 constexpr int kT = 256;
 constexpr int kLdsZ = 2048;        // complex values per LDS buffer of the z kernels
 constexpr int kMaxNx = 512;
+constexpr int kIB = 4;             // items per thread whose global loads issue together
 
 __device__ __forceinline__ size_t qidx(int f, int y, int z, int x, int ny, int nz, int nx)
 {
@@ -81,40 +82,69 @@ __global__ void __launch_bounds__(kT) k_smag_z(SmagArgs a, FftPlan pz, int B)
     const FftPlan pl = plan_in_lds(pz, lds + (size_t)2 * B * nz);
     const int f = blockIdx.y, y = blockIdx.z, x0 = blockIdx.x * B;
     const int nb = min(B, nxh - x0);
-    for (int idx = threadIdx.x; idx < nz * B; idx += blockDim.x) {
-        const int z = idx / B, b = idx - z * B;
-        double2 v = make_double2(0., 0.);
-        if (b < nb && z != nz / 2) {
-            const int x = x0 + b;
+    // the fields this block reads: fa, and fb for the y derivatives of u and
+    // w (p1 is loaded from fa again where it is not needed)
+    int fa = f, fb = f;
+    if (f >= 3) {
+        const int i = (f - 3) / 3, j = (f - 3) - 3 * i;
+        fa = fb = i;
+        if (j == 1) {
+            fa = i == 0 ? 1 : (i == 1 ? 0 : 3);
+            fb = i == 0 ? 4 : (i == 1 ? 2 : 1);
+        }
+    }
+    const int tot = nz * B;
+    for (int base = threadIdx.x; base < tot; base += kIB * kT) {
+        double2 p0[kIB], p1[kIB];
+        double al[kIB], be[kIB];
+#pragma unroll
+        for (int i = 0; i < kIB; i++) {
+            const int idx = min(base + i * kT, tot - 1);
+            const int z = idx / B, x = min(x0 + idx - z * B, nxh - 1);
             const size_t o = sidx(0, z, y, x, nz, ny, nxh);
-            const double al = a.alfa[x], be = a.beta[z];
-            if (f < 3) {
-                v = a.U[f * F + o];
-                if (x == 0 && z == 0) {
-                    if (f == 0)
-                        v.x -= a.u0low;
-                    if (f == 2)
-                        v.x -= a.w0low;
-                }
-            } else {
-                const int i = (f - 3) / 3, j = (f - 3) - 3 * i;
-                if (j != 1) {
-                    const double k = j == 0 ? al : be;
-                    const double2 w = a.U[i * F + o];
-                    v = make_double2(-k * w.y, k * w.x);
-                } else if (i == 0) {        // du/dy = i alpha v - om_z
-                    const double2 vv = a.U[F + o], oz = a.U[4 * F + o];
-                    v = make_double2(-al * vv.y - oz.x, al * vv.x - oz.y);
-                } else if (i == 1) {        // dv/dy = -i alpha u - i beta w
-                    const double2 uu = a.U[o], ww = a.U[2 * F + o];
-                    v = make_double2(al * uu.y + be * ww.y, -al * uu.x - be * ww.x);
-                } else {                    // dw/dy = om_x + i beta v
-                    const double2 ox = a.U[3 * F + o], vv = a.U[F + o];
-                    v = make_double2(ox.x - be * vv.y, ox.y + be * vv.x);
+            p0[i] = a.U[fa * F + o];
+            p1[i] = a.U[fb * F + o];
+            al[i] = a.alfa[x];
+            be[i] = a.beta[z];
+        }
+#pragma unroll
+        for (int i = 0; i < kIB; i++) {
+            const int idx = base + i * kT;
+            if (idx >= tot)
+                continue;
+            const int z = idx / B, b = idx - z * B;
+            double2 v = make_double2(0., 0.);
+            if (b < nb && z != nz / 2) {
+                const int x = x0 + b;
+                if (f < 3) {
+                    v = p0[i];
+                    if (x == 0 && z == 0) {
+                        if (f == 0)
+                            v.x -= a.u0low;
+                        if (f == 2)
+                            v.x -= a.w0low;
+                    }
+                } else {
+                    const int ii = (f - 3) / 3, j = (f - 3) - 3 * ii;
+                    if (j != 1) {
+                        const double k = j == 0 ? al[i] : be[i];
+                        const double2 w = p0[i];
+                        v = make_double2(-k * w.y, k * w.x);
+                    } else if (ii == 0) {       // du/dy = i alpha v - om_z
+                        const double2 vv = p0[i], oz = p1[i];
+                        v = make_double2(-al[i] * vv.y - oz.x, al[i] * vv.x - oz.y);
+                    } else if (ii == 1) {       // dv/dy = -i alpha u - i beta w
+                        const double2 uu = p0[i], ww = p1[i];
+                        v = make_double2(al[i] * uu.y + be[i] * ww.y,
+                                         -al[i] * uu.x - be[i] * ww.x);
+                    } else {                    // dw/dy = om_x + i beta v
+                        const double2 ox = p0[i], vv = p1[i];
+                        v = make_double2(ox.x - be[i] * vv.y, ox.y + be[i] * vv.x);
+                    }
                 }
             }
+            buf0[b * nz + z] = v;
         }
-        buf0[b * nz + z] = v;
     }
     __syncthreads();
     const double2 *res = lds_fft(buf0, buf1, nz, B, pl, +1.0);
@@ -126,17 +156,57 @@ __global__ void __launch_bounds__(kT) k_smag_z(SmagArgs a, FftPlan pz, int B)
 }
 
 // half spectrum row (k < nk given, zero above) -> packed Z_k for the
-// inverse real transform of length 2M (only Re X_0, no Nyquist term)
-__device__ __forceinline__ double2 pack_row(const double2 *row, int k, int M, int nk,
-                                            const double2 *wx)
+// inverse real transform of length 2M (only Re X_0, no Nyquist term).
+// pack_load reads X_k and X_{M-k} through clamped addresses (so that the
+// loads of several items can issue before any is used); pack_vals forms Z_k.
+__device__ __forceinline__ void pack_load(const double2 *row, int k, int M, int nk,
+                                          double2 &Xk, double2 &Xm)
 {
-    double2 Xk = k < nk ? row[k] : make_double2(0., 0.);
+    const int km = M - k;
+    const double2 p = row[min(k, nk - 1)], q = row[min(km, nk - 1)];
+    const double2 z = make_double2(0., 0.);
+    Xk = k < nk ? p : z;
+    Xm = (k > 0 && km < nk) ? q : z;
+}
+
+__device__ __forceinline__ double2 pack_vals(double2 Xk, double2 Xm, int k, const double2 *wx)
+{
     if (k == 0)
         Xk.y = 0.;
-    const int km = M - k;
-    double2 Xm = (k > 0 && km < nk) ? row[km] : make_double2(0., 0.);
     Xm.y = -Xm.y;
     return cadd(cadd(Xk, Xm), mul_is(cmul(wx[k], csub(Xk, Xm)), 1.0));
+}
+
+// NF packed rows [f][RW][M] of a [f][ny][nz][nk] spectrum into A (rows
+// r >= nr zero); the global loads of kIB items per thread issue together
+__device__ __forceinline__ void pack_rows(double2 *A, const double2 *src, int NF, int RW, int nr,
+                                          int y, int z0, int ny, int nz, int M, int nk,
+                                          const double2 *wx)
+{
+    const int tot = NF * RW * M;
+    for (int base = threadIdx.x; base < tot; base += kIB * kT) {
+        double2 xk[kIB], xm[kIB];
+#pragma unroll
+        for (int i = 0; i < kIB; i++) {
+            const int idx = min(base + i * kT, tot - 1);
+            const int f = idx / (RW * M);
+            const int rem = idx - f * RW * M;
+            const int r = rem / M, k = rem - r * M;
+            pack_load(src + qidx(f, y, min(z0 + r, nz - 1), 0, ny, nz, nk), k, M, nk, xk[i],
+                      xm[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < kIB; i++) {
+            const int idx = base + i * kT;
+            if (idx < tot) {
+                const int f = idx / (RW * M);
+                const int rem = idx - f * RW * M;
+                const int r = rem / M, k = rem - r * M;
+                A[(f * RW + r) * M + k] =
+                    r < nr ? pack_vals(xk[i], xm[i], k, wx) : make_double2(0., 0.);
+            }
+        }
+    }
 }
 
 // packed forward transform result -> X_k, k < M
@@ -151,25 +221,18 @@ __device__ __forceinline__ double2 unpack_row(const double2 *fin, int k, int M,
     return cadd(E, cmul(conjd(wx[k]), O));
 }
 
-__global__ void __launch_bounds__(kT) k_smag_xa(SmagArgs a, FftPlan pm,
-                                                const double2 *__restrict__ wx, int RW)
+__global__ void __launch_bounds__(kT) k_smag_xa(SmagArgs a, FftPlan pm, int RW)
 {
     extern __shared__ double2 lds[];
     const int M = a.nxh, nz = a.nz, ny = a.ny, nq = a.nq;
     const int NF = a.dynamic ? 21 : 12;      // capacity of each buffer in rows of fields
     double2 *A = lds, *Bf = lds + (size_t)NF * RW * M;
+    // the x twiddles exp(+2 pi i t / nx) of the plan are also the half-angle
+    // factors of the real-transform packing; both are read from LDS
     const FftPlan pl = plan_in_lds(pm, lds + (size_t)2 * NF * RW * M);
     const int z0 = blockIdx.x * RW, y = blockIdx.y;
     const int nr = min(RW, nz - z0);
-    for (int idx = threadIdx.x; idx < 12 * RW * M; idx += blockDim.x) {
-        const int f = idx / (RW * M);
-        const int rem = idx - f * RW * M;
-        const int r = rem / M, k = rem - r * M;
-        double2 Z = make_double2(0., 0.);
-        if (r < nr)
-            Z = pack_row(a.Q + qidx(f, y, z0 + r, 0, ny, nz, M), k, M, M, wx);
-        A[(f * RW + r) * M + k] = Z;
-    }
+    pack_rows(A, a.Q, 12, RW, nr, y, z0, ny, nz, M, M, pl.tw);
     __syncthreads();
     double2 *res = lds_fft(A, Bf, M, 12 * RW, pl, +1.0);
     double2 *oth = res == A ? Bf : A;
@@ -232,7 +295,8 @@ __global__ void __launch_bounds__(kT) k_smag_xa(SmagArgs a, FftPlan pm,
         const int rem = idx - f * RW * nq;
         const int r = rem / nq, k = rem - r * nq;
         if (r < nr)
-            a.W[qidx(f, y, z0 + r, k, ny, nz, nq)] = unpack_row(fin + (f * RW + r) * M, k, M, wx);
+            a.W[qidx(f, y, z0 + r, k, ny, nz, nq)] =
+                unpack_row(fin + (f * RW + r) * M, k, M, pl.tw);
     }
 }
 
@@ -246,9 +310,23 @@ __global__ void __launch_bounds__(kT) k_smag_zf(SmagArgs a, FftPlan pz, int B)
     const int f = blockIdx.y, y = blockIdx.z, x0 = blockIdx.x * B;
     const int nb = min(B, nq - x0);
     double2 *W = a.W + qidx(f, y, 0, x0, ny, nz, nq);
-    for (int idx = threadIdx.x; idx < nz * B; idx += blockDim.x) {
-        const int z = idx / B, b = idx - z * B;
-        buf0[b * nz + z] = b < nb ? W[(size_t)z * nq + b] : make_double2(0., 0.);
+    const int tot = nz * B;
+    for (int base = threadIdx.x; base < tot; base += kIB * kT) {
+        double2 v[kIB];
+#pragma unroll
+        for (int i = 0; i < kIB; i++) {
+            const int idx = min(base + i * kT, tot - 1);
+            const int z = idx / B, b = idx - z * B;
+            v[i] = W[(size_t)z * nq + min(b, nb - 1)];
+        }
+#pragma unroll
+        for (int i = 0; i < kIB; i++) {
+            const int idx = base + i * kT;
+            if (idx < tot) {
+                const int z = idx / B, b = idx - z * B;
+                buf0[b * nz + z] = b < nb ? v[i] : make_double2(0., 0.);
+            }
+        }
     }
     __syncthreads();
     double2 *sp = lds_fft(buf0, buf1, nz, B, pl, -1.0);
@@ -273,8 +351,7 @@ __global__ void __launch_bounds__(kT) k_smag_zf(SmagArgs a, FftPlan pz, int B)
 // dynamic procedure, pass 1: the RW rows z0.. of plane y (block (bz, y),
 // z0 = RW bz): inverse x transform of the filtered fields, L_ij, M_ij, and
 // the partial z sums of M.L and M.M per x point into part [ny][nzb][2][nx]
-__global__ void __launch_bounds__(kT) k_smag_xc1(SmagArgs a, FftPlan pm,
-                                                 const double2 *__restrict__ wx, int RW)
+__global__ void __launch_bounds__(kT) k_smag_xc1(SmagArgs a, FftPlan pm, int RW)
 {
     extern __shared__ double2 lds[];
     __shared__ double red[2 * 2 * kT];
@@ -287,13 +364,20 @@ __global__ void __launch_bounds__(kT) k_smag_xc1(SmagArgs a, FftPlan pm,
     const int G = kT / M, n = tid % M, grp = tid / M;
     double num[2] = {0., 0.}, den[2] = {0., 0.};
     const double gd2 = a.tw2 * a.dlt2[y];
+    // the filtered rows keep only kx < nq: one item at a time, with no load
+    // for the modes above
     for (int idx = tid; idx < 21 * RW * M; idx += blockDim.x) {
         const int f = idx / (RW * M);
         const int rem = idx - f * RW * M;
         const int r = rem / M, k = rem - r * M;
         double2 Z = make_double2(0., 0.);
-        if (r < nr)
-            Z = pack_row(a.W + qidx(f, y, z0 + r, 0, ny, nz, nq), k, M, nq, wx);
+        if (r < nr) {
+            const double2 *row = a.W + qidx(f, y, z0 + r, 0, ny, nz, nq);
+            const int km = M - k;
+            const double2 Xk = k < nq ? row[k] : make_double2(0., 0.);
+            const double2 Xm = (k > 0 && km < nq) ? row[km] : make_double2(0., 0.);
+            Z = pack_vals(Xk, Xm, k, pl.tw);
+        }
         A[(f * RW + r) * M + k] = Z;
     }
     __syncthreads();
@@ -365,9 +449,7 @@ __global__ void __launch_bounds__(kT) k_smag_c(SmagArgs a, int nzb)
 // pass 2 (block (bz, y), rows RT bz..): C(x, y) (dynamic, from k_smag_c)
 // or cs^2, then nu_t, tau_ij / (nx nz) and the forward x transform into Q
 // as [6][ny][nz][nx/2]
-__global__ void __launch_bounds__(kT) k_smag_xc2(SmagArgs a, FftPlan pm,
-                                                 const double2 *__restrict__ wx, int RT,
-                                                 int nzb)
+__global__ void __launch_bounds__(kT) k_smag_xc2(SmagArgs a, FftPlan pm, int RT, int nzb)
 {
     extern __shared__ double2 lds[];
     __shared__ double cx[kMaxNx];
@@ -413,7 +495,8 @@ __global__ void __launch_bounds__(kT) k_smag_xc2(SmagArgs a, FftPlan pm,
         const int rem = idx - k * RT * M;
         const int r = rem / M, kx = rem - r * M;
         if (r < nr)
-            a.Q[qidx(k, y, z0 + r, kx, ny, nz, M)] = unpack_row(fin + (k * RT + r) * M, kx, M, wx);
+            a.Q[qidx(k, y, z0 + r, kx, ny, nz, M)] =
+                unpack_row(fin + (k * RT + r) * M, kx, M, pl.tw);
     }
 }
 
@@ -427,10 +510,23 @@ __global__ void __launch_bounds__(kT) k_smag_za(SmagArgs a, FftPlan pz, int B)
     const FftPlan pl = plan_in_lds(pz, lds + (size_t)2 * B * nz);
     const int f = blockIdx.y, y = blockIdx.z, x0 = blockIdx.x * B;
     const int nb = min(B, nxh - x0);
-    for (int idx = threadIdx.x; idx < nz * B; idx += blockDim.x) {
-        const int z = idx / B, b = idx - z * B;
-        buf0[b * nz + z] =
-            b < nb ? a.Q[qidx(f, y, z, x0 + b, ny, nz, nxh)] : make_double2(0., 0.);
+    const int tot = nz * B;
+    for (int base = threadIdx.x; base < tot; base += kIB * kT) {
+        double2 v[kIB];
+#pragma unroll
+        for (int i = 0; i < kIB; i++) {
+            const int idx = min(base + i * kT, tot - 1);
+            const int z = idx / B, b = idx - z * B;
+            v[i] = a.Q[qidx(f, y, z, x0 + min(b, nb - 1), ny, nz, nxh)];
+        }
+#pragma unroll
+        for (int i = 0; i < kIB; i++) {
+            const int idx = base + i * kT;
+            if (idx < tot) {
+                const int z = idx / B, b = idx - z * B;
+                buf0[b * nz + z] = b < nb ? v[i] : make_double2(0., 0.);
+            }
+        }
     }
     __syncthreads();
     const double2 *res = lds_fft(buf0, buf1, nz, B, pl, -1.0);
@@ -485,7 +581,7 @@ int smag_prepare(size_t lds_max)
     return 0;
 }
 
-void launch_smag(SmagArgs a, const FftPlan &pz, const FftPlan &pm, const double2 *wx,
+void launch_smag(SmagArgs a, const FftPlan &pz, const FftPlan &pm,
                  size_t lds_max, hipStream_t s)
 {
     const int B = zblock(a.nz, a.nxh);
@@ -495,7 +591,7 @@ void launch_smag(SmagArgs a, const FftPlan &pz, const FftPlan &pm, const double2
     const int NF = a.dynamic ? 21 : 12;
     const int RA = std::min(a.nz, smag_rows(a.nxh, NF, lds_max));
     hipLaunchKernelGGL(k_smag_xa, dim3((a.nz + RA - 1) / RA, a.ny), dim3(kT),
-                       (size_t)2 * NF * RA * a.nxh * sizeof(double2) + twx, s, a, pm, wx, RA);
+                       (size_t)2 * NF * RA * a.nxh * sizeof(double2) + twx, s, a, pm, RA);
     if (a.dynamic) {
         const int Bq = zblock(a.nz, a.nq);
         hipLaunchKernelGGL(k_smag_zf, dim3((a.nq + Bq - 1) / Bq, 21, a.ny), dim3(kT),
@@ -504,7 +600,7 @@ void launch_smag(SmagArgs a, const FftPlan &pz, const FftPlan &pm, const double2
     if (a.dynamic) {
         const int RC = std::min(a.nz, smag_rows(a.nxh, 21, lds_max));
         hipLaunchKernelGGL(k_smag_xc1, dim3((a.nz + RC - 1) / RC, a.ny), dim3(kT),
-                           (size_t)2 * 21 * RC * a.nxh * sizeof(double2) + twx, s, a, pm, wx,
+                           (size_t)2 * 21 * RC * a.nxh * sizeof(double2) + twx, s, a, pm,
                            RC);
     }
     if (a.dynamic)
@@ -512,7 +608,7 @@ void launch_smag(SmagArgs a, const FftPlan &pz, const FftPlan &pm, const double2
                            smag_zblocks(a.nz, a.nxh, lds_max));
     const int RT = std::min(a.nz, smag_rows(a.nxh, 6, lds_max));
     hipLaunchKernelGGL(k_smag_xc2, dim3((a.nz + RT - 1) / RT, a.ny), dim3(kT),
-                       (size_t)2 * 6 * RT * a.nxh * sizeof(double2) + twx, s, a, pm, wx, RT,
+                       (size_t)2 * 6 * RT * a.nxh * sizeof(double2) + twx, s, a, pm, RT,
                        smag_zblocks(a.nz, a.nxh, lds_max));
     hipLaunchKernelGGL(k_smag_za, dim3((a.nxh + B - 1) / B, 6, a.ny), dim3(kT), lz, s, a, pz, B);
 }
