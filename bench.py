@@ -262,7 +262,7 @@ def main():
                          "FETCH_SIZE/WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the library runs the step on one GPU; --gpus {args.gpus} is not supported "
               "(the step is not sharded across GPUs).", file=sys.stderr)
         sys.exit(2)
 

@@ -1,5 +1,5 @@
 """A-posteriori: LES runs on the coarse grid with no model, with ADM-RT (the
-reference's own closure, the judge) and with the learned closure, against
+reference's own closure, the yardstick) and with the learned closure, against
 the filtered DNS of this solver.
 
 The DNS is the 128x97x128 channel (Re_cl 4200) from the seeded 10%

@@ -2,7 +2,7 @@
 
 Tolerances, all FP64:
 * velocities: <= 1e-12 relative L2 per component against reference bla
-  after 100 steps (the north-star bar), and per step against the oracle;
+  after 100 steps (the FP64 parity bar), and per step against the oracle;
 * one half-step from identical inputs: step 2 (H) 1e-13; step 3
   velocities 1e-12: the Helmholtz tau systems for lam = k^2 + 2 Re/(a+b)
   ~ 1e6 have condition numbers ~ ny^2 (a few), so two FP64 codes that round

@@ -49,7 +49,7 @@ def test_oracle_matches_reference_bla(golden_c1, nsteps, key):
     assert st == 0
     u = pyoracle.velocities(ur, ui)
     ref = golden_c1[key]
-    # <= 1e-12 relative L2 in every velocity component
+    # the FP64 parity bar: <= 1e-12 relative L2 in every velocity component
     errs = rel_l2(u, ref)
     assert max(errs) < 1e-12, errs
     # the z-oddball plane holds the unnormalised nonlinear term in the
