@@ -76,7 +76,7 @@ def alg_per_launch(mixed: bool):
 
 def traffic_file(mixed: bool) -> str:
     form = "mixed" if mixed else "fp64"
-    return os.path.join(HERE, "profiles", f"hbm_traffic_c2_128x97x128_{form}.md")
+    return os.path.join(HERE, "profiles", f"hbm_traffic_step_128x97x128_{form}.md")
 
 
 def profiled_traffic(family: str, mixed: bool):
