@@ -176,16 +176,19 @@ __device__ __forceinline__ void solve_w(int n, double lam, const double2 (&gp)[2
         for (int q = 0; q < NC; q++)
             X[q][c] = wv::from_next<SW>(A[q][c][0], 0.0, l);
     }
+    {
+        double Xc[2][NC];
 #pragma unroll
-    for (int c = 0; c < 2; c++) {
-        double Xc[NC];
+        for (int c = 0; c < 2; c++)
 #pragma unroll
-        for (int q = 0; q < NC; q++)
-            Xc[q] = X[q][c];
-        wv::scan_suffix<SW, NC, true>(Xc, Y[c], l);
+            for (int q = 0; q < NC; q++)
+                Xc[c][q] = X[q][c];
+        wv::scan_suffix2<SW, NC>(Xc, Y, l);
 #pragma unroll
-        for (int q = 0; q < NC; q++)
-            X[q][c] = Xc[q];
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int q = 0; q < NC; q++)
+                X[q][c] = Xc[c][q];
     }
     double rho[NC][2][E];
 #pragma unroll

@@ -193,6 +193,44 @@ __device__ __forceinline__ void scan_suffix(double (&X)[NX], double &Y, int l)
     }
 }
 
+// two independent affine suffix scans (classes c = 0, 1 with their own Y)
+// in one pass, so that the lane reads of both issue before either is used;
+// each class gets exactly the operations of scan_suffix
+template <int SW, int NX>
+__device__ __forceinline__ void scan_suffix2(double (&X)[2][NX], double (&Y)[2], int l)
+{
+    if (SW != kWave) {
+        scan_suffix<SW, NX, true>(X[0], Y[0], l);
+        scan_suffix<SW, NX, true>(X[1], Y[1], l);
+        return;
+    }
+    dpp_step<0x101, NX, true>(X[0], Y[0]);
+    dpp_step<0x101, NX, true>(X[1], Y[1]);
+    dpp_step<0x102, NX, true>(X[0], Y[0]);
+    dpp_step<0x102, NX, true>(X[1], Y[1]);
+    dpp_step<0x104, NX, true>(X[0], Y[0]);
+    dpp_step<0x104, NX, true>(X[1], Y[1]);
+    dpp_step<0x108, NX, true>(X[0], Y[0]);
+    dpp_step<0x108, NX, true>(X[1], Y[1]);
+    const int row = l >> 4;
+#pragma unroll
+    for (int d = 1; d <= 2; d *= 2) {
+        const int src = 16 * ((row + d) & 3);
+        double Xo[2][NX], Yo[2];
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+#pragma unroll
+            for (int q = 0; q < NX; q++)
+                Xo[c][q] = bperm(X[c][q], src);
+            Yo[c] = bperm(Y[c], src);
+        }
+        if (row + d < 4) {
+            combine<NX, true>(X[0], Y[0], Xo[0], Yo[0]);
+            combine<NX, true>(X[1], Y[1], Xo[1], Yo[1]);
+        }
+    }
+}
+
 // sum over each segment of NX channels, the same value on every lane of
 // the segment
 template <int SW, int NX>
@@ -222,13 +260,24 @@ __device__ __forceinline__ void sum_all(double (&X)[NX], int l)
     if (SW == 64) {
         // every lane fetches its row's total, adds the partner row's (lane
         // xor 16), then the other half's (lane xor 32): (r0 + r1) + (r2 + r3)
-        // on every lane, the operands of each addition only commuted
+        // on every lane, the operands of each addition only commuted.  Each
+        // round reads all channels before any is used.
+        double t[NX];
 #pragma unroll
-        for (int q = 0; q < NX; q++) {
-            const double t = bperm(X[q], 16 * row + 15);
-            const double p = t + bperm(t, l ^ 16);
-            X[q] = p + bperm(p, l ^ 32);
-        }
+        for (int q = 0; q < NX; q++)
+            t[q] = bperm(X[q], 16 * row + 15);
+#pragma unroll
+        for (int q = 0; q < NX; q++)
+            X[q] = bperm(t[q], l ^ 16);
+#pragma unroll
+        for (int q = 0; q < NX; q++)
+            t[q] = t[q] + X[q];
+#pragma unroll
+        for (int q = 0; q < NX; q++)
+            X[q] = bperm(t[q], l ^ 32);
+#pragma unroll
+        for (int q = 0; q < NX; q++)
+            X[q] = t[q] + X[q];
         return;
     }
 #pragma unroll
