@@ -1303,215 +1303,23 @@ void orc_pressure(orc_t *o, const double *ur, const double *ui, double *pr, doub
 /* relaxation term of nonlinbl.f:920-966                               */
 /* ------------------------------------------------------------------ */
 
-/* interior five-point filter at offsets (-dmm, -dm, 0, dp, dpp): explicit
- * weights ex[5], implicit im[5] (coeff_filter_centered, i3mom = 0,
- * init_filter.f:399-498) */
-static void les_centered(double dmm, double dm, double dp, double dpp, double cutoff,
-                         double *ex, double *im)
-{
-    double s3 = 2.*dpp*pow(dmm,3)*dm*dm + 2.*dpp*dpp*dmm*dmm*dm*dm - 2.*dpp*pow(dmm,4)*dm
-              - 2.*dpp*dpp*pow(dmm,3)*dm + 2.*dpp*pow(dmm,3)*dp*dp + 2.*dpp*dpp*dmm*dmm*dp*dp
-              + 2.*dpp*pow(dmm,4)*dp + 2.*dpp*dpp*pow(dmm,3)*dp + pow(dp,3)*dm*dm*dmm
-              - dp*pow(dm,4)*dmm + 2.*dp*dp*dm*dm*dmm*dmm + dp*pow(dm,3)*dmm*dmm
-              - pow(dp,3)*dm*dm*dpp + dp*pow(dm,4)*dpp + 2.*dp*dp*dm*dm*dpp*dpp
-              + dp*pow(dm,3)*dpp*dpp + dm*pow(dp,4)*dmm - pow(dm,3)*dp*dp*dmm
-              + dm*pow(dp,3)*dmm*dmm - dm*pow(dp,4)*dpp + pow(dm,3)*dp*dp*dpp
-              + dm*pow(dp,3)*dpp*dpp;
-    double s4 = 2.*dp*pow(dpp,4)*dm + 2.*pow(dmm,3)*dp*dp*dm - 2.*dp*dp*dm*dm*dmm*dmm
-              - 2.*dmm*pow(dm,4)*dpp + 2.*dpp*dpp*dmm*dmm*dm*dm - 2.*dmm*dmm*pow(dp,3)*dpp
-              + 2.*pow(dmm,4)*dp*dm - 2.*dmm*pow(dm,3)*dpp*dpp + 2.*dmm*pow(dp,3)*dpp*dpp
-              - 2.*pow(dmm,3)*dm*dm*dp + 2.*dmm*dmm*pow(dm,3)*dpp + 2.*pow(dpp,3)*dm*dm*dp
-              - 2.*pow(dpp,3)*dp*dp*dm + 2.*dpp*dpp*dmm*dmm*dp*dp - 2.*dmm*pow(dp,4)*dpp
-              + 4.*dpp*dpp*pow(dmm,4) - 2.*dp*dp*dm*dm*dpp*dpp + dmm*dmm*pow(dp,4)
-              + dm*dm*pow(dpp,4) - 2.*pow(dmm,3)*pow(dm,3) + dpp*dpp*pow(dp,4)
-              + 4.*pow(dpp,4)*dmm*dmm + dmm*dmm*pow(dm,4) + dpp*dpp*pow(dm,4)
-              + 8.*pow(dpp,3)*pow(dmm,3) + 2.*pow(dpp,3)*pow(dm,3) + pow(dmm,4)*dm*dm
-              + 2.*pow(dmm,3)*pow(dp,3) - 2.*pow(dpp,3)*pow(dp,3) + pow(dmm,4)*dp*dp
-              + dp*dp*pow(dpp,4) + 4.*dmm*dp*dp*dpp*dm*dm + 2.*dmm*dp*dp*dm*dpp*dpp
-              - 2.*dmm*dm*dm*dp*dpp*dpp + 2.*dmm*dmm*dp*dpp*dm*dm + 4.*dmm*dmm*dp*dm*dpp*dpp
-              - 2.*dmm*dmm*dm*dpp*dp*dp;
-    double appx = -0.5 * s3 / s4;
-    double apx = -0.5 * (dm*dm + 2.*dmm*appx*dpp - dmm*dm + 2.*appx*dpp*dpp) /
-                 (dp*dp - dm*dm + dmm*dp + dmm*dm);
-    double amx = 0.5 - apx;
-    double ammx = (dpp*appx + dp*apx - dm*amx) / dmm;
-    double anx = 1. - (appx + apx + amx + ammx);
-    double rk = PI * cutoff;
-    double gre = ammx*cos(-2.*rk) + amx*cos(-rk) + anx + apx*cos(rk) + appx*cos(2.*rk);
-    double gim = ammx*sin(-2.*rk) + amx*sin(-rk) + apx*sin(rk) + appx*sin(2.*rk);
-    double a = (gre-1.)*(gre-1.) + gim*gim, b = 2.*(gre-1.), c = 1. - 4.*gre*gre - 4.*gim*gim;
-    double rf = (-b - sqrt(b*b - 4.*a*c)) / (2.*a);
-    ex[0] = ammx; ex[1] = amx; ex[2] = anx; ex[3] = apx; ex[4] = appx;
-    im[0] = rf*ammx; im[1] = rf*amx; im[3] = rf*apx; im[4] = rf*appx;
-    im[2] = 1. - (im[4] + im[3] + im[1] + im[0]);
-}
-
-/* one-sided filter at offsets (-dm, 0, dp, d2, d3), zero third moment
- * (coeff_filter_r1_l3, i3mom = 1, init_filter.f:529-625) */
-static void les_one_sided(double dm, double dp, double d2, double d3, double cutoff,
-                          double *ex, double *im)
-{
-    double a3x = 0.5*dp*dm*(-dp*dm - d2*dp + d2*d2 + d2*dm) /
-        (-dm*dm*dp*dp - dp*dp*d2*d3 - dp*dp*d2*dm + d3*d3*dp*dp + dp*d2*d2*d3 + dp*d2*d2*dm
-         - pow(d3,3)*dp + dp*d3*d2*dm + dm*dm*d2*dp - dp*dm*d3*d3 - d2*d2*d3*d3 + pow(d3,3)*d2
-         - dm*d2*d2*d3 + pow(d3,3)*dm - d3*d2*dm*dm + d3*d3*dm*dm);
-    double apx = 0.5*(dm*dm - 2.*dm*dm*a3x - 2.*d2*a3x*d3 + d2*dm - 2.*d2*dm*a3x + 2.*a3x*d3*d3) /
-                 (-dp*dp + dm*dm + d2*dp + d2*dm);
-    double amx = 0.5 - apx - a3x;
-    double a2x = -(a3x*d3 + apx*dp - amx*dm) / d2;
-    double anx = 1. - (a3x + a2x + apx + amx);
-    double rk = PI * cutoff;
-    double gre = amx*cos(-rk) + anx + apx*cos(rk) + a2x*cos(2.*rk) + a3x*cos(3.*rk);
-    double gim = amx*sin(-rk) + apx*sin(rk) + a2x*sin(2.*rk) + a3x*sin(3.*rk);
-    double a = (gre-1.)*(gre-1.) + gim*gim, b = 2.*(gre-1.), c = 1. - 4.*gre*gre - 4.*gim*gim;
-    double rf = (-b - sqrt(b*b - 4.*a*c)) / (2.*a);
-    ex[0] = amx; ex[1] = anx; ex[2] = apx; ex[3] = a2x; ex[4] = a3x;
-    im[0] = rf*amx; im[2] = rf*apx; im[3] = rf*a2x; im[4] = rf*a3x;
-    im[1] = 1. - (im[4] + im[3] + im[2] + im[0]);
-}
-
-typedef struct les_s {
-    int ny, N;
-    double chi;
-    double *lpf;                       /* [nz][nxh] lpfxz of bla.f:1503-1518 */
-    double *gw1, *gw2;                 /* [ny][5] gewy1, gewy2 */
-    double *dg[5];                     /* LU of the pentadiagonal (diags) */
-    double f3, fm3;
-} les_t;
+/* The filter operators themselves come from the caller (oracle/les_ref.py
+ * derives them from their defining conditions): lpf [nz][nxh], the x-z
+ * transfer function of bla.f:1503-1518, and L = I - H_y^N [ny][ny], so that
+ * u - H_y^N u = L u. */
+struct les_s {
+    int ny;
+    double *lpf;
+    double *L;
+};
 
 static void les_free(struct les_s *L)
 {
     if (!L)
         return;
     free(L->lpf);
-    free(L->gw1);
-    free(L->gw2);
-    for (int i = 0; i < 5; i++)
-        free(L->dg[i]);
+    free(L->L);
     free(L);
-}
-
-/* init_filter.f:9-361 and bla.f:1494-1518 */
-static les_t *les_build(int nx, int ny, int nz, double cutoff, int iord, const double *eta)
-{
-    les_t *L = calloc(1, sizeof(les_t));
-    int nxh = nx / 2, n = ny;
-    L->ny = ny;
-    L->N = iord + 1;
-    double ex[5], im[5];
-    les_centered(2., 1., 1., 2., cutoff, ex, im);
-    int mz = nz / 2 + 1, mx = nxh + 1;
-    double *gx = malloc(sizeof(double) * (size_t)mx), *gz = malloc(sizeof(double) * (size_t)mz);
-    for (int pass = 0; pass < 2; pass++) {
-        int m = pass ? mz : mx;
-        double *g = pass ? gz : gx;
-        for (int i = 0; i < m; i++) {
-            double rk = i * PI / (m - 1);
-            double ra = ex[0]*cos(-2.*rk) + ex[1]*cos(-rk) + ex[2] + ex[3]*cos(rk) + ex[4]*cos(2.*rk);
-            double rc = im[0]*cos(-2.*rk) + im[1]*cos(-rk) + im[2] + im[3]*cos(rk) + im[4]*cos(2.*rk);
-            double rb = ex[0]*sin(-2.*rk) + ex[1]*sin(-rk) + ex[3]*sin(rk) + ex[4]*sin(2.*rk);
-            double rd = im[0]*sin(-2.*rk) + im[1]*sin(-rk) + im[3]*sin(rk) + im[4]*sin(2.*rk);
-            g[i] = (ra*rc + rb*rd) / (rc*rc + rd*rd);
-        }
-    }
-    L->lpf = malloc(sizeof(double) * (size_t)nz * nxh);
-    for (int z = 0; z < nz; z++)
-        for (int x = 0; x < nxh; x++) {
-            int kz = z < nz / 2 ? z : nz - z;        /* gewspecz(nz-j+2), bla.f:1507 */
-            L->lpf[(size_t)z * nxh + x] = 1. - pow(1. - gx[x] * gz[kz], L->N);
-        }
-    free(gx);
-    free(gz);
-    L->gw1 = calloc(5 * (size_t)n, sizeof(double));
-    L->gw2 = calloc(5 * (size_t)n, sizeof(double));
-    L->gw1[0] = L->gw2[0] = 1.;
-    L->gw1[5 * (n - 1) + 4] = L->gw2[5 * (n - 1) + 4] = 1.;
-    les_one_sided(eta[0] - eta[1], eta[1] - eta[2], eta[1] - eta[3], eta[1] - eta[4], cutoff,
-                  L->gw1 + 5, L->gw2 + 5);
-    for (int k = 2; k <= n - 3; k++)
-        les_centered(eta[k - 2] - eta[k], eta[k - 1] - eta[k], eta[k] - eta[k + 1],
-                     eta[k] - eta[k + 2], cutoff, L->gw1 + 5 * k, L->gw2 + 5 * k);
-    {
-        int k = n - 2;        /* coeff_filter_r3_l1: the mirrored one-sided stencil */
-        double e1[5], i1[5];
-        les_one_sided(eta[k] - eta[k + 1], eta[k - 1] - eta[k], eta[k - 2] - eta[k],
-                      eta[k - 3] - eta[k], cutoff, e1, i1);
-        for (int l = 0; l < 5; l++) {
-            L->gw1[5 * k + l] = e1[4 - l];
-            L->gw2[5 * k + l] = i1[4 - l];
-        }
-    }
-    /* diags (init_filter.f:305-343): ud2, ud1, hd, od1, od2 */
-    for (int i = 0; i < 5; i++)
-        L->dg[i] = calloc((size_t)n, sizeof(double));
-    double *ud2 = L->dg[0], *ud1 = L->dg[1], *hd = L->dg[2], *od1 = L->dg[3], *od2 = L->dg[4];
-    const double *g2 = L->gw2;
-    hd[0] = hd[n - 1] = 1.;
-    L->f3 = -g2[5 + 4] / g2[10 + 4];
-    L->fm3 = -g2[5 * (n - 2)] / g2[5 * (n - 3)];
-    ud1[1] = g2[5] + g2[10] * L->f3;
-    hd[1] = g2[6] + g2[11] * L->f3;
-    od1[1] = g2[7] + g2[12] * L->f3;
-    od2[1] = g2[8] + g2[13] * L->f3;
-    ud2[n - 2] = g2[5 * (n - 2) + 1] + g2[5 * (n - 3) + 1] * L->fm3;
-    ud1[n - 2] = g2[5 * (n - 2) + 2] + g2[5 * (n - 3) + 2] * L->fm3;
-    hd[n - 2] = g2[5 * (n - 2) + 3] + g2[5 * (n - 3) + 3] * L->fm3;
-    od1[n - 2] = g2[5 * (n - 2) + 4] + g2[5 * (n - 3) + 4] * L->fm3;
-    for (int k = 2; k <= n - 3; k++) {
-        ud2[k] = g2[5 * k];
-        ud1[k] = g2[5 * k + 1];
-        hd[k] = g2[5 * k + 2];
-        od1[k] = g2[5 * k + 3];
-        od2[k] = g2[5 * k + 4];
-    }
-    /* pntdiag_lu (bandsolver.f:70-140) */
-    ud2[0] = ud2[1] = ud1[0] = 0.;
-    od1[n - 1] = od2[n - 2] = od2[n - 1] = 0.;
-    double h = 1. / hd[0];
-    od1[0] *= h;
-    od2[0] *= h;
-    hd[1] -= ud1[1] * od1[0];
-    h = 1. / hd[1];
-    od1[1] = (od1[1] - ud1[1] * od2[0]) * h;
-    od2[1] *= h;
-    for (int i = 2; i < n; i++) {
-        ud1[i] -= ud2[i] * od1[i - 2];
-        hd[i] = hd[i] - ud2[i] * od2[i - 2] - ud1[i] * od1[i - 1];
-        if (i < n - 1) {
-            h = 1. / hd[i];
-            od1[i] = (od1[i] - ud1[i] * od2[i - 1]) * h;
-            od2[i] *= h;
-        }
-    }
-    return L;
-}
-
-/* hpf_y.f on one sequence d[ny] in place; rs is scratch */
-static void les_hpf_y(const les_t *L, double *d, double *rs)
-{
-    int n = L->ny;
-    const double *g1 = L->gw1, *g2 = L->gw2;
-    for (int j = 0; j < n; j++)
-        rs[j] = 0.;
-    for (int l = 0; l < 5; l++) {
-        rs[1] += (g2[5 + l] - g1[5 + l]) * d[l];
-        rs[n - 2] += (g2[5 * (n - 2) + l] - g1[5 * (n - 2) + l]) * d[n - 5 + l];
-    }
-    for (int l = 0; l < 5; l++)
-        for (int j = 2; j <= n - 3; j++)
-            rs[j] += (g2[5 * j + l] - g1[5 * j + l]) * d[j - 2 + l];
-    rs[1] += rs[2] * L->f3;
-    rs[n - 2] += rs[n - 3] * L->fm3;
-    const double *ud2 = L->dg[0], *ud1 = L->dg[1], *hd = L->dg[2], *od1 = L->dg[3],
-                 *od2 = L->dg[4];
-    rs[0] *= 1. / hd[0];
-    rs[1] = (rs[1] - ud1[1] * rs[0]) * (1. / hd[1]);
-    for (int j = 2; j < n; j++)
-        rs[j] = (rs[j] - ud2[j] * rs[j - 2] - ud1[j] * rs[j - 1]) * (1. / hd[j]);
-    d[n - 1] = rs[n - 1];
-    d[n - 2] = rs[n - 2] - od1[n - 2] * d[n - 1];
-    for (int j = n - 3; j >= 0; j--)
-        d[j] = rs[j] - od1[j] * d[j + 1] - od2[j] * d[j + 2];
 }
 
 /* External SGS force for the next orc_nonlin calls (sigma of the
@@ -1571,19 +1379,23 @@ void orc_set_fringe(orc_t *o, const double *lam, const double *base)
     memcpy(o->fbase, base, nb * sizeof(double));
 }
 
-int orc_set_les(orc_t *o, int iles, double cutoff, int iord, double chi)
+int orc_set_les(orc_t *o, const double *lpf, const double *L, double chi)
 {
     les_free(o->les);
     o->les = NULL;
     free(o->gur);
     free(o->gui);
     o->gur = o->gui = NULL;
-    if (iles == 0)
+    if (!lpf || !L)
         return 0;
-    if (iles != 1 || iord < 0 || iord > 5 || !(cutoff > 0.) || cutoff > 1.)
-        return -1;
-    o->les = les_build(o->nx, o->ny, o->nz, cutoff, iord, o->eta);
-    o->les->chi = chi;
+    struct les_s *F = calloc(1, sizeof(struct les_s));
+    size_t nl = (size_t)o->nz * o->nxh, nm = (size_t)o->ny * o->ny;
+    F->ny = o->ny;
+    F->lpf = malloc(sizeof(double) * nl);
+    F->L = malloc(sizeof(double) * nm);
+    memcpy(F->lpf, lpf, sizeof(double) * nl);
+    memcpy(F->L, L, sizeof(double) * nm);
+    o->les = F;
     o->les_chi = chi;                   /* chi/dstar, dstar = 1 (bla.f:2220) */
     size_t fs = (size_t)o->nz * o->ny * o->nxh;
     o->gur = malloc(sizeof(double) * 3 * fs);
@@ -1591,56 +1403,26 @@ int orc_set_les(orc_t *o, int iles, double cutoff, int iord, double chi)
     return 0;
 }
 
-/* high-pass filtered velocities g = u - lpf (u - H_y^N u) of slots 0-2
- * (linearbl.f:1871-1962, ihighorder = 2) into o->gur, o->gui */
+/* high-pass filtered velocities g = u - lpf (u - H_y^N u) = u - lpf L u of
+ * slots 0-2 (linearbl.f:1871-1962, ihighorder = 2) into o->gur, o->gui */
 static void les_gu(orc_t *o, const double *ur, const double *ui)
 {
-    const les_t *L = o->les;
+    const struct les_s *F = o->les;
     int ny = o->ny, nxh = o->nxh;
-#pragma omp parallel
-    {
-        double *d = malloc(sizeof(double) * (size_t)ny), *rs = malloc(sizeof(double) * (size_t)ny);
-#pragma omp for collapse(2) schedule(static)
-        for (int s = 0; s < 3; s++)
-            for (int z = 0; z < o->nz; z++)
-                for (int x = 0; x < nxh; x++)
-                    for (int part = 0; part < 2; part++) {
-                        const double *u = part ? ui : ur;
-                        double *g = part ? o->gui : o->gur;
-                        for (int y = 0; y < ny; y++)
-                            d[y] = u[IDX(o, s, z, y, x)];
-                        for (int r = 0; r < L->N; r++)
-                            les_hpf_y(L, d, rs);
-                        double lp = L->lpf[(size_t)z * nxh + x];
-                        for (int y = 0; y < ny; y++) {
-                            double uu = u[IDX(o, s, z, y, x)];
-                            g[IDX(o, s, z, y, x)] = uu - lp * (uu - d[y]);
-                        }
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int s = 0; s < 3; s++)
+        for (int z = 0; z < o->nz; z++)
+            for (int x = 0; x < nxh; x++) {
+                double lp = F->lpf[(size_t)z * nxh + x];
+                for (int part = 0; part < 2; part++) {
+                    const double *u = part ? ui : ur;
+                    double *g = part ? o->gui : o->gur;
+                    for (int y = 0; y < ny; y++) {
+                        double lu = 0.;
+                        for (int k = 0; k < ny; k++)
+                            lu += F->L[(size_t)y * ny + k] * u[IDX(o, s, z, k, x)];
+                        g[IDX(o, s, z, y, x)] = u[IDX(o, s, z, y, x)] - lp * lu;
                     }
-        free(d);
-        free(rs);
-    }
-}
-
-/* the filter operators for tests: lpf[nz][nxh] and the y matrix
- * L = I - H_y^N [ny][ny], H_y applied to the columns of the identity */
-int orc_les_filter(int nx, int ny, int nz, double cutoff, int iord, double *lpf, double *Lm)
-{
-    double *eta = cheb_cos_table(ny);
-    les_t *L = les_build(nx, ny, nz, cutoff, iord, eta);
-    memcpy(lpf, L->lpf, sizeof(double) * (size_t)nz * (nx / 2));
-    double *d = malloc(sizeof(double) * (size_t)ny), *rs = malloc(sizeof(double) * (size_t)ny);
-    for (int c = 0; c < ny; c++) {
-        for (int y = 0; y < ny; y++)
-            d[y] = y == c ? 1. : 0.;
-        for (int r = 0; r < L->N; r++)
-            les_hpf_y(L, d, rs);
-        for (int y = 0; y < ny; y++)
-            Lm[(size_t)y * ny + c] = (y == c ? 1. : 0.) - d[y];
-    }
-    free(d);
-    free(rs);
-    free(eta);
-    les_free(L);
-    return 0;
+                }
+            }
 }

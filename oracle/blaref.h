@@ -58,14 +58,14 @@ int orc_run(orc_t *o, double *ur, double *ui, double *puw, int nsteps,
  * hold u, v, w, omega_x, omega_z. */
 void orc_pressure(orc_t *o, const double *ur, const double *ui, double *pr, double *pi);
 
-/* ADM-RT LES (sgs.i iles = 1, ihighorder = 2): iles 0 turns it off.
- * cutoff = 1/cutoff_inv, iord, chi as read by rparamles.f; dstar = 1.
- * Returns 0, or -1 for unsupported parameters. */
-int orc_set_les(orc_t *o, int iles, double cutoff, int iord, double chi);
+/* ADM-RT LES (sgs.i iles = 1, ihighorder = 2): the relaxation term of
+ * nonlinbl.f:920-966 with the filter operators lpf [nz][nx/2] (x-z transfer
+ * function, bla.f:1503-1518) and L = I - H_y^(iord+1) [ny][ny] (wall-normal
+ * low-pass, hpf_y.f) given by the caller, and chi/dstar (dstar = 1);
+ * lpf = NULL turns it off.  Returns 0. */
+int orc_set_les(orc_t *o, const double *lpf, const double *L, double chi);
 
 /* ---- building blocks exposed for unit tests ---- */
-/* the ADM-RT filter operators: lpf[nz][nx/2] (bla.f:1503-1518) and the
- * wall-normal low-pass L = I - H_y^(iord+1) as a dense [ny][ny] matrix */
 /* External SGS force sigma [3][nz][ny][nxh] (physical y) subtracted from H
  * (times nxp nzp, not on the z-oddball row) in later orc_nonlin calls, as
  * nonlinbl.f:1011-1050 does for the eddy-viscosity models; NULL: off. */
@@ -79,7 +79,6 @@ void orc_set_fringe(orc_t *o, const double *lam, const double *base);
  * orc_linear calls; NULL: off. */
 void orc_set_wall(orc_t *o, const double *vr, const double *vi);
 
-int orc_les_filter(int nx, int ny, int nz, double cutoff, int iord, double *lpf, double *L);
 /* complex DFT, y_k = sum_j x_j exp(sign 2 pi i jk/n), in place on (re,im) */
 void orc_cdft(int n, double *re, double *im, int sign);
 /* vrfftb convention: half spectrum X[0..n/2] (re,im) -> x[0..n-1] real */

@@ -50,12 +50,10 @@ def lib():
                                             _dp]
         L.orc_pressure.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_set_les.restype = C.c_int
-        L.orc_set_les.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double]
+        L.orc_set_les.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]
         L.orc_set_force.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_set_fringe.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_set_wall.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
-        L.orc_les_filter.restype = C.c_int
-        L.orc_les_filter.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _dp, _dp]
         _lib = L
     return _lib
 
@@ -109,9 +107,17 @@ class Oracle:
         return st, cfl.value
 
     def set_les(self, iles=1, cutoff=2.0 / 3.0, iord=5, chi=0.2):
-        """ADM-RT relaxation model (sgs.i iles = 1) on, or off with iles = 0."""
-        if lib().orc_set_les(self.h, iles, cutoff, iord, chi) != 0:
+        """ADM-RT relaxation model (sgs.i iles = 1) on, or off with iles = 0;
+        the filter operators come from les_ref (their defining conditions)."""
+        if iles == 0:
+            self._les = None
+            lib().orc_set_les(self.h, None, None, 0.0)
+            return
+        if iles != 1 or not 0 <= iord <= 5:
             raise ValueError("unsupported LES parameters")
+        lpf, L = les_filter(self.nx, self.ny, self.nz, cutoff, iord)
+        self._les = (np.ascontiguousarray(lpf), np.ascontiguousarray(L))
+        lib().orc_set_les(self.h, self._les[0].ctypes.data, self._les[1].ctypes.data, chi)
 
     def set_fringe(self, lam=None, base=None):
         """Fringe forcing with lam [nxp] and base flow [3][ny][nxp]; None: off."""
@@ -140,11 +146,9 @@ class Oracle:
 
 
 def les_filter(nx, ny, nz, cutoff, iord):
-    """(lpf [nz][nx/2], L [ny][ny]) of the ADM-RT filters."""
-    lpf = np.zeros((nz, nx // 2))
-    L = np.zeros((ny, ny))
-    lib().orc_les_filter(nx, ny, nz, cutoff, iord, lpf, L)
-    return lpf, L
+    """(lpf [nz][nx/2], L [ny][ny]) of the ADM-RT filters (les_ref)."""
+    import les_ref
+    return les_ref.filter_operators(nx, ny, nz, cutoff, iord)
 
 
 def velocities(ur, ui):

@@ -8,6 +8,11 @@ Pins:
   cutoff_inv 3, iord 2, chi 2 after 10 steps.  The start field excites every
   resolved wavenumber, so the model moves v and w by about 1e-2 relative
   within 10 steps (dns_u10 in the fixture) and 1e-12 pins it;
+* the row stencils (oracle/les_ref.py, derived from their conditions)
+  against those conditions checked one by one: moments 0-2, B(pi) = 0, the
+  third moment (wall rows), minimum dispersion by brute force over the
+  one-parameter family the other conditions leave (interior rows), and the
+  implicit factor's |G(pi cutoff)| = 1/2;
 * the filter operators against what the mathematics fixes: the y low-pass
   reproduces polynomials up to degree 2 (second-order moments) but not
   degree 6, and its rows at the walls are those of the identity (the
@@ -15,7 +20,8 @@ Pins:
   and crosses 1 - (1/2)^N at the cutoff (the implicit part is scaled so that
   the filter is 1/2 there);
 * the library's own filter construction (a C-ABI function that needs no
-  GPU) against the oracle's;
+  GPU; a Lagrange system and dense pivoted solves in C++) against the
+  oracle's (SVD null space and numpy solves);
 * on the GPU, the step with the model against the same fixture, FP64 at
   1e-12 and mixed precision at 1e-5.
 """
@@ -65,6 +71,49 @@ def test_sgs_file_round_trip_and_checks():
     assert e.value.code == 345432
 
 
+def test_row_stencils_satisfy_their_conditions():
+    import les_ref
+    ny = 33
+    eta = np.cos(np.pi * np.arange(ny) / (ny - 1))
+    for cutoff in (2.0 / 3.0, 1.0 / 3.0, 0.9):
+        for k in (1, 2, 5, ny // 2, ny - 3, ny - 2):
+            cols = (np.arange(0, 5) if k == 1 else np.arange(ny - 5, ny) if k == ny - 2
+                    else np.arange(k - 2, k + 3))
+            j = cols - k
+            p = eta[k] - eta[cols]
+            wall = k in (1, ny - 2)
+            b, a = les_ref.stencil(p, j, wall, cutoff)
+            h = np.abs(p).max()
+            mom = [float(b @ (p / h) ** m) for m in range(4)]
+            assert abs(mom[0] - 1) < 1e-14 and abs(mom[1]) < 1e-14 and abs(mom[2]) < 1e-14
+            assert abs(b @ (-1.0) ** j) < 1e-14                  # B(pi) = 0
+            if wall:
+                assert abs(mom[3]) < 1e-13
+            else:
+                # brute force over the family b + t n (n: null vector of the
+                # four other conditions): the odd part is smallest at t = 0
+                Cm = np.array([np.ones(5), p / h, (p / h) ** 2, (-1.0) ** j])
+                n = np.linalg.svd(Cm)[2][-1]
+                odd = lambda w: (w[3] - w[1]) ** 2 + (w[4] - w[0]) ** 2
+                ts = np.linspace(-1, 1, 2001) * 0.05
+                best = ts[np.argmin([odd(b + t * n) for t in ts])]
+                assert abs(best) <= 0.05 / 1000 + 1e-15, (k, cutoff, best)
+            # implicit part: a_j = r b_j off the centre, weights sum to 1,
+            # and |B/A| = 1/2 at the cutoff
+            c = list(j).index(0)
+            r = a[(c + 1) % 5] / b[(c + 1) % 5]
+            assert np.allclose(np.delete(a, c), r * np.delete(b, c), rtol=1e-13, atol=0)
+            assert abs(a.sum() - 1) < 1e-14
+            kc = np.pi * cutoff
+            G = les_ref.transfer(b, j, kc)[0] / les_ref.transfer(a, j, kc)[0]
+            assert abs(abs(G) - 0.5) < 1e-13, (k, cutoff, abs(G))
+    # the homogeneous filter is real, 1 at k = 0, 0 at the grid Nyquist
+    g = les_ref.homogeneous_transfer(17, 2.0 / 3.0)
+    assert abs(g[0] - 1) < 1e-15 and abs(g[-1]) < 1e-15
+    with pytest.raises(ValueError):
+        les_ref.filter_operators(NX, NY, NZ, 1.0, 5)
+
+
 def test_filter_operators_mathematics():
     ny = 33
     lpf, L = pyoracle.les_filter(NX, ny, NZ, 2.0 / 3.0, 5)
@@ -84,11 +133,14 @@ def test_filter_operators_mathematics():
 
 def test_library_filter_matches_oracle():
     from simson_amd.gpu import les_filter
-    for cut, iord in ((2.0 / 3.0, 5), (1.0 / 3.0, 2)):
-        lpf, L = pyoracle.les_filter(NX, NY, NZ, cut, iord)
-        lpf2, L2 = les_filter(NX, NY, NZ, cut, iord)
-        assert np.abs(lpf - lpf2).max() < 1e-15
+    for ny, cut, iord in ((NY, 2.0 / 3.0, 5), (NY, 1.0 / 3.0, 2), (129, 2.0 / 3.0, 5)):
+        lpf, L = pyoracle.les_filter(NX, ny, NZ, cut, iord)
+        lpf2, L2 = les_filter(NX, ny, NZ, cut, iord)
+        assert np.abs(lpf - lpf2).max() < 1e-14
         assert np.abs(L - L2).max() < 1e-13
+    from simson_amd.gpu import SimsonError
+    with pytest.raises(SimsonError):
+        les_filter(NX, NY, NZ, 1.0, 5)
 
 
 @pytest.mark.parametrize("case", ["a", "b"])
