@@ -3,6 +3,7 @@
 Runs in the build container only (it needs /root/reference and amdflang):
 
     python tests/golden/make_golden.py            # c1_32x33x32.npz
+    python tests/golden/make_golden.py long       # c1_1000_32x33x32.npz
     python tests/golden/make_golden.py pressure   # p_32x33x32.npz
     python tests/golden/make_golden.py les        # les_32x33x32.npz
     python tests/golden/make_golden.py xys        # xys_32x33x32.npz
@@ -85,6 +86,40 @@ def main():
                             amp_t=amp[:, 0], amp_px=amp[:, 11], amp=amp,
                             cfl=np.array(cfl), init_sha256=np.array(sha))
         print("wrote", os.path.join(HERE, "c1_32x33x32.npz"), "cfl lines", len(cfl),
+              "amp rows", amp.shape)
+    finally:
+        shutil.rmtree(work)
+
+
+def long():
+    """The C1 case over its full 1000 steps (maxit 3000 substeps): the fields
+    after 500 and 1000 steps, amp.stat every 10th step and the CFL*dt lines,
+    into c1_1000_32x33x32.npz."""
+    if not os.path.exists(BIN):
+        subprocess.check_call([os.path.join(REPO, "oracle", "build_ref.sh"),
+                               str(NX), str(NY), str(NZ), "1", "no", "0",
+                               os.path.basename(BIN)])
+    work = tempfile.mkdtemp(prefix="simson_golden_l.")
+    try:
+        ufile.write_u(os.path.join(work, "init.u"), c1_field())
+        p = blai.BlaParams(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=3000,
+                           dt=0.01, nst=3, xl=4 * math.pi, cflux=True, icfl=3,
+                           iamp=30, namamp="amp.stat", tsave=[5.0], nmsave=["t500.u"])
+        with open(os.path.join(work, "bla.i"), "w") as f:
+            f.write(blai.format_channel(p))
+        with open(os.path.join(work, "out.txt"), "w") as out:
+            subprocess.check_call([BIN], cwd=work, stdout=out, stderr=subprocess.STDOUT)
+        log = open(os.path.join(work, "out.txt")).read()
+        if "bla terminated normally" not in log:
+            raise RuntimeError("reference bla did not terminate normally")
+        cfl = [float(m) for m in re.findall(r"CFL\*dt\s+([0-9.Ee+-]+)", log)]
+        amp = np.loadtxt(os.path.join(work, "amp.stat"))
+        sha = hashlib.sha256(open(os.path.join(work, "init.u"), "rb").read()).hexdigest()
+        np.savez_compressed(os.path.join(HERE, "c1_1000_32x33x32.npz"),
+                            u500=ufile.read_u(os.path.join(work, "t500.u")).u,
+                            u1000=ufile.read_u(os.path.join(work, "end.u")).u,
+                            amp=amp, cfl=np.array(cfl), init_sha256=np.array(sha))
+        print("wrote", os.path.join(HERE, "c1_1000_32x33x32.npz"), "cfl lines", len(cfl),
               "amp rows", amp.shape)
     finally:
         shutil.rmtree(work)
@@ -365,5 +400,7 @@ if __name__ == "__main__":
         fringe()
     elif len(sys.argv) > 1 and sys.argv[1] == "wall":
         wall()
+    elif len(sys.argv) > 1 and sys.argv[1] == "long":
+        long()
     else:
         main()

@@ -66,3 +66,36 @@ def test_oracle_matches_reference_bla(golden_c1, nsteps, key):
     # CFL*dt printed by wcflbl at the last evaluation
     cfl_ref = golden_c1["cfl"][nsteps - 1]
     assert abs(cfl - cfl_ref) <= 1e-12 * cfl_ref
+
+
+@pytest.fixture(scope="module")
+def golden_c1_long():
+    from conftest import ROOT
+    return np.load(os.path.join(ROOT, "tests", "golden", "c1_1000_32x33x32.npz"))
+
+
+def test_oracle_matches_reference_bla_1000_steps(golden_c1_long):
+    """The whole C1 run: 1000 RK3 steps at dt = 0.01 (maxit 3000), the
+    fields after 500 and 1000 steps, every CFL*dt line and the amp.stat px
+    column (every 10th step).
+
+    Bars: 1e-12 relative L2 after 500 steps, 1e-11 after 1000.  Between
+    step 100 and step 1000 the perturbation is reshaped by transient growth
+    (|w| doubles, |v| halves), which amplifies any difference in rounding:
+    the absolute difference between two correct FP64 codes grows about 40x
+    over that span, and this restatement ends 2.4e-12 (v) from the reference
+    after 1000 steps (3e-14 after 100)."""
+    f0 = c1_init()
+    o = pyoracle.Oracle(NX, NY, NZ, f0.re, f0.xl, f0.zl, cflux=True)
+    ur, ui, puw = o.new_state(f0.u)
+    st, _ = o.run(ur, ui, puw, 500, 0.01, icfl=3)
+    assert st == 0
+    e500 = rel_l2(pyoracle.velocities(ur, ui), golden_c1_long["u500"])
+    st, cfl = o.run(ur, ui, puw, 500, 0.01, icfl=3)
+    assert st == 0
+    e1000 = rel_l2(pyoracle.velocities(ur, ui), golden_c1_long["u1000"])
+    print(f"oracle vs reference bla: 500 steps {e500}, 1000 steps {e1000}")
+    assert max(e500) < 1e-12 and max(e1000) < 1e-11, (e500, e1000)
+    assert abs(cfl - golden_c1_long["cfl"][-1]) <= 1e-12 * golden_c1_long["cfl"][-1]
+    assert len(golden_c1_long["cfl"]) == 1000
+    assert abs(o.px - golden_c1_long["amp"][-1, 11]) <= 1e-10 * abs(golden_c1_long["amp"][-1, 11])
