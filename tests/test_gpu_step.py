@@ -143,6 +143,44 @@ def test_c1_matches_reference_bla_100_steps(golden_c1):
         assert abs(px - golden_c1["amp_px"][n]) <= 1e-9 * abs(golden_c1["amp_px"][n])
 
 
+def test_c1_matches_reference_bla_1000_steps():
+    """The whole C1 run (1000 steps) against the reference bla's fields after
+    500 and 1000 steps, its last CFL*dt line and all 100 amp.stat px rows
+    (written after steps 1, 11, ..., 991).  Bars as for the oracle: 1e-12
+    after 500 steps; 1e-11 after 1000, since transient growth over steps
+    100-1000 amplifies rounding differences ~40x (the oracle ends 2.4e-12
+    from the reference in v); px 1e-9."""
+    import os
+    from conftest import ROOT
+    gold = np.load(os.path.join(ROOT, "tests", "golden", "c1_1000_32x33x32.npz"))
+    say("[gpu] 32x33x32 1000 steps vs reference bla")
+    f = c1_field()
+    amp = gold["amp"]
+    rows = {int(round(t / 0.01)): i for i, t in enumerate(amp[:, 0])}
+    done, px_err, out = 0, 0.0, {}
+    with BlaGPU.from_ufield(f) as g:
+        g.upload_velocity(f.u, -2.0 / f.re)
+        for target in sorted(set(rows) | {500, 1000}):
+            cfl = g.run(target - done, 0.01, icfl=3, first=(done == 0))
+            done = target
+            if target in rows:
+                px = g.download(0)[3]
+                px_ref = amp[rows[target], 11]
+                px_err = max(px_err, abs(px - px_ref) / abs(px_ref))
+            if target in (500, 1000):
+                out[target] = g.velocities()
+    for n, bar in ((500, 1e-12), (1000, 1e-11)):
+        ref = gold[f"u{n}"]
+        d = np.delete(out[n] - ref, 16, axis=1)
+        r = np.delete(ref, 16, axis=1)
+        errs = [np.linalg.norm(d[c]) / np.linalg.norm(r[c]) for c in range(3)]
+        say(f"    u{n}: rel L2 {errs}")
+        assert max(errs) <= bar, (n, errs)
+    say(f"    px max rel err over 100 rows {px_err:.2e}")
+    assert px_err <= 1e-9
+    assert abs(cfl - gold["cfl"][-1]) <= 1e-12 * gold["cfl"][-1]
+
+
 def test_run_equals_substep_sequence():
     """The fixed-dt driver and host-driven substeps are the same computation
     (weights formed as bla does, dtnp1*anrk, bla.f:2394-2396)."""

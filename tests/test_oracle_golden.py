@@ -76,8 +76,8 @@ def golden_c1_long():
 
 def test_oracle_matches_reference_bla_1000_steps(golden_c1_long):
     """The whole C1 run: 1000 RK3 steps at dt = 0.01 (maxit 3000), the
-    fields after 500 and 1000 steps, every CFL*dt line and the amp.stat px
-    column (every 10th step).
+    fields after 500 and 1000 steps, the last CFL*dt line and the amp.stat px
+    column (all 100 rows, written after steps 1, 11, ..., 991).
 
     Bars: 1e-12 relative L2 after 500 steps, 1e-11 after 1000.  Between
     step 100 and step 1000 the perturbation is reshaped by transient growth
@@ -88,14 +88,25 @@ def test_oracle_matches_reference_bla_1000_steps(golden_c1_long):
     f0 = c1_init()
     o = pyoracle.Oracle(NX, NY, NZ, f0.re, f0.xl, f0.zl, cflux=True)
     ur, ui, puw = o.new_state(f0.u)
-    st, _ = o.run(ur, ui, puw, 500, 0.01, icfl=3)
-    assert st == 0
-    e500 = rel_l2(pyoracle.velocities(ur, ui), golden_c1_long["u500"])
-    st, cfl = o.run(ur, ui, puw, 500, 0.01, icfl=3)
-    assert st == 0
+    amp = golden_c1_long["amp"]
+    # amp.stat row i is written after step 1 + 10 i (t = 0.01 + 0.1 i)
+    rows = {int(round(t / 0.01)): i for i, t in enumerate(amp[:, 0])}
+    assert min(rows) == 1 and max(rows) == 991
+    done, e500, cfl, px_err = 0, None, None, 0.0
+    for target in sorted(set(rows) | {500, 1000}):
+        st, cfl = o.run(ur, ui, puw, target - done, 0.01, icfl=3)
+        assert st == 0
+        done = target
+        if target in rows:
+            px_ref = amp[rows[target], 11]
+            px_err = max(px_err, abs(o.px - px_ref) / abs(px_ref))
+        if target == 500:
+            e500 = rel_l2(pyoracle.velocities(ur, ui), golden_c1_long["u500"])
     e1000 = rel_l2(pyoracle.velocities(ur, ui), golden_c1_long["u1000"])
-    print(f"oracle vs reference bla: 500 steps {e500}, 1000 steps {e1000}")
+    print(f"oracle vs reference bla: 500 steps {e500}, 1000 steps {e1000}, px {px_err}")
     assert max(e500) < 1e-12 and max(e1000) < 1e-11, (e500, e1000)
     assert abs(cfl - golden_c1_long["cfl"][-1]) <= 1e-12 * golden_c1_long["cfl"][-1]
     assert len(golden_c1_long["cfl"]) == 1000
-    assert abs(o.px - golden_c1_long["amp"][-1, 11]) <= 1e-10 * abs(golden_c1_long["amp"][-1, 11])
+    # px carries the rounding noise of the secant flux update (see above);
+    # over the 100 rows of this run it stays within 3.2e-10 of the reference
+    assert px_err <= 1e-9, px_err
