@@ -107,6 +107,15 @@ struct simson_gpu {
     double wall_tomeg = 0., wall_p0 = 0., wall_m0 = 0., tnow = 0.;
     double2 *wallP = nullptr, *wallM = nullptr;
     double mflux = 0., u0upp = 0., u0low = 0., w0upp = 0., w0low = 0.;
+    // the y transforms with their parity-split forms: analysis (linear step
+    // and pressure, prhs), synthesis, the y derivative of the stresses and
+    // the ADM-RT low-pass; split blocks live in 'extra'
+    ChebOp oTf, oTp, oTb, oDm, oLm;
+    std::vector<void *> extra;
+    // CFL read-back of simson_gpu_run: the latest value of each evaluation
+    // copied to pinned host memory (two slots) behind an event
+    double *cfl_host = nullptr;
+    hipEvent_t cfl_ev[2] = {nullptr, nullptr};
     bool timing = false;
     std::vector<TimedLaunch> launches;
     std::vector<hipEvent_t> pool;
@@ -203,6 +212,13 @@ void free_all(simson_gpu_t *g)
     for (void *q : ptrs)
         if (q)
             (void)hipFree(q);
+    for (void *q : g->extra)
+        (void)hipFree(q);
+    if (g->cfl_host)
+        (void)hipHostFree(g->cfl_host);
+    for (auto e : g->cfl_ev)
+        if (e)
+            (void)hipEventDestroy(e);
     for (auto &t : g->launches) {
         (void)hipEventDestroy(t.a);
         (void)hipEventDestroy(t.b);
@@ -225,6 +241,47 @@ template <class T>
 int h2d(simson_gpu_t *g, T *d, const T *h, size_t n)
 {
     CK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, g->stream));
+    return 0;
+}
+
+// A transform operator: the dense device matrices A (FP64) and A32 (FP32 or
+// null) of the host matrix Ah, and its parity-split blocks when ny has a
+// split kernel (FP32 blocks when A32 is given).  SIMSON_CHEB=dense keeps the
+// dense kernels (for comparisons).
+int make_op(simson_gpu_t *g, ChebOp &op, const std::vector<double> &Ah, const double *A,
+            const float *A32, int in_col, int out_col)
+{
+    op = ChebOp{};
+    op.A = A;
+    op.A32 = A32;
+    const char *env = getenv("SIMSON_CHEB");
+    if (env && strcmp(env, "dense") == 0)
+        return 0;
+    std::vector<double> M;
+    int flip = 0;
+    const int K = cheb_split(Ah.data(), g->ny, in_col, out_col, M, &flip);
+    if (K == 0)
+        return 0;
+    double *S = nullptr;
+    CK(hipMalloc((void **)&S, M.size() * sizeof(double)));
+    g->extra.push_back(S);
+    if (h2d(g, S, M.data(), M.size()))
+        return SIMSON_E_HIP;
+    if (A32) {
+        std::vector<float> M32(M.begin(), M.end());
+        float *S32 = nullptr;
+        CK(hipMalloc((void **)&S32, M32.size() * sizeof(float)));
+        g->extra.push_back(S32);
+        if (h2d(g, S32, M32.data(), M32.size()))
+            return SIMSON_E_HIP;
+        // the copy reads host memory that goes out of scope on return
+        CK(hipStreamSynchronize(g->stream));
+        op.S32 = S32;
+    }
+    CK(hipStreamSynchronize(g->stream));
+    op.S = S;
+    op.K = K;
+    op.flip = flip;
     return 0;
 }
 
@@ -375,7 +432,7 @@ Step2Args<float2> nonlin_args32(simson_gpu_t *g, int do_cfl)
 // derivatives (the tau_ij of les.f and the first half of sgs_stress.f)
 template <typename V>
 void closure_stresses(simson_gpu_t *g, Step2Args<V> c, V *PG, V *TAU, V *DT, const V *twz,
-                      const V *twx, const void *Dm, bool fp32)
+                      const V *twx, bool fp32)
 {
     c.P = PG;
     c.Uw = TAU;
@@ -386,9 +443,9 @@ void closure_stresses(simson_gpu_t *g, Step2Args<V> c, V *PG, V *TAU, V *DT, con
     GemmIO io = gemm_io(g, TAU, 0, DT, 0, {1, 3, 4}, {0, 1, 2}, g->nz / 2);
     timed(g, F_CDY, [&] {
         if (fp32)
-            launch_cheb_mfma_f32((const float *)Dm, (const float *)TAU, (float *)DT, io, g->stream);
+            launch_cheb_f32(g->oDm, (const float *)TAU, (float *)DT, io, g->stream);
         else
-            launch_cheb_gemm((const double *)Dm, io, g->stream);
+            launch_cheb_gemm(g->oDm, io, g->stream);
     });
 }
 
@@ -467,6 +524,12 @@ int alloc_stress(simson_gpu_t *g, bool fp64, bool fp32)
             if (dalloc(g, &g->Dm32, D32.size()) || h2d(g, g->Dm32, D32.data(), D32.size()))
                 return SIMSON_E_HIP;
         }
+        if (int st = make_op(g, g->oDm, D, g->Dm, g->Dm32, 0, 0))
+            return st;
+        if (g->mixed && !g->oDm.S32 && cheb_mfma_ks(ny) == 0) {
+            g->err = "the y derivative has no matrix-core form for this ny";
+            return SIMSON_E_ARG;
+        }
     }
     if (fp64 && !g->TAU && (dalloc(g, &g->TAU, nt) || dalloc(g, &g->DT, nd)))
         return SIMSON_E_HIP;
@@ -488,12 +551,11 @@ void smag_stresses(simson_gpu_t *g)
         launch_narrow(g->TAU, g->TAU32, 6 * field_size(g), g->stream);
         GemmIO io = gemm_io(g, g->TAU32, 0, g->DT32, 0, {1, 3, 4}, {0, 1, 2}, g->nz / 2);
         timed(g, F_CDY, [&] {
-            launch_cheb_mfma_f32(g->Dm32, (const float *)g->TAU32, (float *)g->DT32, io,
-                                 g->stream);
+            launch_cheb_f32(g->oDm, (const float *)g->TAU32, (float *)g->DT32, io, g->stream);
         });
     } else {
         GemmIO io = gemm_io(g, g->TAU, 0, g->DT, 0, {1, 3, 4}, {0, 1, 2}, g->nz / 2);
-        timed(g, F_CDY, [&] { launch_cheb_gemm(g->Dm, io, g->stream); });
+        timed(g, F_CDY, [&] { launch_cheb_gemm(g->oDm, io, g->stream); });
     }
 }
 
@@ -507,15 +569,14 @@ int do_nonlin(simson_gpu_t *g, int do_cfl)
             // Wf = L u of the velocities (linearbl.f:1871-1962), row layout
             GemmIO io = gemm_io(g, g->U32, 0, g->Wf32, 0, {0, 1, 2}, {0, 1, 2}, odd);
             timed(g, F_LES, [&] {
-                launch_cheb_mfma_f32(g->Lm32, (const float *)g->U32, (float *)g->Wf32, io,
-                                     g->stream);
+                launch_cheb_f32(g->oLm, (const float *)g->U32, (float *)g->Wf32, io, g->stream);
             });
             a.les = 1;
             a.Wf = g->Wf32;
         }
         if (g->clos) {
             closure_stresses<float2>(g, a, g->PG32, g->TAU32, g->DT32, g->twz32, g->twx32,
-                                     g->Dm32, true);
+                                     true);
             a.clos = 1;
             a.TAU = g->TAU32;
             a.DT = g->DT32;
@@ -543,13 +604,13 @@ int do_nonlin(simson_gpu_t *g, int do_cfl)
     NonlinArgs a = nonlin_args(g, do_cfl);
     if (g->les) {
         GemmIO io = gemm_io(g, g->U, 0, g->Wf, 0, {0, 1, 2}, {0, 1, 2}, odd);
-        timed(g, F_LES, [&] { launch_cheb_gemm(g->Lm, io, g->stream); });
+        timed(g, F_LES, [&] { launch_cheb_gemm(g->oLm, io, g->stream); });
         a.les = 1;
         a.Wf = g->Wf;
     }
     if (g->clos) {
         closure_stresses<double2>(g, a, g->PG, g->TAU, g->DT, (const double2 *)g->twz,
-                                  (const double2 *)g->twx, g->Dm, false);
+                                  (const double2 *)g->twx, false);
         a.clos = 1;
         a.TAU = g->TAU;
         a.DT = g->DT;
@@ -590,9 +651,9 @@ int do_linear(simson_gpu_t *g, double an, double bn, double anp1, double bnp1)
     GemmIO fw = gemm_io(g, g->U, 0, g->C, 1, {0, 1, 2}, {0, 1, 2}, odd);
     timed(g, F_CHF, [&] {
         if (g->mixed)
-            launch_cheb_mfma_f32(g->Tf32, (const float *)g->U32, (float *)g->C32, fw, g->stream);
+            launch_cheb_f32(g->oTf, (const float *)g->U32, (float *)g->C32, fw, g->stream);
         else
-            launch_cheb_gemm(g->Tf, fw, g->stream);
+            launch_cheb_gemm(g->oTf, fw, g->stream);
     });
     ColArgs ca = col_args(g);
     if (g->wall) {
@@ -613,9 +674,9 @@ int do_linear(simson_gpu_t *g, double an, double bn, double anp1, double bnp1)
     GemmIO bw = gemm_io(g, g->C, 1, g->U, 0, {0, 1, 2, 3, 4}, {0, 1, 2, 3, 4}, odd);
     timed(g, F_CHB, [&] {
         if (g->mixed)
-            launch_cheb_mfma_f32(g->Tb32, (const float *)g->C32, (float *)g->U32, bw, g->stream);
+            launch_cheb_f32(g->oTb, (const float *)g->C32, (float *)g->U32, bw, g->stream);
         else
-            launch_cheb_gemm(g->Tb, bw, g->stream);
+            launch_cheb_gemm(g->oTb, bw, g->stream);
     });
     if (g->mixed) {
         g->u64_stale = true;
@@ -630,11 +691,11 @@ int do_prhs(simson_gpu_t *g, double an)
 {
     refresh_u64(g);
     GemmIO fw = gemm_io(g, g->U, 0, g->C, 1, {0, 1, 2}, {0, 1, 2}, -1);
-    timed(g, F_PCHF, [&] { launch_cheb_gemm(g->Tp, fw, g->stream); });
+    timed(g, F_PCHF, [&] { launch_cheb_gemm(g->oTp, fw, g->stream); });
     ColArgs ca = col_args(g);
     timed(g, F_PRHS, [&] { launch_prhs(ca, an, g->stream); });
     GemmIO bw = gemm_io(g, g->C, 1, g->U, 0, {3, 4}, {3, 4}, -1);
-    timed(g, F_PCHB, [&] { launch_cheb_gemm(g->Tb, bw, g->stream); });
+    timed(g, F_PCHB, [&] { launch_cheb_gemm(g->oTb, bw, g->stream); });
     g->u32_stale = g->mixed;
     CK(hipGetLastError());
     return 0;
@@ -670,7 +731,7 @@ int simson_gpu_init(const simson_params_t *p, simson_gpu_t **out)
         return SIMSON_E_ARG;
     // the FP32 path exists only as the specialised kernels
     if (p->precision == SIMSON_MIXED &&
-        (!has_zfft4(nzp) || !has_xrow4(nxp / 2) || cheb_mfma_ks(p->ny) == 0))
+        (!has_zfft4(nzp) || !has_xrow4(nxp / 2) || !cheb_f32_ok(p->ny)))
         return SIMSON_E_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
@@ -783,6 +844,16 @@ int simson_gpu_init(const simson_params_t *p, simson_gpu_t **out)
             h2d(g, g->twx32, twx32.data(), twx32.size()) ||
             h2d(g, g->Tf32, Tf32.data(), Tf32.size()) || h2d(g, g->Tb32, Tb32.data(), Tb32.size()))
             return fail(SIMSON_E_HIP);
+    }
+    if (int st = make_op(g, g->oTf, Tf, g->Tf, g->Tf32, 0, 1))
+        return fail(st);
+    if (int st = make_op(g, g->oTp, Tp, g->Tp, nullptr, 0, 1))
+        return fail(st);
+    if (int st = make_op(g, g->oTb, Tb, g->Tb, g->Tb32, 1, 0))
+        return fail(st);
+    if (g->mixed && !g->oTf.S32 && cheb_mfma_ks(ny) == 0) {
+        g->err = "no matrix-core transform for this ny";
+        return fail(SIMSON_E_ARG);
     }
     // pressure gradient set-up (bla.f:1622-1662)
     double px;
@@ -928,6 +999,11 @@ int simson_gpu_run(simson_gpu_t *g, int nsteps, double dt, int icfl, int first, 
     const int nst = 3;
     const double dtn = dt, dtnp1 = dt;
     double an = dtn * anrk[0], bn = 0., anp1 = an, bnp1 = 0.;
+    if (!g->cfl_host) {
+        CK(hipHostMalloc((void **)&g->cfl_host, 2 * sizeof(double), hipHostMallocDefault));
+        CK(hipEventCreateWithFlags(&g->cfl_ev[0], hipEventDisableTiming));
+        CK(hipEventCreateWithFlags(&g->cfl_ev[1], hipEventDisableTiming));
+    }
     CK(hipMemsetAsync(g->cfl, 0, 2 * sizeof(double), g->stream));
     if (first) {
         int st = do_prhs(g, anp1);
@@ -935,13 +1011,40 @@ int simson_gpu_run(simson_gpu_t *g, int nsteps, double dt, int icfl, int first, 
             return st;
     }
     static const double cnrk[3] = {0., 8. / 15., 2. / 3.};
+    const double pi = 3.14159265358979323846;
+    // CFL*dt of wcflbl.f:87 from the maximum of boxcfl.f; NaN (a field that
+    // has blown up) reaches here as +infinity
+    auto cfl_dt = [&](double m) { return (m * pi + 2. * std::fabs(g->p.rot)) * dt; };
     const double tstep = g->tnow;
-    std::vector<double> cfls;
+    // The CFL stop of wcflbl.f:93-98 is checked at every evaluation, one
+    // evaluation late: before the substep of evaluation n + 1 is queued the
+    // host waits for the value of evaluation n, so the GPU keeps the
+    // substeps in between queued.  On a stop the run ends after the substep
+    // just before the next evaluation (the remaining substeps of the step
+    // for icfl = 3).
+    int nev = 0, done = 0;
+    double last = 0.;
+    auto check = [&](int ev) -> int {
+        CK(hipEventSynchronize(g->cfl_ev[ev & 1]));
+        last = cfl_dt(g->cfl_host[ev & 1]);
+        return !(last <= 10.);
+    };
+    int stop = 0;
     for (int it = 1; it <= nst * nsteps; it++) {
         const int do_cfl = icfl > 0 && ((it - 1) % icfl) == 0;
+        if (do_cfl && nev > 0 && check(nev - 1)) {
+            stop = 1;
+            break;
+        }
         int st = do_nonlin(g, do_cfl);
         if (st)
             return st;
+        if (do_cfl) {
+            CK(hipMemcpyAsync(g->cfl_host + (nev & 1), g->cfl, sizeof(double),
+                              hipMemcpyDeviceToHost, g->stream));
+            CK(hipEventRecord(g->cfl_ev[nev & 1], g->stream));
+            nev++;
+        }
         an = anp1;
         bn = bnp1;
         anp1 = dtnp1 * anrk[it % nst];
@@ -951,17 +1054,16 @@ int simson_gpu_run(simson_gpu_t *g, int nsteps, double dt, int icfl, int first, 
         st = do_linear(g, an, bn, anp1, bnp1);
         if (st)
             return st;
+        done = it;
     }
-    g->tnow = tstep + dt * (double)nsteps;
-    double c[2] = {0., 0.};
-    CK(hipMemcpyAsync(c, g->cfl, 2 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    // time at the end of the last completed substep
+    g->tnow = tstep + dt * (double)(done / nst) + dt * cnrk[done % nst];
     CK(hipStreamSynchronize(g->stream));
-    const double pi = 3.14159265358979323846;
-    const double last = (c[0] * pi + 2. * std::fabs(g->p.rot)) * dt;   // wcflbl.f:87
-    const double worst = (c[1] * pi + 2. * std::fabs(g->p.rot)) * dt;
+    if (!stop && nev > 0 && check(nev - 1))
+        stop = 1;
     if (cfl_out)
         *cfl_out = last;
-    if (icfl > 0 && worst > 10.)
+    if (icfl > 0 && stop)
         return 324234;                                                  // wcflbl.f:93-98
     return 0;
 }
@@ -1203,7 +1305,7 @@ int setup_smag(simson_gpu_t *g, const simson_les_t *p)
     // (12 KiB) aside
     const size_t lds = (size_t)std::max(65536, lmax) - 12288;
     if (!smag_fits(g->nz, g->nxh, lds) || g->nx % 4 != 0 || !has_zfft4(g->nzp) ||
-        (g->mixed && cheb_mfma_ks(g->ny) == 0)) {
+        (g->mixed && !cheb_f32_ok(g->ny))) {
         g->err = "Smagorinsky: unsupported grid size";
         return SIMSON_E_ARG;
     }
@@ -1282,7 +1384,7 @@ int simson_gpu_les(simson_gpu_t *g, const simson_les_t *p)
                  "supported";
         return SIMSON_E_ARG;
     }
-    if (!has_zfft4(g->nzp) || (g->mixed && cheb_mfma_ks(g->ny) == 0)) {
+    if (!has_zfft4(g->nzp) || (g->mixed && !cheb_f32_ok(g->ny))) {
         g->err = "ADM-RT needs nzp among the specialised z transform lengths";
         return SIMSON_E_ARG;
     }
@@ -1307,6 +1409,12 @@ int simson_gpu_les(simson_gpu_t *g, const simson_les_t *p)
         if (h2d(g, g->Lm32, L32.data(), L32.size()))
             return SIMSON_E_HIP;
     }
+    if (int st = make_op(g, g->oLm, L, g->Lm, g->Lm32, 0, 0))
+        return st;
+    if (g->mixed && !g->oLm.S32 && cheb_mfma_ks(ny) == 0) {
+        g->err = "the ADM-RT low-pass has no matrix-core form for this ny";
+        return SIMSON_E_ARG;
+    }
     g->les_chi = p->chi;
     g->les = true;
     g->clos = false;
@@ -1326,8 +1434,9 @@ int simson_gpu_closure(simson_gpu_t *g, const float *params, const double *in_sc
     }
     if (!in_scale || !in_shift || !yfeat || !out_scale || !out_shift)
         return SIMSON_E_ARG;
-    if (!has_zfft4(g->nzp) || !has_xrow4(g->nxp / 2) || cheb_mfma_ks(g->ny) == 0) {
-        g->err = "the learned closure needs the specialised FFT lengths and ny <= 128";
+    if (!has_zfft4(g->nzp) || !has_xrow4(g->nxp / 2) || !cheb_f32_ok(g->ny)) {
+        g->err = "the learned closure needs the specialised FFT lengths and an ny with "
+                 "matrix-core transforms";
         return SIMSON_E_ARG;
     }
     const int npar = simson_mlp_nparams(), ny = g->ny;
@@ -1392,11 +1501,11 @@ int simson_gpu_pressure(simson_gpu_t *g, double *pr, double *pi)
                       g->stream));
     // Chebyshev analysis, linearp.f per column, synthesis of p
     GemmIO fw = gemm_io(g, g->Q, 0, g->Qc, 1, {0, 1, 2, 3, 4}, {0, 1, 2, 3, 4}, odd);
-    launch_cheb_gemm(g->Tf, fw, g->stream);
+    launch_cheb_gemm(g->oTf, fw, g->stream);
     launch_pressure_cols(g->Qc, g->kfac.gp, g->alfa, g->beta, g->nz, g->nxh, g->ny, g->p.re,
                          (double)g->nxp * (double)g->nzp, g->pscr, g->Qc, g->stream);
     GemmIO bw = gemm_io(g, g->Qc, 1, g->PR, 0, {0}, {0}, odd);
-    launch_cheb_gemm(g->Tb, bw, g->stream);
+    launch_cheb_gemm(g->oTb, bw, g->stream);
     // the z-oddball plane keeps W, as slot 8 does after nonlinp.f
     const size_t plane = (size_t)g->ny * g->nxh;
     CK(hipMemcpyAsync(g->PR + (size_t)odd * plane, g->Q + 3 * F + (size_t)odd * plane,

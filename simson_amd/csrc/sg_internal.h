@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <vector>
 
 #include "simson_gpu.h"
 
@@ -13,6 +14,15 @@ constexpr int kMaxRad = 16;
 constexpr int kNSlot = 7;          // host-visible slots: u v w om_x om_z p_v p_om
 constexpr int kNPhys = 5;          // slots kept at physical y on the device
 constexpr int kScratch = 16;       // per-column scratch arrays of the start-up kernels
+
+// Maximum for the CFL reductions with NaN taken as +infinity: a field that
+// has blown up must fail the CFL check, and fmax alone drops a NaN operand.
+// Once a partial maximum is +infinity, fmax keeps it.
+template <typename T>
+__device__ __forceinline__ T cfl_max(T m, T v)
+{
+    return v != v ? (T)__builtin_inf() : fmax(m, v);
+}
 
 // Radix schedule and twiddle table of one FFT length.
 struct FftPlan {
@@ -209,13 +219,39 @@ void launch_xys(const XysArgs &a, const FftPlan &pz, const FftPlan &pm, const do
 void launch_amp(const double2 *U, const double *alfa, const double *beta, int nz, int ny, int nxh,
                 double *amp, hipStream_t s);
 
-void launch_cheb_gemm(const double *A, const GemmIO &io, hipStream_t s);
-// MFMA form of the transform for ny <= 128; false when ny is outside it
+// A y transform applied to every plane: the dense matrix [ny][ny] (FP64,
+// and FP32 in the mixed form) and, where ny has a parity-split kernel and
+// the matrix the parity structure of the Gauss-Lobatto grid, its packed
+// half-size blocks (sg_cheb.hip)
+struct ChebOp {
+    const double *A = nullptr;
+    const float *A32 = nullptr;
+    const double *S = nullptr;
+    const float *S32 = nullptr;
+    int K = 0;        // split kernel size, ny = 8 (K - 1) + 1; 0: none
+    int flip = 0;     // the operator swaps parities (a y derivative)
+};
+// FP64 transform: split kernel, else the dense MFMA kernel (ny <= 128),
+// else a VALU product
+void launch_cheb_gemm(const ChebOp &op, const GemmIO &io, hipStream_t s);
+// FP32 transform (mixed precision; in/out replace io.in/io.out); false
+// when ny has neither an FP32 split nor a dense MFMA kernel
+bool launch_cheb_f32(const ChebOp &op, const float *in, float *out, const GemmIO &io,
+                     hipStream_t s);
+// MFMA form of the dense transform for ny <= 128; false when ny is outside it
 bool launch_cheb_mfma(const double *A, const GemmIO &io, hipStream_t s);
-// FP32 form (mixed precision): in/out replace io.in/io.out
 bool launch_cheb_mfma_f32(const float *A, const float *in, float *out, const GemmIO &io,
                           hipStream_t s);
 int  cheb_mfma_ks(int ny);
+// parity split (host): packed blocks of A for the given layouts and the
+// kernel size K, or 0 (no split kernel for ny, or no parity structure)
+int  cheb_split(const double *A, int ny, int in_col, int out_col, std::vector<double> &M,
+                int *flip);
+bool launch_cheb_par(const double *M, int K, int flip, const GemmIO &io, hipStream_t s);
+bool launch_cheb_par_f32(const float *M, int K, int flip, const float *in, float *out,
+                         const GemmIO &io, hipStream_t s);
+// an FP32 transform exists for ny (split kernel or dense MFMA kernel)
+bool cheb_f32_ok(int ny);
 // nzp / nxp/2 with a four-step step-2 kernel (needed by the FP32 path)
 bool has_zfft4(int nzp);
 bool has_xrow4(int m);

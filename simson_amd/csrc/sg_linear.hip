@@ -292,8 +292,11 @@ __global__ void __launch_bounds__(256) k_prhs(ColArgs a, double an)
 
 }  // namespace
 
-void launch_cheb_gemm(const double *A, const GemmIO &io, hipStream_t s)
+void launch_cheb_gemm(const ChebOp &op, const GemmIO &io, hipStream_t s)
 {
+    if (op.S && launch_cheb_par(op.S, op.K, op.flip, io, s))
+        return;
+    const double *A = op.A;
     if (launch_cheb_mfma(A, io, s))
         return;
     dim3 grid((2 * io.nxh + GBN - 1) / GBN, (io.ny + GBM - 1) / GBM, io.nf * io.nz);

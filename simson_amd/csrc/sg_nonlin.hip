@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(256) k_xrow(NonlinArgs a, FftPlan pm, const do
         if (a.do_cfl && r < nr) {
             const double ce = fabs(u.x) * dxi + fabs(v.x) * dyinv + fabs(w.x) * dzi;
             const double co = fabs(u.y) * dxi + fabs(v.y) * dyinv + fabs(w.y) * dzi;
-            cflmax = fmax(cflmax, fmax(ce, co));
+            cflmax = cfl_max(cfl_max(cflmax, ce), co);
         }
         double2 h1, h2, h3;
         h1.x = v.x * (oz.x + r2) - w.x * oy.x;
@@ -352,7 +352,7 @@ __global__ void __launch_bounds__(256) k_xrow_t(NonlinArgs a, const double2 *__r
         if (a.do_cfl && r < nr) {
             const double ce = fabs(u.x) * dxi + fabs(v.x) * dyinv + fabs(w.x) * dzi;
             const double co = fabs(u.y) * dxi + fabs(v.y) * dyinv + fabs(w.y) * dzi;
-            cflmax = fmax(cflmax, fmax(ce, co));
+            cflmax = cfl_max(cfl_max(cflmax, ce), co);
         }
         double2 h1, h2, h3;
         h1.x = v.x * (oz.x + r2) - w.x * oy.x;

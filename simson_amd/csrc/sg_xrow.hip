@@ -207,7 +207,7 @@ __global__ void __launch_bounds__((XBlock<V, M>::NT)) k_xrow4(Step2Args<V> a, co
                              fabs(Wm + w[i].x) * dzi;
                 const T co = fabs(Um + u[i].y) * dxi + fabs(v[i].y) * dyinv +
                              fabs(Wm + w[i].y) * dzi;
-                cflmax = fmax(cflmax, fmax(ce, co));
+                cflmax = cfl_max(cfl_max(cflmax, ce), co);
             }
         }
         // H = u x (om + 2 Omega e_z) (advection.f:52-72).  With the mean

@@ -245,3 +245,22 @@ def test_cfl_stop_code():
         with pytest.raises(Exception) as ei:
             g.run(1, 0.5, icfl=3)
         assert getattr(ei.value, "code", None) == 324234
+
+
+def test_nan_field_stops():
+    """A field that holds a NaN fails the CFL check: the CFL maxima take NaN
+    as +infinity (fmax would drop it), so the run returns the stop code of
+    wcflbl.f:93-98 at the first evaluation instead of running on."""
+    say("[gpu] NaN field: CFL stop code")
+    f = c1_field()
+    u = f.u.copy()
+    u[1, 3, 10, 2] = np.nan
+    with BlaGPU.from_ufield(f) as g:
+        g.upload_velocity(u, -2.0 / f.re)
+        with pytest.raises(Exception) as ei:
+            g.run(50, 0.01, icfl=3)
+        assert getattr(ei.value, "code", None) == 324234
+    # the same run with a finite field is not stopped
+    with BlaGPU.from_ufield(f) as g:
+        g.upload_velocity(f.u, -2.0 / f.re)
+        assert g.run(3, 0.01, icfl=3) < 10.0
