@@ -114,6 +114,16 @@ int simson_gpu_download(simson_gpu_t *g, double *ur, double *ui, int nslots,
  * (wampbl.f).  Synchronises. */
 int simson_gpu_amp(simson_gpu_t *g, double *amp);
 
+/* The same plane statistics without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The per-y sums go to a fixed scratch array of the library,
+ * [20][257] doubles (41 KB) per device, instead of the handle's buffer;
+ * ny above 257 returns SIMSON_E_ARG.  All calls on one GPU share that
+ * array, so calls that overlap on one GPU (from several threads or
+ * handles) race; it keeps the last call's contents.  Synchronises. */
+int simson_gpu_amp_noalloc(simson_gpu_t *g, double *amp);
+
 /* x-y statistics of boxxys.f (the per-plane call of bla.f:2089-2105, made
  * at the start of every ixys-th step from t = txys on) for the current
  * velocity state, accumulated into xys, a C array [nxys][ny][nx] whose

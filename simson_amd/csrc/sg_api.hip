@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -11,7 +12,37 @@
 
 using namespace sg;
 
+// Plane statistics of simson_gpu_amp_noalloc: the per-y sums [20][ny] go to
+// this file-scope array instead of the handle's buffer, for callers that
+// cannot allocate device memory.  Each device holds its own copy until the
+// process ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns on that device (amp_scratch_address).
+constexpr int kAmpScratchNy = 257;
+static __device__ __align__(256) double scratch[20 * kAmpScratchNy];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+double *scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on device dev, which must be current; looked
+// up once per device
+hipError_t amp_scratch_address(int dev, double **out)
+{
+    if (dev < 0 || dev >= kMaxDevices)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        scratch_ptr[dev] = (double *)p;
+    }
+    *out = scratch_ptr[dev];
+    return hipSuccess;
+}
 
 const char *const kFamilies[] = {"zsyn", "xrow", "zana", "cfl_reduce", "cheb_fwd",
                                  "tau", "cheb_bwd", "prhs_cheb_fwd", "prhs",
@@ -40,6 +71,7 @@ struct simson_gpu {
     double *W = nullptr, *puw = nullptr, *px = nullptr;
     double *cfl_blk = nullptr, *cfl = nullptr;
     double *amp = nullptr;                // [20][ny] plane statistics
+    double *amp_scratch = nullptr;        // this device's `scratch` (amp_noalloc)
     // pressure work (allocated at the first pressure call): Q rows
     // [5][nz][ny][nxh] (H1, H2, H3, W, v), Qc the same as Chebyshev columns,
     // PR the pressure at physical y, pscr per-column scratch
@@ -754,6 +786,17 @@ int simson_gpu_init(const simson_params_t *p, simson_gpu_t **out)
         g->err = "hipStreamCreate failed";
         return fail(SIMSON_E_HIP);
     }
+    {
+        // this device's copy of the statistics scratch (amp_noalloc)
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess)
+            e = amp_scratch_address(dev, &g->amp_scratch);
+        if (e != hipSuccess) {
+            g->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return fail(SIMSON_E_HIP);
+        }
+    }
     const int nx = g->nx, ny = g->ny, nz = g->nz, nxh = g->nxh;
     const size_t F = (size_t)nz * ny * nxh;
     const size_t ncol = (size_t)nz * nxh;
@@ -1122,6 +1165,24 @@ int simson_gpu_amp(simson_gpu_t *g, double *amp)
     CK(hipGetLastError());
     CK(hipMemcpyAsync(amp, g->amp, 20 * (size_t)g->ny * sizeof(double), hipMemcpyDeviceToHost,
                       g->stream));
+    CK(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int simson_gpu_amp_noalloc(simson_gpu_t *g, double *amp)
+{
+    if (!g || !amp)
+        return SIMSON_E_ARG;
+    if (g->ny > kAmpScratchNy) {
+        g->err = "amp_noalloc: ny above the scratch limit of 257";
+        return SIMSON_E_ARG;
+    }
+    // every entry [20][ny] is written by the kernel, so nothing needs clearing
+    refresh_u64(g);
+    launch_amp(g->U, g->alfa, g->beta, g->nz, g->ny, g->nxh, g->amp_scratch, g->stream);
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(amp, g->amp_scratch, 20 * (size_t)g->ny * sizeof(double),
+                      hipMemcpyDeviceToHost, g->stream));
     CK(hipStreamSynchronize(g->stream));
     return 0;
 }

@@ -26,7 +26,8 @@ NSLOT = 7
 EXPORTS = (
     "simson_gpu_init", "simson_gpu_upload", "simson_gpu_prhs", "simson_gpu_nonlin",
     "simson_gpu_linear", "simson_gpu_substep",
-    "simson_gpu_run", "simson_gpu_download", "simson_gpu_amp", "simson_gpu_xys",
+    "simson_gpu_run", "simson_gpu_download", "simson_gpu_amp", "simson_gpu_amp_noalloc",
+    "simson_gpu_xys",
     "simson_gpu_pressure",
     "simson_gpu_les", "simson_les_filter", "simson_gpu_closure", "simson_gpu_sync",
     "simson_gpu_fringe", "simson_fringe_function", "simson_gpu_wall", "simson_gpu_set_time",
@@ -94,6 +95,7 @@ def lib():
         L.simson_gpu_download.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                           C.POINTER(C.c_double)]
         L.simson_gpu_amp.argtypes = [vp, _dp]
+        L.simson_gpu_amp_noalloc.argtypes = [vp, _dp]
         L.simson_gpu_xys.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int, _dp]
         L.simson_gpu_pressure.argtypes = [vp, _dp, _dp]
         L.simson_gpu_les.argtypes = [vp, C.POINTER(SimsonLes)]
