@@ -79,6 +79,14 @@ int simson_gpu_prhs(simson_gpu_t *g, double an);
  * partial max (synchronises). */
 int simson_gpu_nonlin(simson_gpu_t *g, int do_cfl, double *cfl_out);
 
+/* The boxcfl maximum of the current velocity field without stepping it
+ * (the CFL evaluation of getdt.f that sets the first step of a run with a
+ * variable time step, bla.f:1407-1415): *cfl receives the maximum, so that
+ * the CFL number of a step dt is (*cfl pi + 2 |rot|) dt as wcflbl.f:87
+ * forms it.  A NaN in the field gives +infinity.  The state is unchanged.
+ * Synchronises. */
+int simson_gpu_cfl(simson_gpu_t *g, double *cfl);
+
 /* Step 3 of a substep alone: linearbl over all z planes (bla.f:2433-2475)
  * with the weights an, bn, anp1, bnp1.  px_out (may be NULL) receives the
  * pressure gradient afterwards (synchronises). */

@@ -79,6 +79,7 @@ struct simson_gpu {
     double *pscr = nullptr;
     // tau factorisations: k^2 once, lam^2 for up to three values of a+b
     LamFactor kfac;
+    double *kfT = nullptr;            // the k^2 factors as [2][ny][ncol] planes
     LamFactor lam[3];
     double lam_apb[3] = {0., 0., 0.};
     long lam_used[3] = {0, 0, 0};
@@ -233,7 +234,7 @@ void free_all(simson_gpu_t *g)
 {
     void *ptrs[] = {g->U, g->P, g->C, g->Pp, g->twz, g->twx, g->W, g->puw, g->px,
                     g->cfl_blk, g->cfl, g->amp, g->Tf, g->Tp, g->Tb, g->alfa, g->beta, g->deta,
-                    g->kfac.gp, g->lam[0].gp, g->lam[0].vh, g->lam[0].sh,
+                    g->kfac.gp, g->kfT, g->lam[0].gp, g->lam[0].vh, g->lam[0].sh,
                     g->lam[1].gp, g->lam[1].vh, g->lam[1].sh,
                     g->lam[2].gp, g->lam[2].vh, g->lam[2].sh, g->U32, g->P32, g->C32,
                     g->twz32, g->twx32, g->Tf32, g->Tb32, g->Q, g->Qc, g->PR, g->pscr,
@@ -418,7 +419,7 @@ int lam_slot(simson_gpu_t *g, double apb)
         if (g->lam_used[i] < g->lam_used[v])
             v = i;
     ColArgs ca = col_args(g);
-    timed(g, F_FAC, [&] { launch_factor_lam(ca, apb, g->lam[v], g->kfac, g->stream); });
+    timed(g, F_FAC, [&] { launch_factor_lam(ca, apb, g->lam[v], g->kfT, g->stream); });
     g->lam_apb[v] = apb;
     g->lam_used[v] = g->clock;
     return v;
@@ -804,7 +805,7 @@ int simson_gpu_init(const simson_params_t *p, simson_gpu_t **out)
     if (dalloc(g, &g->U, kNPhys * F) || dalloc(g, &g->P, (size_t)6 * ny * nzp * nxh) ||
         dalloc(g, &g->C, 5 * F) || dalloc(g, &g->Pp, 2 * F) ||
         dalloc(g, &g->W, (size_t)kScratch * ny * ncol) ||
-        dalloc(g, &g->kfac.gp, F) || dalloc(g, &g->lam[0].gp, F) ||
+        dalloc(g, &g->kfac.gp, F) || dalloc(g, &g->kfT, 2 * F) || dalloc(g, &g->lam[0].gp, F) ||
         dalloc(g, &g->lam[1].gp, F) || dalloc(g, &g->lam[2].gp, F) ||
         dalloc(g, &g->lam[0].vh, F) || dalloc(g, &g->lam[1].vh, F) ||
         dalloc(g, &g->lam[2].vh, F) || dalloc(g, &g->lam[0].sh, ncol) ||
@@ -911,7 +912,7 @@ int simson_gpu_init(const simson_params_t *p, simson_gpu_t **out)
         return fail(SIMSON_E_HIP);
     // tau factorisation for lam = k^2 (fixed for the run)
     ColArgs ca = col_args(g);
-    launch_factor_k2(ca, g->kfac, g->stream);
+    launch_factor_k2(ca, g->kfac, g->kfT, g->stream);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(g->stream) != hipSuccess) {
         g->err = "k^2 factorisation failed";
         return fail(SIMSON_E_HIP);
@@ -991,6 +992,35 @@ int simson_gpu_nonlin(simson_gpu_t *g, int do_cfl, double *cfl_out)
         CK(hipStreamSynchronize(g->stream));
         *cfl_out = c[0];
     }
+    return 0;
+}
+
+int simson_gpu_cfl(simson_gpu_t *g, double *cfl)
+{
+    if (!g || !cfl)
+        return SIMSON_E_ARG;
+    // the z synthesis and the x-row pass with the CFL maximum; the z
+    // analysis is left out, so the state (and its storage slots) is
+    // unchanged
+    if (g->mixed) {
+        refresh_u32(g);
+        Step2Args<float2> a = nonlin_args32(g, 1);
+        launch_zsyn4(a, g->twz32, g->stream);
+        launch_xrow4(a, g->twx32, g->stream);
+        launch_cfl_reduce(g->cfl_blk, xrow4_blocks(a), g->cfl, g->stream);
+    } else {
+        NonlinArgs a = nonlin_args(g, 1);
+        if (!launch_zsyn4(a, (const double2 *)g->twz, g->stream))
+            launch_zsyn(a, g->pz, g->stream);
+        const bool four = launch_xrow4(a, (const double2 *)g->twx, g->stream);
+        if (!four)
+            launch_xrow(a, g->pm, g->twx, g->stream);
+        launch_cfl_reduce(g->cfl_blk, four ? xrow4_blocks(a) : xrow_blocks(a), g->cfl,
+                          g->stream);
+    }
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(cfl, g->cfl, sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    CK(hipStreamSynchronize(g->stream));
     return 0;
 }
 

@@ -263,9 +263,11 @@ void launch_mean_synth(const double *Tb, const double2 *C, double2 *U, int ny, i
                        hipStream_t s);
 void launch_prhs(const ColArgs &a, double an, hipStream_t s);
 
-// tau factorisations: k^2 once, lam^2 for one value of a+b
-void launch_factor_k2(const ColArgs &a, LamFactor kf, hipStream_t s);
-void launch_factor_lam(const ColArgs &a, double apb, LamFactor lf, LamFactor kf,
+// tau factorisations: k^2 once (also kept as kfT [2][ny][ncol], gamma and
+// 1/pivot planes), lam^2 for one value of a+b (uses a.W planes 0-4 as work
+// space)
+void launch_factor_k2(const ColArgs &a, LamFactor kf, double *kfT, hipStream_t s);
+void launch_factor_lam(const ColArgs &a, double apb, LamFactor lf, const double *kfT,
                        hipStream_t s);
 // one linear substep, one wavefront per column
 // (fp32_cols: the Chebyshev columns are C32, else C)

@@ -9,7 +9,7 @@
 //   k_factor_*   the tau Helmholtz factorisations (gamma, 1/pivot) per column
 //                for lam = k^2 and for lam^2 = k^2 + 2 Re/(a+b), plus the
 //                homogeneous v_h (setmacrhs.f + trid.f), computed only when
-//                a+b takes a new value;
+//                a+b takes a new value (every substep with a variable dt);
 //   k_prhs       one thread per column: om_x, om_z, p_v, p_omega and puw from
 //                the velocities at start-up.
 #include "sg_internal.h"
@@ -129,23 +129,33 @@ __device__ void dcheb_seq(int n, ColView a, ColView b)
 }
 
 // Tau solve with stored factors (gamma, 1/pivot) per element, see sg_tau.h.
+// (gamma, 1/pivot) of one column as two strided arrays
+struct GpStrided {
+    ColView x, y;
+    __device__ __forceinline__ double g(int k) const { return x[k]; }
+    __device__ __forceinline__ double ip(int k) const { return y[k]; }
+};
+
+template <typename GP>
 __device__ void seq_solve(int n, double lam, ColView f, bool fzero, double bce, double bco,
-                          const double2 *gp, ColView a, ColView rho)
+                          const GP &gp, ColView a, ColView rho)
 {
     for (int c = 0; c < 2; c++) {
         const int klast = ((n - 1 - c) & 1) == 0 ? n - 1 : n - 2;
         double r_up = 0.0;
+#pragma unroll 8
         for (int k = klast; k >= c + 2; k -= 2) {
             const double R = fzero ? 0.0 : tau_rhs(n, k, f[k - 2], f[k],
                                                    k + 2 < n ? f[k + 2] : 0.0);
             const double U = tau_up(n, k, lam);
-            const double rk = (R - U * r_up) * gp[k].y;
+            const double rk = (R - U * r_up) * gp.ip(k);
             rho[k] = rk;
             r_up = rk;
         }
         double P = 0.0, Q = 1.0, sP = 0.0, sQ = 1.0;
+#pragma unroll 8
         for (int k = c + 2; k <= klast; k += 2) {
-            const double g = gp[k].x;
+            const double g = gp.g(k);
             P = rho[k] - g * P;
             Q = -g * Q;
             sP += P;
@@ -153,15 +163,16 @@ __device__ void seq_solve(int n, double lam, ColView f, bool fzero, double bce, 
         }
         double prev = ((c == 0 ? bce : bco) - sP) / sQ;
         a[c] = prev;
+#pragma unroll 8
         for (int k = c + 2; k <= klast; k += 2) {
-            prev = rho[k] - gp[k].x * prev;
+            prev = rho[k] - gp.g(k) * prev;
             a[k] = prev;
         }
     }
 }
 
 // gamma_k, 1/pivot_k for lam (bottom-up elimination of each parity chain)
-__device__ void factor_seq(int n, double lam, double2 *gp)
+__device__ void factor_seq(int n, double lam, ColView gx, ColView gy)
 {
     for (int c = 0; c < 2; c++) {
         const int klast = ((n - 1 - c) & 1) == 0 ? n - 1 : n - 2;
@@ -169,14 +180,22 @@ __device__ void factor_seq(int n, double lam, double2 *gp)
         for (int k = klast; k >= c + 2; k -= 2) {
             const double piv = tau_diag(n, k, lam) - tau_up(n, k, lam) * g_up;
             const double g = tau_low(k, lam) / piv;
-            gp[k] = make_double2(g, 1.0 / piv);
+            gx[k] = g;
+            gy[k] = 1.0 / piv;
             g_up = g;
         }
-        gp[c] = make_double2(0.0, 0.0);
+        gx[c] = 0.0;
+        gy[c] = 0.0;
     }
 }
 
-__global__ void __launch_bounds__(256) k_factor_k2(ColArgs a, LamFactor kf)
+// Both factorisation kernels run one thread per column with every
+// per-column array in a [k][column] layout (stride ncol), so that the
+// lanes of a wave touch consecutive addresses; k_cols_to_rows then writes
+// the tau kernel's [column][k] layout through LDS tiles.
+
+// lam = k^2: the factors in both layouts (kf and kfT [k][column])
+__global__ void __launch_bounds__(256) k_factor_k2(ColArgs a, double *kfT)
 {
     const int ncol = a.nz * a.nxh;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -184,11 +203,14 @@ __global__ void __launch_bounds__(256) k_factor_k2(ColArgs a, LamFactor kf)
         return;
     const int z = t / a.nxh, x = t - z * a.nxh;
     const double k2 = a.alfa[x] * a.alfa[x] + a.beta[z] * a.beta[z];
-    factor_seq(a.ny, k2, kf.gp + (size_t)t * a.ny);
+    const size_t plane = (size_t)a.ny * ncol;
+    factor_seq(a.ny, k2, ColView{kfT + t, (size_t)ncol}, ColView{kfT + plane + t, (size_t)ncol});
 }
 
-__global__ void __launch_bounds__(256) k_factor_lam(ColArgs a, double apb, LamFactor lf,
-                                                    LamFactor kf)
+// lam^2 = k^2 + 2 Re/(a+b): factors and the homogeneous v_h with its parity
+// wall sums, into the scratch W (0-1: factors, 2: phi_h, 3: rho, 4: v_h)
+__global__ void __launch_bounds__(64) k_factor_lam(ColArgs a, double apb, LamFactor lf,
+                                                   const double *kfT)
 {
     const int ncol = a.nz * a.nxh;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -196,21 +218,57 @@ __global__ void __launch_bounds__(256) k_factor_lam(ColArgs a, double apb, LamFa
         return;
     const int z = t / a.nxh, x = t - z * a.nxh;
     const int n = a.ny;
+    const size_t plane = (size_t)n * ncol;
     const double k2 = a.alfa[x] * a.alfa[x] + a.beta[z] * a.beta[z];
     const double lam2 = k2 + 2. * a.re / apb;
-    double2 *gp = lf.gp + (size_t)t * n;
-    factor_seq(n, lam2, gp);
+    auto W = [&](int i) { return ColView{a.W + (size_t)i * plane + t, (size_t)ncol}; };
+    const GpStrided gp{W(0), W(1)};
+    factor_seq(n, lam2, gp.x, gp.y);
     // homogeneous solutions: phi_h (bc 1,1) with lam^2, then v_h (bc 0,0)
     // with k^2 (linearbl.f:625-626, 1000-1001)
-    ColView phh{a.W + t, (size_t)ncol};
-    ColView rho{a.W + (size_t)n * ncol + t, (size_t)ncol};
-    ColView vh{lf.vh + (size_t)t * n, 1};
+    const ColView phh = W(2), rho = W(3), vh = W(4);
     seq_solve(n, lam2, phh, true, 1., 1., gp, phh, rho);
-    seq_solve(n, k2, phh, false, 0., 0., kf.gp + (size_t)t * n, vh, rho);
+    const GpStrided kgp{ColView{const_cast<double *>(kfT) + t, (size_t)ncol},
+                        ColView{const_cast<double *>(kfT) + plane + t, (size_t)ncol}};
+    seq_solve(n, k2, phh, false, 0., 0., kgp, vh, rho);
     double s[2] = {0., 0.};
+#pragma unroll 8
     for (int k = 0; k < n; k++)
         s[k & 1] += (double)k * (double)k * vh[k];
     lf.sh[t] = make_double2(s[0], s[1]);
+}
+
+// [k][column] -> [column][k]: out2 (double2 from the planes a, b) and,
+// when out1 is given, out1 from the plane c
+constexpr int kTT = 32;
+__global__ void __launch_bounds__(256) k_cols_to_rows(const double *__restrict__ pa,
+                                                      const double *__restrict__ pb,
+                                                      const double *__restrict__ pc,
+                                                      double2 *__restrict__ out2,
+                                                      double *__restrict__ out1, int n, int ncol)
+{
+    __shared__ double ta[kTT][kTT + 1], tb[kTT][kTT + 1], tc[kTT][kTT + 1];
+    const int c0 = blockIdx.x * kTT, k0 = blockIdx.y * kTT;
+    const int tx = threadIdx.x & (kTT - 1), ty = threadIdx.x / kTT;   // 32 x 8
+    for (int r = ty; r < kTT; r += 8) {
+        const int k = k0 + r, c = c0 + tx;
+        const bool ok = k < n && c < ncol;
+        const size_t i = (size_t)k * ncol + c;
+        ta[r][tx] = ok ? pa[i] : 0.;
+        tb[r][tx] = ok ? pb[i] : 0.;
+        if (out1)
+            tc[r][tx] = ok ? pc[i] : 0.;
+    }
+    __syncthreads();
+    for (int r = ty; r < kTT; r += 8) {
+        const int c = c0 + r, k = k0 + tx;
+        if (c < ncol && k < n) {
+            const size_t o = (size_t)c * n + k;
+            out2[o] = make_double2(ta[tx][r], tb[tx][r]);
+            if (out1)
+                out1[o] = tc[tx][r];
+        }
+    }
 }
 
 __global__ void __launch_bounds__(256) k_prhs(ColArgs a, double an)
@@ -309,17 +367,28 @@ void launch_prhs(const ColArgs &a, double an, hipStream_t s)
     hipLaunchKernelGGL(k_prhs, dim3((ncol + 255) / 256), dim3(256), 0, s, a, an);
 }
 
-void launch_factor_k2(const ColArgs &a, LamFactor kf, hipStream_t s)
+void launch_factor_k2(const ColArgs &a, LamFactor kf, double *kfT, hipStream_t s)
 {
-    const int ncol = a.nz * a.nxh;
-    hipLaunchKernelGGL(k_factor_k2, dim3((ncol + 255) / 256), dim3(256), 0, s, a, kf);
+    const int ncol = a.nz * a.nxh, n = a.ny;
+    const size_t plane = (size_t)n * ncol;
+    hipLaunchKernelGGL(k_factor_k2, dim3((ncol + 255) / 256), dim3(256), 0, s, a, kfT);
+    dim3 tg((ncol + kTT - 1) / kTT, (n + kTT - 1) / kTT);
+    hipLaunchKernelGGL(k_cols_to_rows, tg, dim3(256), 0, s, kfT, kfT + plane,
+                       (const double *)nullptr, kf.gp, (double *)nullptr, n, ncol);
 }
 
-void launch_factor_lam(const ColArgs &a, double apb, LamFactor lf, LamFactor kf, hipStream_t s)
+void launch_factor_lam(const ColArgs &a, double apb, LamFactor lf, const double *kfT,
+                       hipStream_t s)
 {
-    const int ncol = a.nz * a.nxh;
-    hipLaunchKernelGGL(k_factor_lam, dim3((ncol + 255) / 256), dim3(256), 0, s, a, apb, lf,
-                       kf);
+    const int ncol = a.nz * a.nxh, n = a.ny;
+    const size_t plane = (size_t)n * ncol;
+    // one thread per column in blocks of one wave: the columns are few
+    // (nz nx/2) and each is a latency-bound chain, so they are spread over
+    // as many CUs as possible
+    hipLaunchKernelGGL(k_factor_lam, dim3((ncol + 63) / 64), dim3(64), 0, s, a, apb, lf, kfT);
+    dim3 tg((ncol + kTT - 1) / kTT, (n + kTT - 1) / kTT);
+    hipLaunchKernelGGL(k_cols_to_rows, tg, dim3(256), 0, s, a.W, a.W + plane, a.W + 4 * plane,
+                       lf.gp, lf.vh, n, ncol);
 }
 
 }  // namespace sg

@@ -190,14 +190,26 @@ __global__ void __launch_bounds__(256) k_zana(NonlinArgs a, FftPlan pz, int B)
     }
 }
 
-__global__ void k_cfl_reduce(const double *blk, int n, double *out)
+// one block of kCflT threads; each thread keeps kCflU loads in flight
+constexpr int kCflT = 1024, kCflU = 8;
+
+__global__ void __launch_bounds__(kCflT) k_cfl_reduce(const double *blk, int n, double *out)
 {
     double m = 0.;
-    for (int i = threadIdx.x; i < n; i += blockDim.x)
-        m = fmax(m, blk[i]);
+    for (int i0 = threadIdx.x; i0 < n; i0 += kCflT * kCflU) {
+        double v[kCflU];
+#pragma unroll
+        for (int u = 0; u < kCflU; u++) {
+            const int i = i0 + u * kCflT;
+            v[u] = i < n ? blk[i] : 0.;
+        }
+#pragma unroll
+        for (int u = 0; u < kCflU; u++)
+            m = fmax(m, v[u]);
+    }
     for (int off = 32; off > 0; off >>= 1)
         m = fmax(m, __shfl_xor(m, off));
-    __shared__ double wmax[4];
+    __shared__ double wmax[kCflT / 64];
     if ((threadIdx.x & 63) == 0)
         wmax[threadIdx.x >> 6] = m;
     __syncthreads();
@@ -532,7 +544,7 @@ void launch_zana(const NonlinArgs &a, const FftPlan &pz, hipStream_t s)
 
 void launch_cfl_reduce(const double *blk, int n, double *out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_cfl_reduce, dim3(1), dim3(256), 0, s, blk, n, out);
+    hipLaunchKernelGGL(k_cfl_reduce, dim3(1), dim3(kCflT), 0, s, blk, n, out);
 }
 
 }  // namespace sg

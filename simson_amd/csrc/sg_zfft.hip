@@ -47,7 +47,35 @@ struct ZPlan {
     static constexpr int tasks = B * (N1 > N2 ? N1 : N2);
     static constexpr int NT = tasks >= 256 ? 256 : ((tasks + 63) / 64) * 64;
     static constexpr size_t lds = (size_t)(N + B * SCOL) * sizeof(V);
+    // blocks of consecutive x groups that share one 128-byte line of the
+    // x-contiguous planes
+    static constexpr int G0 = 128 / (B * (int)sizeof(V));
+    static constexpr int G = G0 < 1 ? 1 : (G0 > 8 ? 8 : G0);
 };
+
+// Logical (x group, field, y) of a block.  The hardware hands the block of
+// linear id p to XCD p mod 8, each XCD with its own L2; the G blocks of
+// consecutive x groups that read or write one cache line are given linear
+// ids 8 apart (p = 8 G q + 8 s + xcd -> logical 8 G q + G xcd + s), so
+// that they run on one XCD at about the same time and the line is fetched
+// from memory once.  The last partial chunk keeps the identity mapping.
+template <int G>
+__device__ __forceinline__ void block_xfy(int &bx, int &f, int &y)
+{
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int total = gx * gy * gridDim.z;
+    int l = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    if (G > 1) {
+        constexpr int C = 8 * G;
+        const int q = l / C, r = l - q * C;
+        if ((q + 1) * C <= total)
+            l = q * C + (r & 7) * G + (r >> 3);
+    }
+    bx = l % gx;
+    const int rest = l / gx;
+    f = rest % gy;
+    y = rest / gy;
+}
 
 __device__ __forceinline__ size_t uidx(int slot, int z, int y, int x, int nz, int ny, int nxh)
 {
@@ -99,7 +127,9 @@ __global__ void __launch_bounds__((ZPlan<V, N>::NT)) k_zsyn4(Step2Args<V> a, con
     V *tw = reinterpret_cast<V *>(lds_raw);     // exp(+2 pi i t / N)
     V *A = tw + N;                              // [B][SCOL]
     const int nz = a.nz, nxh = a.nxh, ny = a.ny, tid = threadIdx.x;
-    const int f = blockIdx.y, y = blockIdx.z, x0 = blockIdx.x * B;
+    int bx, f, y;
+    block_xfy<P::G>(bx, f, y);
+    const int x0 = bx * B;
     const int nb = min(B, nxh - x0);
     for (int i = tid; i < N; i += NT)
         tw[i] = twg[i];
@@ -205,7 +235,9 @@ __global__ void __launch_bounds__((ZPlan<V, N>::NT)) k_zana4(Step2Args<V> a, con
     V *tw = reinterpret_cast<V *>(lds_raw);
     V *A = tw + N;
     const int nz = a.nz, nxh = a.nxh, ny = a.ny, tid = threadIdx.x;
-    const int h = blockIdx.y, y = blockIdx.z, x0 = blockIdx.x * B;
+    int bx, h, y;
+    block_xfy<P::G>(bx, h, y);
+    const int x0 = bx * B;
     const int nb = min(B, nxh - x0);
     for (int i = tid; i < N; i += NT)
         tw[i] = twg[i];

@@ -15,6 +15,11 @@
 ! (boxamp.f, nonlinbl.f:430; boxxys.f) and the linearbl plane loop
 ! (bla.f:2433-2475)
 ! through ISO_C_BINDING.
+!
+! Run as `blagpu --pressure` it also writes the pressure file name.p beside
+! every saved field, in the record layout of bla/wdiscp.f, as bla does when
+! par.f sets pressure = 1 (bla.f:2660, 2968): the pressure of the saved
+! state, from nonlinp + linearp (simson_gpu_pressure).
 program blagpu
   use iso_c_binding
   use simson_gpu_mod
@@ -43,6 +48,10 @@ program blagpu
   real(dp), allocatable :: ur(:,:,:,:), ui(:,:,:,:), urx(:)
   ! time stepping (rkcoeff.f:21-32)
   real(dp) :: anrk(3), bnrk(3), dtn, dtnp1, an, bn, anp1, bnp1, cflp, cfl, px
+  real(dp) :: dtmax, cflmax, tleft
+  logical :: vart, do_pressure
+  character(len=32) :: arg
+  real(dp), allocatable :: prr(:,:,:), pri(:,:,:)
   real(dp), allocatable :: amp(:,:), wint(:)
   ! x-y statistics 1-15 of boxxys.f, xys(x, y, i), and the sum of weights
   integer, parameter :: nxys = 15
@@ -55,6 +64,11 @@ program blagpu
   type(c_ptr) :: g
   integer(8) :: c0, c1, crate
 
+  do_pressure = .false.
+  do i = 1, command_argument_count()
+     call get_command_argument(i, arg)
+     if (trim(arg) == '--pressure') do_pressure = .true.
+  end do
   anrk = [8._dp/15._dp, 5._dp/12._dp, 3._dp/4._dp]
   cnrk = [0._dp, 8._dp/15._dp, 2._dp/3._dp]
   bnrk = [0._dp, -17._dp/60._dp, -5._dp/12._dp]
@@ -179,8 +193,7 @@ program blagpu
   close(10)
   if (pert .or. cim .or. suction .or. isfd /= 0 .or. imhd /= 0 .or. &
        loctyp /= 0 .or. tripf .or. (wbci /= 0 .and. wbci /= 1) .or. ibc /= 0 .or. nst /= 3 &
-       .or. mwave > 0 .or. npl > 0 &
-       .or. dt <= 0._dp) then
+       .or. mwave > 0 .or. npl > 0) then
      write(*,*) 'blagpu: bla.i requests a feature outside the channel step library'
      call stopnow(13)
   end if
@@ -188,6 +201,13 @@ program blagpu
   if (icfl > 0 .and. icfl < nst) icfl = nst
   if (icfl > 0) icfl = min(icfl / nst * nst, 3 * nst)
   maxit = maxit / nst * nst
+  ! variable time step (rparambl.f:1996-2017): dt <= 0, with |dt| as the
+  ! largest step when negative; the CFL target scaled by the RK3 stability
+  ! bound sqrt(3)
+  vart = dt <= 0._dp
+  dtmax = 1.e30_dp
+  if (dt < 0._dp) dtmax = -dt
+  cflmax = cflmaxin * sqrt(3._dp)
 
   ! sgs.i (rparamles.f:23-165): only ADM-RT (iles = 1) or no model
   if (sgs) then
@@ -299,6 +319,15 @@ program blagpu
 
   ! ---------------- time stepping (bla.f:1465-1477, 1790-2885) ----------------
   dtn = dt
+  if (vart) then
+     ! first step from the CFL of the initial field (getdt.f, bla.f:1407-1415)
+     call chk(simson_gpu_cfl(g, cflp))
+     dtn = cflmax / (cflp * pi + 2._dp * abs(rot))
+     write(*,*) 'First time step      : ', dtn / dstar
+  end if
+  ! the fringe's stability bound on the step (bla.f:1418-1432)
+  if (spat .and. fr%fmax > 0._dp) dtmax = min(dtmax, 1.75_dp / fr%fmax)
+  dtn = min(dtn, dtmax)
   an = dtn * anrk(1)
   dtnp1 = dtn
   anp1 = an
@@ -343,6 +372,28 @@ program blagpu
      else
         call chk(simson_gpu_nonlin(g, 0_c_int, cflp))
      end if
+     ! the next step's dt from the latest CFL number (bla.f:2355-2380)
+     if (vart .and. mod(it, nst) == 0) then
+        if (tmax > 0._dp) then
+           tleft = min(tmax, tsave(isave)) - t - dtn
+        else
+           tleft = tsave(isave) - t - dtn
+        end if
+        if (cfl / cflmax > 10._dp / 9._dp) then
+           dtnp1 = dtn * cflmax / cfl
+        else
+           dtnp1 = dtn * (0.8_dp + 0.2_dp * cflmax / cfl)
+        end if
+        dtnp1 = min(dtnp1, dtmax)
+        if (.5_dp * dtnp1 < tleft .and. tleft < 4._dp * dtnp1) then
+           if (tmax > 0._dp) then
+              write(*,*) 'Adjusting time step to ', min(tmax, tsave(isave)) / dstar
+           else
+              write(*,*) 'Adjusting time step to ', tsave(isave) / dstar
+           end if
+           dtnp1 = tleft / aint(tleft / dtnp1 + 1._dp)
+        end if
+     end if
      an = anp1
      bn = bnp1
      anp1 = dtnp1 * anrk(mod(it, nst) + 1)
@@ -355,6 +406,7 @@ program blagpu
         t = t + dtn
         xs = xs - dtn * u0low                        ! bla.f:2606-2609
         zs = zs - dtn * w0low
+        if (vart) dtn = dtnp1
         if (t + .5_dp * dtn > tsave(isave)) then
            call save_field(trim(nmsave(isave)))
            isave = isave + 1
@@ -506,6 +558,29 @@ contains
     end do
     close(11)
     write(*,*) '** Write velocity field ** ', trim(name), t / dstar
+    if (do_pressure) call save_pressure(name // '.p')
   end subroutine save_field
+
+  ! wdiscp.f: the header of the velocity file, then slot 8 (the pressure at
+  ! physical y) as one record per (z, y) of nx/2 complex values
+  subroutine save_pressure(name)
+    character(len=*), intent(in) :: name
+    integer :: z, y
+    if (.not. allocated(prr)) allocate(prr(nx / 2, nyp, nzc), pri(nx / 2, nyp, nzc))
+    call chk(simson_gpu_pressure(g, prr, pri))
+    open(11, file=name, form='unformatted', status='replace')
+    write(11) re, .false., xl, zl, t, 0._dp
+    write(11) nx, nyp, nzc, nfzsym
+    write(11) fltype, dstar
+    if (fltype == 4 .or. fltype == 5) write(11) bstart, blength
+    do z = 1, nzc
+       do y = 1, nyp
+          urx(1:nx:2) = prr(:, y, z)
+          urx(2:nx:2) = pri(:, y, z)
+          write(11) urx
+       end do
+    end do
+    close(11)
+  end subroutine save_pressure
 
 end program blagpu

@@ -72,6 +72,12 @@ module simson_gpu_mod
        real(c_double), intent(out) :: cfl_out
      end function simson_gpu_nonlin
 
+     integer(c_int) function simson_gpu_cfl(g, cfl) bind(c, name='simson_gpu_cfl')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: g
+       real(c_double), intent(out) :: cfl
+     end function simson_gpu_cfl
+
      integer(c_int) function simson_gpu_linear(g, an, bn, anp1, bnp1, px_out) &
           bind(c, name='simson_gpu_linear')
        import :: c_int, c_ptr, c_double
@@ -95,6 +101,13 @@ module simson_gpu_mod
        type(c_ptr), value :: g
        real(c_double), intent(out) :: amp(*)
      end function simson_gpu_amp
+
+     integer(c_int) function simson_gpu_pressure(g, pr, pim) &
+          bind(c, name='simson_gpu_pressure')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: g
+       real(c_double), intent(out) :: pr(*), pim(*)
+     end function simson_gpu_pressure
 
      integer(c_int) function simson_gpu_xys(g, dtn, xs, zs, nxys, xys) &
           bind(c, name='simson_gpu_xys')

@@ -26,7 +26,7 @@ NSLOT = 7
 EXPORTS = (
     "simson_gpu_init", "simson_gpu_upload", "simson_gpu_prhs", "simson_gpu_nonlin",
     "simson_gpu_linear", "simson_gpu_substep",
-    "simson_gpu_run", "simson_gpu_download", "simson_gpu_amp", "simson_gpu_amp_noalloc",
+    "simson_gpu_run", "simson_gpu_cfl", "simson_gpu_download", "simson_gpu_amp", "simson_gpu_amp_noalloc",
     "simson_gpu_xys",
     "simson_gpu_pressure",
     "simson_gpu_les", "simson_les_filter", "simson_gpu_closure", "simson_gpu_sync",
@@ -94,6 +94,7 @@ def lib():
                                      C.POINTER(C.c_double)]
         L.simson_gpu_download.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                           C.POINTER(C.c_double)]
+        L.simson_gpu_cfl.argtypes = [vp, C.POINTER(C.c_double)]
         L.simson_gpu_amp.argtypes = [vp, _dp]
         L.simson_gpu_amp_noalloc.argtypes = [vp, _dp]
         L.simson_gpu_xys.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int, _dp]
@@ -220,6 +221,13 @@ class BlaGPU:
         cfl = C.c_double(0.0)
         self._check(lib().simson_gpu_run(self.h, nsteps, dt, icfl, int(first), C.byref(cfl)))
         return cfl.value
+
+    def cfl(self) -> float:
+        """The boxcfl maximum of the current field (getdt.f), without
+        stepping it; the CFL number of a step dt is (cfl pi + 2|rot|) dt."""
+        c = C.c_double(0.0)
+        self._check(lib().simson_gpu_cfl(self.h, C.byref(c)))
+        return c.value
 
     def sync(self):
         self._check(lib().simson_gpu_sync(self.h))

@@ -10,6 +10,7 @@ Runs in the build container only (it needs /root/reference and amdflang):
     python tests/golden/make_golden.py smag       # smag_32x33x32.npz
     python tests/golden/make_golden.py fringe     # fringe_32x33x32.npz
     python tests/golden/make_golden.py wall       # wall_32x33x32.npz
+    python tests/golden/make_golden.py vart       # vart_32x33x32.npz
 
 1. builds the reference `bla` for 32x33x32 (serial, pressure off) with
    oracle/build_ref.sh into oracle/_ref/;
@@ -387,6 +388,46 @@ def wall():
     print("wrote", os.path.join(HERE, "wall_32x33x32.npz"), sorted(res))
 
 
+def vart():
+    """Variable time step fixture (bla.i dt = 0, rparambl.f:1996-2017): the
+    C1 field, cflmaxin 0.8, 20 steps, a save at t = 1 (which makes bla.f
+    shorten the steps before it, bla.f:2355-2380), the CFL evaluated every
+    full step.  Keys: u_save (the field at t = 1), u_end (after 20 steps), cfl
+    and dt (every CFL*dt log line: the CFL number and the step)."""
+    if not os.path.exists(BIN):
+        subprocess.check_call([os.path.join(REPO, "oracle", "build_ref.sh"),
+                               str(NX), str(NY), str(NZ), "1", "no", "0",
+                               os.path.basename(BIN)])
+    work = tempfile.mkdtemp(prefix="simson_golden_v.")
+    try:
+        ufile.write_u(os.path.join(work, "init.u"), c1_field())
+        p = blai.BlaParams(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=60,
+                           dt=0.0, nst=3, cflmaxin=0.8, xl=4 * math.pi, cflux=True, icfl=3,
+                           tsave=[1.0], nmsave=["t1.u"])
+        with open(os.path.join(work, "bla.i"), "w") as f:
+            f.write(blai.format_channel(p))
+        with open(os.path.join(work, "out.txt"), "w") as out:
+            subprocess.check_call([BIN], cwd=work, stdout=out, stderr=subprocess.STDOUT)
+        log = open(os.path.join(work, "out.txt")).read()
+        if "bla terminated normally" not in log:
+            raise RuntimeError("reference bla (vart) did not terminate normally")
+        pairs = re.findall(r"CFL\*dt\s+([0-9.Ee+-]+)\s+dt\s+([0-9.Ee+-]+)", log)
+        first = re.search(r"First time step\s*:\s*([0-9.Ee+-]+)", log)
+        sha = hashlib.sha256(open(os.path.join(work, "init.u"), "rb").read()).hexdigest()
+        np.savez_compressed(os.path.join(HERE, "vart_32x33x32.npz"),
+                            u_save=ufile.read_u(os.path.join(work, "t1.u")).u,
+                            t_save=np.array(ufile.read_u(os.path.join(work, "t1.u")).t),
+                            u_end=ufile.read_u(os.path.join(work, "end.u")).u,
+                            t_end=np.array(ufile.read_u(os.path.join(work, "end.u")).t),
+                            cfl=np.array([float(a) for a, _ in pairs]),
+                            dt=np.array([float(b) for _, b in pairs]),
+                            dt_first=np.array(float(first.group(1))),
+                            init_sha256=np.array(sha))
+        print("wrote", os.path.join(HERE, "vart_32x33x32.npz"), "lines", len(pairs))
+    finally:
+        shutil.rmtree(work)
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "pressure":
         pressure()
@@ -402,5 +443,7 @@ if __name__ == "__main__":
         wall()
     elif len(sys.argv) > 1 and sys.argv[1] == "long":
         long()
+    elif len(sys.argv) > 1 and sys.argv[1] == "vart":
+        vart()
     else:
         main()

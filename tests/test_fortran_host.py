@@ -330,3 +330,68 @@ def test_fortran_host_wall_matches_reference_bla():
         np.testing.assert_allclose(cfl, golden["a_cfl"], rtol=1e-10)
     finally:
         shutil.rmtree(work)
+
+
+@pytest.mark.gpu
+def test_fortran_host_variable_dt_matches_reference_bla():
+    """bla.i dt = 0 (variable time step, rparambl.f:1996-2017): the first
+    step from the CFL of the initial field (getdt.f), then each step from
+    the latest CFL number, shortened before the save at t = 1
+    (bla.f:2355-2380).  The host's dt sequence, CFL lines and fields equal
+    the reference run's (tests/golden/vart_32x33x32.npz): dt to 1e-12
+    relative (it is a rational function of the CFL maximum), fields 1e-12."""
+    _need_binary()
+    print("[gpu] blagpu with a variable time step 32x33x32", flush=True)
+    golden = np.load(os.path.join(ROOT, "tests", "golden", "vart_32x33x32.npz"))
+    work = _c1_dir(maxit=60, dt=0.0, cflmaxin=0.8, iamp=0, tsave=[1.0], nmsave=["t1.u"])
+    try:
+        r = _run(work)
+        assert r.returncode == 0, r.stdout + r.stderr
+        pairs = re.findall(r"CFL\*dt\s+([0-9.Ee+-]+)\s+dt\s+([0-9.Ee+-]+)", r.stdout)
+        dt = np.array([float(b) for _, b in pairs])
+        cfl = np.array([float(a) for a, _ in pairs])
+        first = float(re.search(r"First time step\s*:\s*([0-9.Ee+-]+)", r.stdout).group(1))
+        print(f"    first dt {first!r} (reference {float(golden['dt_first'])!r})", flush=True)
+        assert abs(first - float(golden["dt_first"])) <= 1e-12 * float(golden["dt_first"])
+        assert len(dt) == len(golden["dt"])
+        err_dt = float(np.abs(dt - golden["dt"]).max() / golden["dt"].max())
+        err_cfl = float(np.abs(cfl - golden["cfl"]).max() / golden["cfl"].max())
+        print(f"    dt sequence {err_dt:.1e}, CFL*dt {err_cfl:.1e}", flush=True)
+        assert err_dt <= 1e-12 and err_cfl <= 1e-12
+        for name, key, tkey in (("t1.u", "u_save", "t_save"), ("end.u", "u_end", "t_end")):
+            uf = ufile.read_u(os.path.join(work, name))
+            assert abs(uf.t - float(golden[tkey])) <= 1e-12 * float(golden[tkey])
+            u = np.delete(uf.u, 16, axis=1)
+            rr = np.delete(golden[key], 16, axis=1)
+            for c in range(3):
+                err = _rel(u[c], rr[c])
+                print(f"    {name} comp {c}: rel L2 {err:.2e}", flush=True)
+                assert err <= 1e-12, (name, c, err)
+    finally:
+        shutil.rmtree(work)
+
+
+@pytest.mark.gpu
+def test_fortran_host_writes_pressure_files():
+    """`blagpu --pressure` writes name.p beside every saved field (wdiscp.f's
+    layout, what bla writes with par.f pressure = 1): the t1.u.p and t10.u.p
+    of the C1 case equal the reference's (tests/golden/p_32x33x32.npz) to
+    1e-9, the pressure bar (its mean mode is an O(1e-5) balance of O(1)
+    terms)."""
+    _need_binary()
+    print("[gpu] blagpu --pressure 32x33x32", flush=True)
+    golden = np.load(os.path.join(ROOT, "tests", "golden", "p_32x33x32.npz"))
+    work = _c1_dir(maxit=30, iamp=0)
+    try:
+        r = subprocess.run([BLAGPU, "--pressure"], cwd=work, capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        for name, key in (("t1.u.p", "p1"), ("t10.u.p", "p10")):
+            pf = ufile.read_u(os.path.join(work, name), ncomp=1)
+            assert (pf.nx, pf.nyp, pf.nz) == (32, 33, 32)
+            err = _rel(np.delete(pf.u[0], 16, axis=0), np.delete(golden[key], 16, axis=0))
+            print(f"    {name}: rel L2 {err:.2e}", flush=True)
+            assert err <= 1e-9, (name, err)
+        assert os.path.exists(os.path.join(work, "end.u.p"))
+    finally:
+        shutil.rmtree(work)
