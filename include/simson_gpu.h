@@ -41,13 +41,16 @@ extern "C" {
  * RK3-CN partial right-hand sides p_v, p_omega, puw stay FP64.  Uploads,
  * prhs and downloads are FP64 (downloaded velocities carry FP32 rounding).
  * Requires nzp = 3nz/2 and nxp/2 = 3nx/4 among the specialised FFT lengths
- * and ny <= 127. */
+ * and ny among the matrix-core transform sizes (ny <= 127 with
+ * ny = 1 mod 2, or ny = 8K + 1 up to 257 with the parity-split form). */
 #define SIMSON_MIXED 1
 
 typedef struct simson_params {
     int nx, ny, nz;      /* par.f nx, ny (= nyp, nfyd = 0), nz; nfxd = nfzd = 1 */
     int precision;       /* SIMSON_FP64 or SIMSON_MIXED */
-    int ibc;             /* bla.i ibc; only 0 (no-slip walls) */
+    int ibc;             /* bla.i ibc: 0 (no-slip walls) or 101 (free
+                            stream du_i/dy = dU_i/dy above, no-slip below;
+                            needs cflux = 0, see simson_gpu_free_stream) */
     int cflux;           /* bla.i cflux (constant mass flux) */
     double re;           /* Reynolds number from the .u header */
     double xl, zl;       /* box lengths (bla.i xl, .u zl) */
@@ -153,9 +156,8 @@ int simson_gpu_xys(simson_gpu_t *g, double dtn, double xs, double zs, int nxys, 
  * wdiscp.f writes to <name>.p.  pr, pi receive p at physical y as C arrays
  * [z][y][x]; the z-oddball plane holds the transform of u.u there, as in
  * the reference.  The wall term -(v - v_prev)/dt of linearp.f is zero for the
- * no-slip walls this library supports.  Needs nzp and nxp/2 among the
- * specialised FFT lengths.  Allocates its work buffers at the first call;
- * synchronises. */
+ * no-slip walls.  Needs ibc = 0 and nzp and nxp/2 among the specialised FFT
+ * lengths.  Allocates its work buffers at the first call; synchronises. */
 int simson_gpu_pressure(simson_gpu_t *g, double *pr, double *pi);
 
 /* Large-eddy models of bla when bla.i sets sgs (sgs.i, rparamles.f:62-81):
@@ -243,6 +245,54 @@ int simson_gpu_wall(simson_gpu_t *g, const simson_wall_t *w);
  * SIMSON_E_ARG. */
 int simson_wall_velocity(int nx, int nz, double xl, double zl, const simson_wall_t *w,
                          double tc, double *vr, double *vi);
+
+/* Falkner-Skan-Cooke similarity table (bls/fsc.f; the fsc.dat that rbla.f
+ * reads): on eta_i = ymax i/(n-1), i < n, the rows
+ *   tab[8 i ..] = eta, f, f', f'', f''', g, g', g''
+ * of f''' = -f f'' - beta (1 - f'^2), beta = 2m/(m+1), f(0) = f'(0) = 0,
+ * f'(ymax) = 1 (shooting with Newton's method to |f'(ymax) - 1| <= eps, one
+ * RK4 step per interval as fsc.f), and g'' = -f g', g(0) = 0, g(ymax) = 1.
+ * dstar2 = ymax - f(ymax), the displacement thickness in eta.  Host only;
+ * returns 0 or SIMSON_E_ARG (bad input, m < -0.0904 or no convergence). */
+int simson_fsc_table(double m, int n, double ymax, double eps, double *tab, double *dstar2);
+
+/* Spatial boundary-layer base flow of cbflow.f:161-300 (fltype 6 Blasius,
+ * 7 Falkner-Skan with m = rlam, 8 Falkner-Skan-Cooke with spanv), lengths
+ * in the units of the .u file (re and xl as stored there, dstar = 2/yl). */
+typedef struct simson_baseflow {
+    int nx, ny;          /* par.f nx, ny */
+    int fltype;          /* 6, 7 or 8 */
+    double xl, re;       /* box length and Reynolds number of the .u file */
+    double dstar;        /* .u dstar: displacement thickness at the box
+                            centre in the units of y + 1 */
+    double dstar2;       /* the table's displacement thickness in eta */
+    double bstart, blength;   /* .u blending zone */
+    double rlam, spanv;  /* m and the spanwise scale (fltype 7, 8) */
+    double xsc;          /* frame position (0 for a fixed box) */
+    double u0low, w0low; /* lower-wall velocities */
+} simson_baseflow_t;
+
+/* The base flow base [3][ny][3nx/2] (u, v, w at x_j = -xl/2 + j xl/nxp +
+ * xsc and eta_y = cos(pi y/(ny-1)): the layout simson_gpu_fringe takes),
+ * from the similarity table tab [ntab][8] of simson_fsc_table (or fsc.dat);
+ * v from continuity with v(-1) = 0.  With bjr/bji [3][nx/2] and du0upp not
+ * NULL, also the data of simson_gpu_free_stream: dU_i/dy at y = +1 as x
+ * Fourier coefficients / nxp (bflow.f:80-105) and their kx = 0 streamwise
+ * value.  Host only; returns 0 or SIMSON_E_ARG. */
+int simson_base_flow(const simson_baseflow_t *b, const double *tab, int ntab, double *base,
+                     double *bjr, double *bji, double *du0upp);
+
+/* Free-stream data of ibc = 101 (replaces evalbc.f:142-159's use of
+ * bflow.f's bu1jr and du0upp): bjr/bji [3][nx/2], component-major, the
+ * base flow's dU/dy, dV/dy, dW/dy at y = +1 per kx (Fourier coefficients
+ * normalised by 1/nxp, bflow.f:80-105), imposed on the kz = 0 plane, or both
+ * NULL for zero; du0upp the kx = 0 value of dU/dy there, the mean mode's
+ * condition (linearbl.f:1516-1531).  With ibc = 101 the mean mode's wall
+ * values are U = 1 above (then replaced by du0upp) and 0 below, as bla.f:
+ * 1360-1365 sets them for every base flow.  Until called, b = 0 and
+ * du0upp = 0.  Returns SIMSON_E_ARG unless init had ibc = 101.
+ * Synchronises. */
+int simson_gpu_free_stream(simson_gpu_t *g, const double *bjr, const double *bji, double du0upp);
 
 /* The time t of the start of the next substep, the tc of bla.f:1797 that
  * time-dependent boundary forcing uses (0 after init).  simson_gpu_run

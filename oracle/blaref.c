@@ -351,6 +351,9 @@ struct orc {
     double *fr, *fi;                   /* [3][nz][ny][nxh] external force or NULL */
     double *flam, *fbase;              /* fringe [nxp] and base flow [3][ny][nxp] or NULL */
     double *wvr, *wvi;                 /* lower-wall v [nz][nxh] (spectral) or NULL */
+    int ibc;                           /* 0 (no-slip) or 101 (free stream above) */
+    double *bjr, *bji;                 /* [3][nxh] dU_i/dy at y = 1 of the base flow */
+    double du0upp;                     /* its mean (kx = 0) streamwise part */
 };
 
 struct les_s;
@@ -421,6 +424,8 @@ void orc_destroy(orc_t *o)
     free(o->fbase);
     free(o->wvr);
     free(o->wvi);
+    free(o->bjr);
+    free(o->bji);
     free(o->gur);
     free(o->gui);
     free(o);
@@ -625,6 +630,93 @@ static double mass_flux(int n, const double *u)
     return m;
 }
 
+/* ibc = 101 (evald.f, evalbc.f:142-159, solvbc.f, linearbl.f:946-1150):
+ * the upper boundary y = +1 is a free stream, du_i/dy = dU_i/dy of the base
+ * flow there (the kz = 0 plane; zero elsewhere), the lower one the no-slip
+ * wall.  Three homogeneous solutions that vanish with their slope at the
+ * wall are added to the particular one (v_p, om_p, which has v = dv/dy = 0
+ * at both walls):
+ *   vh1 = v_h with each parity class scaled to unit slope at +1
+ *         (dvh1/dy(+1) = 2), so vh1(+-1) = 0, dvh1/dy(-1) = 0;
+ *   vh2 = the k^2 Helmholtz solution with v = 1 per class (v(+1) = 2,
+ *         v(-1) = 0), its wall slopes cancelled with v_h;
+ *   omh = phi_h, the lam^2 solution with 1 per class (om(+1) = 2).
+ * By continuity and the definition of omega_y, du/dy and dw/dy at +1 fix
+ *   d2v/dy2 = -i (alpha U' + beta W'),  d om/dy = i (beta U' - alpha W'),
+ * and dv/dy = V' directly, so v -= c1 vh1 + c2 vh2, om -= c3 omh with
+ *   c1 = -V'/2, c2 = (d2v_p - c1 A1 - d2v*)/A2, c3 = (dom_p - dom*)/B,
+ * A1, A2 = d2/dy2 of vh1, vh2 at +1 and B = domh/dy there.  The reference
+ * run of tests/golden/blasius_64x33x8.npz pins this form (to 1e-13 after one
+ * step). */
+static void free_stream_bc(orc_t *o, int z, int x, double alfa, double beta, double *vr,
+                           double *vi, double *omr, double *omi, const double *phh,
+                           const double *vh, const double *Sh, work_t *W)
+{
+    int n = o->ny;
+    double *vh1 = W->v[43], *vh2 = W->v[44], *t1 = W->v[45], *t2 = W->v[46];
+    double *zero = W->v[39];
+    double k2 = alfa * alfa + beta * beta;
+    /* vh2 and its wall slopes cancelled with v_h, per class */
+    tau_solve(n, k2, zero, 1., 1., vh2, W->gam, W->rho);
+    dcheb(n, vh2, t1);
+    double S2[2] = {0., 0.};
+    for (int k = 0; k < n; k++)
+        S2[k & 1] += t1[k];
+    for (int k = 0; k < n; k++) {
+        int c = k & 1;
+        vh2[k] += (-S2[1 - c] / Sh[1 - c]) * vh[k];
+        vh1[k] = vh[k] * (1. / Sh[1 - c]);
+    }
+    /* derivatives at y = +1: sums of the derivative coefficients */
+    double A1 = 0., A2 = 0., B = 0., d2r = 0., d2i = 0., d1r = 0., d1i = 0.;
+    dcheb(n, vh1, t1);
+    dcheb(n, t1, t2);
+    for (int k = 0; k < n; k++)
+        A1 += t2[k];
+    dcheb(n, vh2, t1);
+    dcheb(n, t1, t2);
+    for (int k = 0; k < n; k++)
+        A2 += t2[k];
+    dcheb(n, phh, t1);
+    for (int k = 0; k < n; k++)
+        B += t1[k];
+    dcheb(n, vr, t1);
+    dcheb(n, t1, t2);
+    for (int k = 0; k < n; k++)
+        d2r += t2[k];
+    dcheb(n, vi, t1);
+    dcheb(n, t1, t2);
+    for (int k = 0; k < n; k++)
+        d2i += t2[k];
+    dcheb(n, omr, t1);
+    for (int k = 0; k < n; k++)
+        d1r += t1[k];
+    dcheb(n, omi, t1);
+    for (int k = 0; k < n; k++)
+        d1i += t1[k];
+    /* targets from the base flow's U', V', W' (kz = 0 plane) */
+    double Ur = 0., Ui = 0., Vr = 0., Vi = 0., Wr = 0., Wi = 0.;
+    if (z == 0 && o->bjr) {
+        Ur = o->bjr[x];
+        Ui = o->bji[x];
+        Vr = o->bjr[o->nxh + x];
+        Vi = o->bji[o->nxh + x];
+        Wr = o->bjr[2 * o->nxh + x];
+        Wi = o->bji[2 * o->nxh + x];
+    }
+    double sr = alfa * Ur + beta * Wr, si = alfa * Ui + beta * Wi;   /* d2v* = -i s */
+    double tr = beta * Ur - alfa * Wr, ti = beta * Ui - alfa * Wi;   /* dom* = i t */
+    double c1r = -.5 * Vr, c1i = -.5 * Vi;
+    double c2r = (d2r - c1r * A1 - si) / A2, c2i = (d2i - c1i * A1 + sr) / A2;
+    double c3r = (d1r + ti) / B, c3i = (d1i - tr) / B;
+    for (int k = 0; k < n; k++) {
+        vr[k] -= c1r * vh1[k] + c2r * vh2[k];
+        vi[k] -= c1i * vh1[k] + c2i * vh2[k];
+        omr[k] -= c3r * phh[k];
+        omi[k] -= c3i * phh[k];
+    }
+}
+
 static void linear_plane(orc_t *o, double *ur, double *ui, double *puw, int z,
                          double an, double bn, double anp1, double bnp1, work_t *W)
 {
@@ -746,6 +838,8 @@ static void linear_plane(orc_t *o, double *ur, double *ui, double *puw, int z,
         /* ibc = 0: the particular solutions satisfy u=v=w=0 at the walls,
          * so the 3x3 boundary system (evald/evalbc/solvbc) has a zero RHS
          * and adds nothing. */
+        if (o->ibc == 101 && !k0)
+            free_stream_bc(o, z, x, alfa, beta, vr, vi, omr, omi, phh, vh, Sh, W);
         /* dv, phi = D^2 v - k^2 v (linearbl.f:1151-1166), d omega */
         dcheb(n, vr, dvr);
         dcheb(n, dvr, tmp);
@@ -771,8 +865,15 @@ static void linear_plane(orc_t *o, double *ur, double *ui, double *puw, int z,
         if (k0) {
             /* mean flow (linearbl.f:1216-1692, tau branch, ibc = 0) */
             double *u0 = W->v[43], *w0 = W->v[44], *du0 = W->v[45], *dw0 = W->v[46];
-            double bu_e = .5 * (o->u0upp + o->u0low), bu_o = .5 * (o->u0upp - o->u0low);
-            double bw_e = .5 * (o->w0upp + o->w0low), bw_o = .5 * (o->w0upp - o->w0low);
+            double u0upp = o->u0upp, u0low = o->u0low, w0upp = o->w0upp, w0low = o->w0low;
+            if (o->ibc == 101) {
+                /* with a base flow: U = 1 above (replaced by the free-stream
+                 * condition below), 0 below (bla.f:1360-1365) */
+                u0upp = 1.;
+                u0low = w0upp = w0low = 0.;
+            }
+            double bu_e = .5 * (u0upp + u0low), bu_o = .5 * (u0upp - u0low);
+            double bw_e = .5 * (w0upp + w0low), bw_o = .5 * (w0upp - w0low);
             tau_solve(n, lam2, fuw1, bu_e, bu_o, u0, W->gam, W->rho);
             tau_solve(n, lam2, fuw2, bw_e, bw_o, w0, W->gam, W->rho);
             if (o->cflux) {
@@ -785,6 +886,27 @@ static void linear_plane(orc_t *o, double *ur, double *ui, double *puw, int z,
                 o->px = o->px * (1. + (o->mflux - m1) / (m2 - m1));
                 fuw1[0] += 2. * re * o->px;
                 tau_solve(n, lam2, fuw1, bu_e, bu_o, u0, W->gam, W->rho);
+            }
+            if (o->ibc == 101) {
+                /* free stream above (linearbl.f:1473-1487): add the lam^2
+                 * homogeneous solution phi_h (1 per class: 2 at y = +1, 0 at
+                 * the wall) so that du0/dy(+1) = du0upp, dw0/dy(+1) = 0 */
+                double *t1 = W->v[42];
+                double d0u = 0., d0w = 0., dh = 0.;
+                dcheb(n, u0, t1);
+                for (int y = 0; y < n; y++)
+                    d0u += t1[y];
+                dcheb(n, w0, t1);
+                for (int y = 0; y < n; y++)
+                    d0w += t1[y];
+                dcheb(n, phh, t1);
+                for (int y = 0; y < n; y++)
+                    dh += t1[y];
+                double cu = (d0u - o->du0upp) / dh, cw = d0w / dh;
+                for (int y = 0; y < n; y++) {
+                    u0[y] -= cu * phh[y];
+                    w0[y] -= cw * phh[y];
+                }
             }
             dcheb(n, u0, du0);
             dcheb(n, w0, dw0);
@@ -1425,4 +1547,23 @@ static void les_gu(orc_t *o, const double *ur, const double *ui)
                     }
                 }
             }
+}
+
+int orc_set_bc(orc_t *o, int ibc, const double *bjr, const double *bji, double du0upp)
+{
+    if (ibc != 0 && ibc != 101)
+        return -1;
+    o->ibc = ibc;
+    o->du0upp = du0upp;
+    free(o->bjr);
+    free(o->bji);
+    o->bjr = o->bji = NULL;
+    if (bjr) {
+        size_t m = 3 * (size_t)o->nxh;
+        o->bjr = malloc(sizeof(double) * m);
+        o->bji = malloc(sizeof(double) * m);
+        memcpy(o->bjr, bjr, sizeof(double) * m);
+        memcpy(o->bji, bji, sizeof(double) * m);
+    }
+    return 0;
 }

@@ -79,6 +79,13 @@ void orc_set_fringe(orc_t *o, const double *lam, const double *base);
  * orc_linear calls; NULL: off. */
 void orc_set_wall(orc_t *o, const double *vr, const double *vi);
 
+/* Boundary conditions at y = +1 (evalbc.f): ibc 0 (no-slip, the default) or
+ * 101 (free stream: du_i/dy = dU_i/dy of the base flow; the lower wall stays
+ * no-slip).  bjr/bji [3][nxh]: dU_i/dy at y = +1 per kx of the kz = 0 plane
+ * (bflow.f's bu1jr(:, i, 2)), NULL for zero; du0upp its kx = 0 streamwise
+ * part for the mean flow.  Returns 0, or -1 for another ibc. */
+int orc_set_bc(orc_t *o, int ibc, const double *bjr, const double *bji, double du0upp);
+
 /* complex DFT, y_k = sum_j x_j exp(sign 2 pi i jk/n), in place on (re,im) */
 void orc_cdft(int n, double *re, double *im, int sign);
 /* vrfftb convention: half spectrum X[0..n/2] (re,im) -> x[0..n-1] real */

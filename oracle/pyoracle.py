@@ -54,6 +54,7 @@ def lib():
         L.orc_set_force.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_set_fringe.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_set_wall.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_set_bc.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_double]
         _lib = L
     return _lib
 
@@ -127,6 +128,21 @@ class Oracle:
         self._fringe = (np.ascontiguousarray(lam, np.float64),
                         np.ascontiguousarray(base, np.float64))
         lib().orc_set_fringe(self.h, self._fringe[0].ctypes.data, self._fringe[1].ctypes.data)
+
+    def set_bc(self, ibc=0, bj=None, du0upp=0.0):
+        """Upper boundary: ibc 0 (wall) or 101 (free stream with the base
+        flow's dU_i/dy at y = +1, bj complex [3][nx/2] for kz = 0, None for
+        zero, and its mean du0upp)."""
+        if bj is None:
+            st = lib().orc_set_bc(self.h, ibc, None, None, du0upp)
+        else:
+            b = np.asarray(bj)
+            self._bc = (np.ascontiguousarray(b.real, np.float64),
+                        np.ascontiguousarray(b.imag, np.float64))
+            st = lib().orc_set_bc(self.h, ibc, self._bc[0].ctypes.data, self._bc[1].ctypes.data,
+                                  du0upp)
+        if st != 0:
+            raise ValueError(f"unsupported ibc {ibc}")
 
     def set_wall(self, vw=None):
         """Lower-wall v_w [nz][nx/2] (complex spectral) for later linear

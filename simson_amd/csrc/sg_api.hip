@@ -136,6 +136,12 @@ struct simson_gpu {
     // lower-wall blowing and suction (simson_gpu_wall): the spectra of
     // max(P, P damp) and min(P, P damp), their (0,0) values, and the time
     // at the start of the next substep (simson_gpu_set_time, advanced by run)
+    // free stream above (ibc = 101, simson_gpu_free_stream): the base
+    // flow's dU_i/dy at y = +1 per kx of the kz = 0 plane [3][nxh] (bj_set
+    // once given) and its mean streamwise value
+    double2 *bj = nullptr;
+    bool bj_set = false;
+    double du0upp = 0.;
     int wall = 0;
     double wall_tomeg = 0., wall_p0 = 0., wall_m0 = 0., tnow = 0.;
     double2 *wallP = nullptr, *wallM = nullptr;
@@ -241,7 +247,9 @@ void free_all(simson_gpu_t *g)
                     g->Lm, g->lpf, g->Lm32, g->Wf, g->Wf32, g->clW, g->clY, g->PG, g->TAU,
                     g->DT, g->PG32, g->TAU32, g->DT32, g->Dm, g->Dm32, g->twzs,
                     g->twxs, g->xysd, g->SQ, g->SS, g->SW, g->dlt2, g->SP, g->flam, g->fbase,
-                    g->flam32, g->fbase32, g->wallP, g->wallM};
+                    g->flam32, g->fbase32, g->wallP, g->wallM, g->bj,
+                    g->lam[0].hb, g->lam[0].hc, g->lam[1].hb, g->lam[1].hc,
+                    g->lam[2].hb, g->lam[2].hc};
     for (void *q : ptrs)
         if (q)
             (void)hipFree(q);
@@ -378,6 +386,16 @@ ColArgs col_args(simson_gpu_t *g)
     a.vwall = nullptr;
     a.vscale = 0.;
     a.v0 = 0.;
+    a.ibc = g->p.ibc;
+    a.bj = g->bj_set ? g->bj : nullptr;
+    a.du0upp = g->du0upp;
+    if (g->p.ibc == 101) {
+        // with a base flow the mean mode takes U = 1 above and no-slip
+        // below before the free-stream condition replaces the upper value
+        // (bla.f:1360-1365)
+        a.u0upp = 1.;
+        a.u0low = a.w0upp = a.w0low = 0.;
+    }
     return a;
 }
 
@@ -749,7 +767,8 @@ int simson_gpu_init(const simson_params_t *p, simson_gpu_t **out)
     if (!p || !out)
         return SIMSON_E_ARG;
     *out = nullptr;
-    if ((p->precision != SIMSON_FP64 && p->precision != SIMSON_MIXED) || p->ibc != 0 ||
+    if ((p->precision != SIMSON_FP64 && p->precision != SIMSON_MIXED) ||
+        (p->ibc != 0 && p->ibc != 101) || (p->ibc == 101 && p->cflux) ||
         p->nx < 4 || p->nz < 4 ||
         p->nx % 4 != 0 || p->nz % 2 != 0 || p->ny < 9 || p->ny % 2 == 0 ||
         p->re <= 0 || p->xl <= 0 || p->zl <= 0)
@@ -757,10 +776,9 @@ int simson_gpu_init(const simson_params_t *p, simson_gpu_t **out)
     const int nxp = p->nx + p->nx / 2, nzp = p->nz + p->nz / 2;
     if (!valid_fft_len(nzp) || !valid_fft_len(nxp / 2))
         return SIMSON_E_ARG;
-    // limits of the kernels: one tau column per wavefront holds at most
-    // 3 x 128 Chebyshev coefficients; the generic FFT kernels keep two z
-    // lines (and twelve half x rows) in 64 KiB of LDS
-    if (p->ny > 383 || 2 * nzp * 16 > 65536 || 12 * (nxp / 2) * 16 > 65536)
+    // one tau column per wavefront holds at most 3 x 128 Chebyshev
+    // coefficients
+    if (p->ny > 383)
         return SIMSON_E_ARG;
     // the FP32 path exists only as the specialised kernels
     if (p->precision == SIMSON_MIXED &&
@@ -769,6 +787,23 @@ int simson_gpu_init(const simson_params_t *p, simson_gpu_t **out)
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return SIMSON_E_NODEV;
+    {
+        // lengths without a specialised kernel use the generic ones, which
+        // keep two z lines, or twelve half x rows and their twiddles, in
+        // LDS: up to the device's limit per block (160 KiB on gfx950), with
+        // the kernels allowed more than the default 64 KiB where needed
+        int dev = 0, lds_max = 65536;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+                hipSuccess)
+            return SIMSON_E_HIP;
+        const long zneed = has_zfft4(nzp) ? 0 : 2L * nzp * 16;
+        const long xneed = has_xrow4(nxp / 2) ? 0 : 14L * (nxp / 2) * 16;
+        if (zneed > lds_max || xneed > lds_max)
+            return SIMSON_E_ARG;
+        if ((zneed > 65536 || xneed > 65536) && nonlin_prepare(lds_max) != 0)
+            return SIMSON_E_HIP;
+    }
     simson_gpu_t *g = new simson_gpu_t;
     g->p = *p;
     g->nx = p->nx;
@@ -818,6 +853,12 @@ int simson_gpu_init(const simson_params_t *p, simson_gpu_t **out)
         dalloc(g, &g->alfa, (size_t)nxh) || dalloc(g, &g->beta, (size_t)nz) ||
         dalloc(g, &g->deta, (size_t)ny) || dalloc(g, &g->twz, (size_t)nzp) ||
         dalloc(g, &g->twx, (size_t)nxp))
+        return fail(SIMSON_E_HIP);
+    if (p->ibc == 101)
+        for (int i = 0; i < 3; i++)
+            if (dalloc(g, &g->lam[i].hb, F) || dalloc(g, &g->lam[i].hc, 4 * ncol))
+                return fail(SIMSON_E_HIP);
+    if (p->ibc == 101 && dalloc(g, &g->bj, 3 * (size_t)nxh))
         return fail(SIMSON_E_HIP);
     // wavenumbers and grid (preprbl.f:34-55)
     std::vector<double> alfa(nxh), beta(nz), eta(ny), deta(ny);
@@ -1264,6 +1305,36 @@ int simson_gpu_xys(simson_gpu_t *g, double dtn, double xs, double zs, int nxys, 
     return 0;
 }
 
+int simson_fsc_table(double m, int n, double ymax, double eps, double *tab, double *dstar2)
+{
+    return sg::fsc_table(m, n, ymax, eps, tab, dstar2);
+}
+
+int simson_base_flow(const simson_baseflow_t *b, const double *tab, int ntab, double *base,
+                     double *bjr, double *bji, double *du0upp)
+{
+    return sg::spatial_base_flow(b, tab, ntab, base, bjr, bji, du0upp);
+}
+
+int simson_gpu_free_stream(simson_gpu_t *g, const double *bjr, const double *bji, double du0upp)
+{
+    if (!g || g->p.ibc != 101 || (!bjr) != (!bji) || !std::isfinite(du0upp))
+        return SIMSON_E_ARG;
+    g->du0upp = du0upp;
+    g->bj_set = false;
+    if (bjr) {
+        const size_t m = 3 * (size_t)g->nxh;
+        std::vector<double2> h(m);
+        for (size_t i = 0; i < m; i++)
+            h[i] = make_double2(bjr[i], bji[i]);
+        if (h2d(g, g->bj, h.data(), m))
+            return SIMSON_E_HIP;
+        CK(hipStreamSynchronize(g->stream));
+        g->bj_set = true;
+    }
+    return 0;
+}
+
 int simson_gpu_set_time(simson_gpu_t *g, double t)
 {
     if (!g)
@@ -1569,6 +1640,10 @@ int simson_gpu_pressure(simson_gpu_t *g, double *pr, double *pi)
 {
     if (!g || !pr || !pi)
         return SIMSON_E_ARG;
+    if (g->p.ibc != 0) {
+        g->err = "pressure: only with walls at both boundaries (ibc = 0)";
+        return SIMSON_E_ARG;
+    }
     if (!has_zfft4(g->nzp) || !has_xrow4(g->nxp / 2) || g->ny > pressure_max_ny()) {
         g->err = "pressure: nzp and nxp/2 need a specialised FFT length";
         return SIMSON_E_ARG;

@@ -8,21 +8,6 @@
 
 namespace sg {
 
-namespace {
-
-// smooth step of step.f: 0 up to 0.02, 1 above 0.98, and
-// 1/(1 + exp(1/(x - 1) + 1/x)) in between
-double smooth_step(double x)
-{
-    if (x <= 0.02)
-        return 0.;
-    if (x <= 0.98)
-        return 1. / (1. + std::exp(1. / (x - 1.) + 1. / x));
-    return 1.;
-}
-
-}  // namespace
-
 // lam[j], j < nxp = 3 nx/2, at x_j = -xl/2 + j xl/nxp + xsc.  As in
 // ffun.f, only the even-point index range xstart..xend (1-based) of the
 // fringe is set, the rest stays zero; a fringe that wraps across the box

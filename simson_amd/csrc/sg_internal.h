@@ -4,11 +4,23 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <cmath>
 #include <vector>
 
 #include "simson_gpu.h"
 
 namespace sg {
+
+// The smooth step of step.f: 0 for x <= 0.02, 1 for x > 0.98, and
+// 1 / (1 + exp(1/(x - 1) + 1/x)) between (host side).
+inline double smooth_step(double x)
+{
+    if (x <= 0.02)
+        return 0.;
+    if (x <= 0.98)
+        return 1. / (1. + std::exp(1. / (x - 1.) + 1. / x));
+    return 1.;
+}
 
 constexpr int kMaxRad = 16;
 constexpr int kNSlot = 7;          // host-visible slots: u v w om_x om_z p_v p_om
@@ -59,6 +71,11 @@ struct LamFactor {
     double2 *gp = nullptr;     // (gamma, 1/pivot)
     double *vh = nullptr;      // v_h (lam^2 stages only)
     double2 *sh = nullptr;     // [z][x] per-parity sums (lam^2 stages only)
+    // free stream above (ibc = 101; lam^2 stages only, else null): the
+    // columns (phi_h, v_h2) and per column (A1, A2, B, 0) = d2v_h1/dy2,
+    // d2v_h2/dy2 and dphi_h/dy at y = +1
+    double2 *hb = nullptr;
+    double *hc = nullptr;
 };
 
 // Arguments of the per-(kx,kz)-column kernels.
@@ -82,6 +99,12 @@ struct ColArgs {
     // 1663-1670)
     const double2 *vwall;
     double vscale, v0;
+    // ibc = 101: du_i/dy at y = +1 follows the base flow, bj [3][nxh] its
+    // dU/dy, dV/dy, dW/dy there per kx of the kz = 0 plane (null: zero),
+    // du0upp the mean streamwise value (evalbc.f:142-159, linearbl.f:1500-1531)
+    int ibc;
+    const double2 *bj;
+    double du0upp;
 };
 
 // Step-2 launch arguments.
@@ -271,6 +294,11 @@ void launch_factor_lam(const ColArgs &a, double apb, LamFactor lf, const double 
                        hipStream_t s);
 // one linear substep, one wavefront per column
 // (fp32_cols: the Chebyshev columns are C32, else C)
+// allow the generic step-2 kernels dynamic LDS up to lds_max bytes
+int nonlin_prepare(int lds_max);
+int fsc_table(double m, int n, double ymax, double eps, double *tab, double *dstar2);
+int spatial_base_flow(const simson_baseflow_t *b, const double *tab, int ntab, double *base,
+                      double *bjr, double *bji, double *du0upp);
 int  launch_tau(const ColArgs &a, LamFactor lf, LamFactor kf, double an, double bn,
                 double anp1, double bnp1, bool fp32_cols, hipStream_t s);
 

@@ -236,6 +236,35 @@ __global__ void __launch_bounds__(64) k_factor_lam(ColArgs a, double apb, LamFac
     for (int k = 0; k < n; k++)
         s[k & 1] += (double)k * (double)k * vh[k];
     lf.sh[t] = make_double2(s[0], s[1]);
+    if (!lf.hc)
+        return;
+    // free stream above (ibc = 101, linearbl.f:946-1098): v_h2 solves the
+    // k^2 problem with v = 1 per parity class (v(+1) = 2, v(-1) = 0), its
+    // wall slopes cancelled with v_h; v_h1 is v_h with each class scaled
+    // to unit slope at y = +1.  A1, A2: d2/dy2 at +1 of v_h1, v_h2
+    // (T_k''(1) = k^2 (k^2 - 1)/3); B: dphi_h/dy at +1 (T_k'(1) = k^2).
+    const ColView vh2 = W(5);
+    seq_solve(n, k2, phh, true, 1., 1., kgp, vh2, rho);
+    double s2[2] = {0., 0.};
+#pragma unroll 8
+    for (int k = 0; k < n; k++)
+        s2[k & 1] += (double)k * (double)k * vh2[k];
+    const double r2[2] = {-s2[0] / s[0], -s2[1] / s[1]};
+    const double r1[2] = {1. / s[0], 1. / s[1]};
+    double A1 = 0., A2 = 0., B = 0.;
+#pragma unroll 8
+    for (int k = 0; k < n; k++) {
+        const double kk = (double)k * (double)k, w2 = kk * (kk - 1.) * (1. / 3.);
+        const double v2 = vh2[k] + r2[k & 1] * vh[k];
+        vh2[k] = v2;
+        A1 += w2 * (vh[k] * r1[k & 1]);
+        A2 += w2 * v2;
+        B += kk * phh[k];
+    }
+    lf.hc[4 * t] = A1;
+    lf.hc[4 * t + 1] = A2;
+    lf.hc[4 * t + 2] = B;
+    lf.hc[4 * t + 3] = 0.;
 }
 
 // [k][column] -> [column][k]: out2 (double2 from the planes a, b) and,
@@ -389,6 +418,10 @@ void launch_factor_lam(const ColArgs &a, double apb, LamFactor lf, const double 
     dim3 tg((ncol + kTT - 1) / kTT, (n + kTT - 1) / kTT);
     hipLaunchKernelGGL(k_cols_to_rows, tg, dim3(256), 0, s, a.W, a.W + plane, a.W + 4 * plane,
                        lf.gp, lf.vh, n, ncol);
+    if (lf.hb)
+        hipLaunchKernelGGL(k_cols_to_rows, tg, dim3(256), 0, s, a.W + 2 * plane,
+                           a.W + 5 * plane, (const double *)nullptr, lf.hb, (double *)nullptr, n,
+                           ncol);
 }
 
 }  // namespace sg

@@ -487,6 +487,16 @@ void launch_zsyn(const NonlinArgs &a, const FftPlan &pz, hipStream_t s)
     hipLaunchKernelGGL(k_zsyn, grid, dim3(256), lds, s, a, pz, B);
 }
 
+int nonlin_prepare(int lds_max)
+{
+    const void *fns[] = {(const void *)k_zsyn, (const void *)k_zana, (const void *)k_xrow};
+    for (const void *fn : fns)
+        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) !=
+            hipSuccess)
+            return -1;
+    return 0;
+}
+
 int xrow_rows(const NonlinArgs &a)
 {
     switch (a.nxp / 2) {

@@ -1,4 +1,4 @@
-// Step 3 of the bla substep (bla/linearbl.f, tau method, ibc = 0) with one
+// Step 3 of the bla substep (bla/linearbl.f, tau method, ibc = 0 or 101) with one
 // wavefront per (kx,kz) column.
 //
 // Lane L of the wave holds, for parity class c (k = c mod 2), the E class
@@ -395,7 +395,8 @@ constexpr int tau_waves_per_eu()
     return E == 1 ? 3 : 2;
 }
 
-template <int SW, int E, typename CV>
+// FS: free stream at y = +1 (ibc = 101) instead of the upper wall
+template <int SW, int E, typename CV, bool FS>
 __global__ void __launch_bounds__(256, (tau_waves_per_eu<SW, E>()))
     k_tau_w(ColArgs a, LamFactor lf, LamFactor kf, double an, double bn, double anp1, double bnp1)
 {
@@ -560,6 +561,60 @@ __global__ void __launch_bounds__(256, (tau_waves_per_eu<SW, E>()))
                 v[1][c][e] = v[1][c][e] + ci * tmp[c][e].x;
             }
         }
+        if constexpr (FS) {
+            // free stream at y = +1 (ibc = 101; evald.f, evalbc.f:142-159,
+            // solvbc.f, linearbl.f:946-1150): du_i/dy = dU_i/dy of the base
+            // flow there (kz = 0 plane, zero elsewhere).  By continuity and
+            // the definition of omega_y that fixes d2v/dy2 = -i (alpha U' +
+            // beta W') and d om/dy = i (beta U' - alpha W') at +1, and dv/dy
+            // = V'.  With v_h1 = v_h scaled per class to unit slope at +1,
+            // v_h2 and phi_h = omega_h (A1, A2, B their derivatives there):
+            //   v -= c1 v_h1 + c2 v_h2,  om -= c3 phi_h,
+            //   c1 = -V'/2, c2 = (d2v_p - c1 A1 - d2v*)/A2,
+            //   c3 = (dom_p - dom*)/B.
+            // The mean column (k0) keeps its own condition below.
+            double q[4] = {0., 0., 0., 0.};
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    const double kk = (double)kidx(c, e) * (double)kidx(c, e);
+                    const double w2 = kk * (kk - 1.) * (1. / 3.);
+                    q[0] += w2 * v[0][c][e];
+                    q[1] += w2 * v[1][c][e];
+                    q[2] += kk * om[0][c][e];
+                    q[3] += kk * om[1][c][e];
+                }
+            wave_sum<SW, 4>(q, l);
+            const double A1 = lf.hc[4 * t], A2 = lf.hc[4 * t + 1], B = lf.hc[4 * t + 2];
+            double2 ju = make_double2(0., 0.), jv = ju, jw = ju;
+            if (z == 0 && a.bj) {
+                ju = a.bj[x];
+                jv = a.bj[a.nxh + x];
+                jw = a.bj[2 * a.nxh + x];
+            }
+            const double sr = alfa * ju.x + beta * jw.x, si = alfa * ju.y + beta * jw.y;
+            const double tr = beta * ju.x - alfa * jw.x, ti = beta * ju.y - alfa * jw.y;
+            const double c1r = k0 ? 0. : -.5 * jv.x, c1i = k0 ? 0. : -.5 * jv.y;
+            const double c2r = k0 ? 0. : (q[0] - c1r * A1 - si) / A2;
+            const double c2i = k0 ? 0. : (q[1] - c1i * A1 + sr) / A2;
+            const double c3r = k0 ? 0. : (q[2] + ti) / B;
+            const double c3i = k0 ? 0. : (q[3] - tr) / B;
+            double2 hb[2][E];
+            io.load(lf.hb, hb);                     // (phi_h, v_h2)
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const double r1 = 1. / shc[c];
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    const double h1 = tmp[c][e].x * r1, h2 = hb[c][e].y, ph = hb[c][e].x;
+                    v[0][c][e] -= c1r * h1 + c2r * h2;
+                    v[1][c][e] -= c1i * h1 + c2i * h2;
+                    om[0][c][e] -= c3r * ph;
+                    om[1][c][e] -= c3i * ph;
+                }
+            }
+        }
     }
     // dv, phi = D^2 v - k^2 v (linearbl.f:1151-1166)
     double dv[2][2][E], ph[2][2][E];
@@ -721,6 +776,34 @@ __global__ void __launch_bounds__(256, (tau_waves_per_eu<SW, E>()))
         if (L == 0)
             *a.px = px;
     }
+    if constexpr (FS) {
+        // free stream above (linearbl.f:1500-1531): add phi_h (the k = 0
+        // lam^2 solution with 1 per class) so that du0/dy(+1) = du0upp and
+        // dw0/dy(+1) = 0; B = dphi_h/dy at +1
+        double q[2] = {0., 0.};
+        double ph[2][E];
+        const double2 *hbp = lf.hb + (size_t)t * n;
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int k = kidx(c, e);
+                const double kk = (double)k * (double)k;
+                ph[c][e] = k < n ? hbp[k].x : 0.;
+                q[0] += kk * uw[0][c][e];
+                q[1] += kk * uw[1][c][e];
+            }
+        wave_sum<SW, 2>(q, l);
+        const double B = lf.hc[4 * t + 2];
+        const double cu = (q[0] - a.du0upp) / B, cw = q[1] / B;
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                uw[0][c][e] -= cu * ph[c][e];
+                uw[1][c][e] -= cw * ph[c][e];
+            }
+    }
     double duw[2][2][E], d2uw[2][2][E];
     dcheb_w<SW, 2, E>(uw, duw, l);
     dcheb_w<SW, 2, E>(duw, d2uw, l);
@@ -761,7 +844,12 @@ void launch_seg(const ColArgs &a, LamFactor lf, LamFactor kf, double an, double 
     const int ncol = a.nz * a.nxh;
     constexpr int per_block = 4 * (WV / SW);
     dim3 grid((ncol + per_block - 1) / per_block);
-    hipLaunchKernelGGL((k_tau_w<SW, E, CV>), grid, dim3(256), 0, s, a, lf, kf, an, bn, anp1, bnp1);
+    if (a.ibc == 101)
+        hipLaunchKernelGGL((k_tau_w<SW, E, CV, true>), grid, dim3(256), 0, s, a, lf, kf, an, bn,
+                           anp1, bnp1);
+    else
+        hipLaunchKernelGGL((k_tau_w<SW, E, CV, false>), grid, dim3(256), 0, s, a, lf, kf, an, bn,
+                           anp1, bnp1);
 }
 
 // Segment width: 64 lanes (one column per wave).  SIMSON_TAU_SEG=16 or 32

@@ -15,19 +15,6 @@
 
 namespace sg {
 
-namespace {
-
-double smooth_step(double x)
-{
-    if (x <= 0.02)
-        return 0.;
-    if (x <= 0.98)
-        return 1. / (1. + std::exp(1. / (x - 1.) + 1. / x));
-    return 1.;
-}
-
-}  // namespace
-
 int wall_spectra(int nx, int nz, double xl, double zl, const simson_wall_t *w, double2 *pmax,
                  double2 *pmin)
 {

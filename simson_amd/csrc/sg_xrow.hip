@@ -32,8 +32,9 @@ namespace {
 
 template <int M>
 struct XPlan {
-    static constexpr int M1 = (M % 4 == 0) ? ((M > 96 && M % 8 == 0) ? 8 : 4)
-                                           : ((M % 2 == 0) ? 2 : 3);
+    static constexpr int M1 = (M == 384) ? 12
+                              : (M % 4 == 0) ? ((M > 96 && M % 8 == 0) ? 8 : 4)
+                                             : ((M % 2 == 0) ? 2 : 3);
     static constexpr int M2 = M / M1;
     static constexpr int SROW = M + M / M1;
     static constexpr int TABN = 2 * M;
@@ -622,16 +623,26 @@ __global__ void __launch_bounds__((CBlock<V, M>::NT))
 }
 
 template <typename V, int M>
-void xclos(const Step2Args<V> &a, const V *tw, const ClosArgs &c, hipStream_t s)
+bool xclos(const Step2Args<V> &a, const V *tw, const ClosArgs &c, hipStream_t s)
 {
     using B = CBlock<V, M>;
+    if constexpr (B::lds > 65536) {
+        // nine fields of one row exceed 64 KiB from M = 384 (FP64): allow
+        // the kernel the larger dynamic LDS once (gfx950 has 160 KiB)
+        static const bool ok = hipFuncSetAttribute((const void *)k_xclos<V, M>,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                   (int)B::lds) == hipSuccess;
+        if (!ok)
+            return false;
+    }
     dim3 grid((a.nzp + B::RW - 1) / B::RW, a.ny);
     hipLaunchKernelGGL((k_xclos<V, M>), grid, dim3(B::NT), B::lds, s, a, tw, c);
+    return true;
 }
 
 }  // namespace
 
-#define SG_X4LENS(X) X(12) X(24) X(48) X(96) X(192) X(288)
+#define SG_X4LENS(X) X(12) X(24) X(48) X(96) X(192) X(288) X(384)
 
 bool has_xrow4(int m)
 {
@@ -692,8 +703,7 @@ bool launch_xclos(const Step2Args<V> &a, const V *wx, const ClosArgs &c, hipStre
     switch (a.nxp / 2) {
 #define SG_CASE(N)                      \
     case N:                             \
-        xclos<V, N>(a, wx, c, s);       \
-        return true;
+        return xclos<V, N>(a, wx, c, s);
         SG_X4LENS(SG_CASE)
 #undef SG_CASE
     default:

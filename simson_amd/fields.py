@@ -59,12 +59,22 @@ def kz_of_index(nz: int) -> np.ndarray:
 
 def channel_field(nx: int, nyp: int, nz: int, xl: float, zl: float, re: float,
                   kmax: int, rms: float, seed: int,
-                  laminar: bool = True) -> UField:
-    """Build the seeded perturbed Poiseuille field as a ``UField``."""
+                  laminar: bool = True, walls: int = 2) -> UField:
+    """Build the seeded perturbed Poiseuille field as a ``UField``.  With
+    walls = 1 the envelopes vanish at the lower wall only, (1+y)^2 for v
+    and (1+y) for omega_y (a boundary layer with a free stream above), and
+    the mean mode is left zero."""
     rng = SplitMix64(seed)
     y = gl_points(nyp)
-    env2 = P.polypow([1.0, 0.0, -1.0], 2)      # (1-y^2)^2
-    env1 = np.array([1.0, 0.0, -1.0])          # (1-y^2)
+    if walls == 2:
+        env2 = P.polypow([1.0, 0.0, -1.0], 2)      # (1-y^2)^2
+        env1 = np.array([1.0, 0.0, -1.0])          # (1-y^2)
+    elif walls == 1:
+        env2 = P.polypow([1.0, 1.0], 2)            # (1+y)^2
+        env1 = np.array([1.0, 1.0])                # (1+y)
+        laminar = False
+    else:
+        raise ValueError("walls must be 1 or 2")
     u = np.zeros((3, nz, nyp, nx // 2), dtype=np.complex128)
     kzs = kz_of_index(nz)
     # Gauss-Legendre rule for the y average of |.|^2 (exact for these degrees)
@@ -107,3 +117,21 @@ def channel_field(nx: int, nyp: int, nz: int, xl: float, zl: float, re: float,
     if laminar:
         u[0, 0, :, 0] = 1.0 - y * y
     return UField(u=u, re=re, xl=xl, zl=zl, t=0.0, xs=0.0, fltype=1, dstar=1.0)
+
+
+def boundary_layer_field(nx: int, nyp: int, nz: int, xl: float, zl: float, re: float,
+                         dstar: float, base: np.ndarray, bstart: float, blength: float,
+                         kmax: int, rms: float, seed: int, fltype: int = 6) -> UField:
+    """A spatial boundary layer for a .u file: the base flow base [3][ny]
+    [3nx/2] (physical points x_j = -xl/2 + j xl/nxp, e.g. from
+    simson_amd.baseflow.spatial_base_flow) as x Fourier modes kx < nx/2,
+    plus the seeded one-wall perturbation of channel_field(walls=1).
+    Lengths, re and the blending zone in the units of the .u file."""
+    f = channel_field(nx, nyp, nz, xl, zl, re, kmax=kmax, rms=rms, seed=seed, walls=1)
+    nxp = 3 * nx // 2
+    spec = np.fft.fft(np.asarray(base, np.float64), axis=2)[:, :, :nx // 2] / nxp
+    f.u[:, 0, :, :] += spec
+    f.fltype = fltype
+    f.dstar = dstar
+    f.extra = {"bstart": bstart, "blength": blength}
+    return f

@@ -31,6 +31,7 @@ EXPORTS = (
     "simson_gpu_pressure",
     "simson_gpu_les", "simson_les_filter", "simson_gpu_closure", "simson_gpu_sync",
     "simson_gpu_fringe", "simson_fringe_function", "simson_gpu_wall", "simson_gpu_set_time",
+    "simson_gpu_free_stream", "simson_fsc_table", "simson_base_flow",
     "simson_wall_velocity",
     "simson_mlp_create", "simson_mlp_nparams", "simson_mlp_params", "simson_mlp_set_data",
     "simson_mlp_train", "simson_mlp_grad", "simson_mlp_infer", "simson_mlp_last_error",
@@ -62,6 +63,12 @@ class SimsonFringe(C.Structure):
 class SimsonWall(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("amp", "damp", "xstart", "xend", "xrise", "xfall",
                                          "zbet", "tomeg", "xsc", "zsc")]
+
+
+class SimsonBaseFlow(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("fltype", C.c_int)] + [
+        (n, C.c_double) for n in ("xl", "re", "dstar", "dstar2", "bstart", "blength", "rlam",
+                                  "spanv", "xsc", "u0low", "w0low")]
 
 
 class SimsonError(RuntimeError):
@@ -107,6 +114,11 @@ def lib():
         L.simson_gpu_sync.argtypes = [vp]
         L.simson_gpu_fringe.argtypes = [vp, C.POINTER(SimsonFringe), C.c_void_p]
         L.simson_gpu_wall.argtypes = [vp, C.POINTER(SimsonWall)]
+        L.simson_gpu_free_stream.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_double]
+        L.simson_fsc_table.argtypes = [C.c_double, C.c_int, C.c_double, C.c_double, _dp,
+                                       C.POINTER(C.c_double)]
+        L.simson_base_flow.argtypes = [C.POINTER(SimsonBaseFlow), _dp, C.c_int, _dp,
+                                       C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]
         L.simson_gpu_set_time.argtypes = [vp, C.c_double]
         L.simson_wall_velocity.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double,
                                            C.POINTER(SimsonWall), C.c_double, _dp, _dp]
@@ -130,12 +142,13 @@ class BlaGPU:
 
     def __init__(self, nx: int, ny: int, nz: int, re: float, xl: float, zl: float,
                  rot: float = 0.0, cflux: bool = True, retau: float = 0.0,
-                 precision: int = SIMSON_FP64):
+                 precision: int = SIMSON_FP64, ibc: int = 0):
         """precision: SIMSON_FP64, or SIMSON_MIXED (FP32 FFTs, products and
-        Chebyshev transforms; FP64 tau solves, mean flow and partial RHS)."""
+        Chebyshev transforms; FP64 tau solves, mean flow and partial RHS).
+        ibc: 0 (no-slip walls) or 101 (free stream above, see free_stream)."""
         self.nx, self.ny, self.nz = nx, ny, nz
         self.precision = precision
-        p = SimsonParams(nx, ny, nz, precision, 0, int(cflux), re, xl, zl, rot, retau)
+        p = SimsonParams(nx, ny, nz, precision, ibc, int(cflux), re, xl, zl, rot, retau)
         h = C.c_void_p()
         self._check(lib().simson_gpu_init(C.byref(p), C.byref(h)))
         self.h = h
@@ -268,6 +281,18 @@ class BlaGPU:
             return
         w = SimsonWall(amp, damp, xstart, xend, xrise, xfall, zbet, tomeg, xsc, zsc)
         self._check(lib().simson_gpu_wall(self.h, C.byref(w)))
+
+    def free_stream(self, bj=None, du0upp: float = 0.0):
+        """ibc = 101 data: bj complex [3][nx/2], the base flow's dU_i/dy at
+        y = +1 per kx (kz = 0 plane; None for zero), and du0upp."""
+        if bj is None:
+            self._check(lib().simson_gpu_free_stream(self.h, None, None, du0upp))
+            return
+        b = np.asarray(bj)
+        br = np.ascontiguousarray(b.real, np.float64)
+        bi = np.ascontiguousarray(b.imag, np.float64)
+        self._check(lib().simson_gpu_free_stream(self.h, br.ctypes.data_as(C.c_void_p),
+                                                 bi.ctypes.data_as(C.c_void_p), du0upp))
 
     def set_time(self, t: float):
         """Time of the start of the next substep (time-dependent forcing)."""

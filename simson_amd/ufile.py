@@ -78,10 +78,10 @@ def write_u(path: str, fld: UField) -> None:
         _rec(f, struct.pack("<di4d", fld.re, 0, fld.xl, fld.zl, fld.t, 0.0))
         _rec(f, struct.pack("<4i", 2 * nxh, nyp, nz, 0))
         _rec(f, struct.pack("<id", fld.fltype, fld.dstar))
-        if fld.fltype in (4, 5):
+        if fld.fltype in (4, 5, 6):
             _rec(f, struct.pack("<2d", fld.extra.get("bstart", 0.0),
                                 fld.extra.get("blength", 0.0)))
-        elif 6 <= fld.fltype <= 9:
+        elif 7 <= fld.fltype <= 9:
             _rec(f, struct.pack("<4d", fld.extra.get("bstart", 0.0),
                                 fld.extra.get("blength", 0.0),
                                 fld.extra.get("rlam", 0.0),

@@ -11,6 +11,7 @@ Runs in the build container only (it needs /root/reference and amdflang):
     python tests/golden/make_golden.py fringe     # fringe_32x33x32.npz
     python tests/golden/make_golden.py wall       # wall_32x33x32.npz
     python tests/golden/make_golden.py vart       # vart_32x33x32.npz
+    python tests/golden/make_golden.py blasius    # blasius_64x33x8.npz
 
 1. builds the reference `bla` for 32x33x32 (serial, pressure off) with
    oracle/build_ref.sh into oracle/_ref/;
@@ -428,6 +429,101 @@ def vart():
         shutil.rmtree(work)
 
 
+# spatial Blasius boundary layer at 64x33x8 (all lengths in displacement
+# thicknesses of the box centre, as bla.i gives them; the .u file holds them
+# times dstar = 2/yl)
+BL = dict(nx=64, ny=33, nz=8, re=400.0, xl=100.0, yl=20.0, zl=20.0, bstart=-35.0,
+          blength=25.0, dt=0.1, kmax=4, rms=1e-2, seed=5,
+          fringe=(1.0, -45.0, 0.0, 15.0, 10.0))
+BL_WALL = (0.01, 1.0, -20.0, -10.0, 3.0, 3.0, 0.5, 0.3)
+BLBIN = os.path.join(REPO, "oracle", "_ref", "bla_64x33x8_serial")
+
+
+def fsc_dat(work):
+    """Run the reference similarity solver (oracle/_ref/fsc, built by
+    oracle/build_fsc.sh) with bls/fsc.i's defaults for Blasius: m 0,
+    n 20001, eps 1e-15, ymax 30, no scalar, no text output."""
+    exe = os.path.join(REPO, "oracle", "_ref", "fsc")
+    if not os.path.exists(exe):
+        subprocess.check_call([os.path.join(REPO, "oracle", "build_fsc.sh")])
+    with open(os.path.join(work, "fsc.i"), "w") as f:
+        f.write("0\n20001\n1e-15\n30\n0\n0\n")
+    with open(os.path.join(work, "fsc.out"), "w") as out:
+        subprocess.check_call([exe], cwd=work, stdout=out, stderr=subprocess.STDOUT)
+    return os.path.join(work, "fsc.dat")
+
+
+def blasius_field(table):
+    from simson_amd import baseflow
+    c = BL
+    ds = 2.0 / c["yl"]
+    base, _, _ = baseflow.spatial_base_flow(c["nx"], c["ny"], c["xl"] * ds, c["re"] / ds, ds,
+                                            table, c["bstart"] * ds, c["blength"] * ds)
+    return fields.boundary_layer_field(c["nx"], c["ny"], c["nz"], c["xl"] * ds, c["zl"] * ds,
+                                       c["re"] / ds, ds, base, c["bstart"] * ds,
+                                       c["blength"] * ds, kmax=c["kmax"], rms=c["rms"],
+                                       seed=c["seed"])
+
+
+def blasius():
+    """Spatial Blasius boundary-layer fixture (fltype 6, spat, ibc 101,
+    cflux off, no pressure gradient) at 64x33x8, Re_delta* 400, box
+    100 x 20 x 20 delta*: the fsc.dat of the reference similarity solver,
+    the base flow blended over [-35, -10] delta*, the fringe (fmax 1 on
+    [-45, 0], rise 15, fall 10) and a seeded one-wall perturbation (rms
+    1e-2, |kx|, |kz| <= 4) that is O(1e-2) up to the free stream, so that
+    the free-stream rows matter.  20 steps at dt = 0.1 delta*/U, saves after
+    1 and 20 steps; case b adds the lower-wall strip (wbci = 1, amp 0.01 on
+    [-20, -10], rise/fall 3, zbet 0.5, tomeg 0.3).  Keys: table (the
+    similarity table every 20th row), dstar2, init, <case>_u1, <case>_u20,
+    and the log's du0upp (times dstar) and x0 (over dstar)."""
+    from simson_amd import baseflow
+    c = BL
+    if not os.path.exists(BLBIN):
+        subprocess.check_call([os.path.join(REPO, "oracle", "build_ref.sh"),
+                               str(c["nx"]), str(c["ny"]), str(c["nz"]), "1", "no", "0",
+                               os.path.basename(BLBIN)])
+    res = {}
+    for key in ("a", "b"):
+        work = tempfile.mkdtemp(prefix="simson_golden_bl.")
+        try:
+            table = baseflow.read_fsc(fsc_dat(work))
+            if key == "a":
+                res["table"] = table.tab[::20]
+                res["dstar2"] = np.array(table.dstar2)
+            f = blasius_field(table)
+            ufile.write_u(os.path.join(work, "init.u"), f)
+            if key == "a":
+                res["init"] = ufile.read_u(os.path.join(work, "init.u")).u
+            fmax, fstart, fend, frise, ffall = c["fringe"]
+            kw = dict(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=60, dt=c["dt"],
+                      nst=3, xl=c["xl"], cflux=False, retau=0.0, ibc=101, icfl=3, spat=True,
+                      fmax=fmax, fstart=fstart, fend=fend, frise=frise, ffall=ffall,
+                      tsave=[c["dt"]], nmsave=["t1.u"])
+            if key == "b":
+                amp, damp, x0, x1, xr, xf, zb, om = BL_WALL
+                kw.update(wbci=1, wamp=amp, wdamp=damp, wxstart=x0, wxend=x1, wxrise=xr,
+                          wxfall=xf, wzbet=zb, wtomeg=om)
+            with open(os.path.join(work, "bla.i"), "w") as fh:
+                fh.write(blai.format_channel(blai.BlaParams(**kw)))
+            with open(os.path.join(work, "out.txt"), "w") as out:
+                subprocess.check_call([BLBIN], cwd=work, stdout=out, stderr=subprocess.STDOUT)
+            log = open(os.path.join(work, "out.txt")).read()
+            if "bla terminated normally" not in log:
+                raise RuntimeError(f"reference bla (blasius {key}) did not terminate normally")
+            res[f"{key}_u1"] = ufile.read_u(os.path.join(work, "t1.u")).u
+            res[f"{key}_u20"] = ufile.read_u(os.path.join(work, "end.u")).u
+            if key == "a":
+                res["du0upp_log"] = np.array(float(re.search(
+                    r"du0upp\s*:\s*([0-9.Ee+-]+)", log).group(1)))
+                res["x0_log"] = np.array(float(re.search(
+                    r"x0\s*:\s*([0-9.Ee+-]+)", log).group(1)))
+        finally:
+            shutil.rmtree(work)
+    np.savez_compressed(os.path.join(HERE, "blasius_64x33x8.npz"), **res)
+    print("wrote", os.path.join(HERE, "blasius_64x33x8.npz"), sorted(res))
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "pressure":
         pressure()
@@ -443,6 +539,8 @@ if __name__ == "__main__":
         wall()
     elif len(sys.argv) > 1 and sys.argv[1] == "long":
         long()
+    elif len(sys.argv) > 1 and sys.argv[1] == "blasius":
+        blasius()
     elif len(sys.argv) > 1 and sys.argv[1] == "vart":
         vart()
     else:

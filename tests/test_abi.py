@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared():
     text = open(os.path.join(ROOT, "include", "simson_gpu.h")).read()
-    return sorted(set(re.findall(r"\b(simson_(?:gpu|les|mlp|fringe|wall)_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(simson_(?:gpu|les|mlp|fringe|wall|fsc|base)_\w+)\s*\(", text)))
 
 
 def test_header_and_python_agree():
@@ -42,9 +42,11 @@ def test_init_rejects_unsupported_settings():
         gpu.SimsonParams(32, 33, 32, 7, 0, 1, 3000.0, 12.5, 6.28, 0.0, 0.0),   # precision
         gpu.SimsonParams(32, 33, 28, 0, 0, 1, 3000.0, 12.5, 6.28, 0.0, 0.0),   # nzp=42 (7)
         gpu.SimsonParams(32, 385, 32, 0, 0, 1, 3000.0, 12.5, 6.28, 0.0, 0.0),  # ny > 383
-        gpu.SimsonParams(32, 33, 1440, 0, 0, 1, 3000.0, 12.5, 6.28, 0.0, 0.0),  # nzp=2160
-        gpu.SimsonParams(480, 33, 32, 0, 0, 1, 3000.0, 12.5, 6.28, 0.0, 0.0),  # nxp/2=360
+        gpu.SimsonParams(32, 33, 32, 0, 101, 1, 3000.0, 12.5, 6.28, 0.0, 0.0),  # 101 + cflux
+        gpu.SimsonParams(32, 33, 32, 0, 100, 0, 3000.0, 12.5, 6.28, 0.0, 0.0),  # ibc=100
     ]
+    # (lengths without a specialised FFT kernel are checked against the
+    # device's LDS limit, so only on a device)
     for p in bad:
         h = C.c_void_p()
         assert gpu.lib().simson_gpu_init(C.byref(p), C.byref(h)) == -1

@@ -1,0 +1,227 @@
+"""Spatial Blasius boundary layer (bla.i spat, fltype 6, ibc 101): the
+similarity table, the base flow with its free-stream data, and the step with
+the free-stream boundary rows.
+
+Pins:
+
+* the reference similarity solver (bls/fsc.f, built by oracle/build_fsc.sh):
+  tests/golden/blasius_64x33x8.npz holds every 20th row of the fsc.dat it
+  wrote, and the library's table equals it bit for bit;
+* closed forms of the Blasius solution in fsc.f's scaling (eta = y
+  sqrt(U/(2 nu x))): f''(0) = 0.33205733621519630 sqrt 2 and the
+  displacement thickness 1.72078765752 / sqrt 2 (the classical constants),
+  f'(ymax) = g(ymax) = 1, the table's f''' column equal to -f f'', and an
+  independent adaptive integration (scipy, rtol 1e-13) from the table's
+  f''(0) that reproduces the profile;
+* the base flow: v(-1) = 0, continuity du/dx + dv/dy = 0 (spectral), the
+  similarity profile at the local leading-edge distance where the blending
+  is complete, the library equal to the numpy restatement of
+  oracle/baseflow_ref.py, and du0upp and x0 equal to the values the
+  reference bla printed for the fixture;
+* the step: from the fixture's start field (the base flow plus a seeded
+  one-wall perturbation of rms 1e-2 that stays O(1e-2) up to the free
+  stream), the oracle with the restated base flow, fringe and free-stream
+  data against the reference bla after 1 and 20 steps, case a without and
+  case b with the lower-wall strip; the library on the GPU with its own base
+  flow likewise; and the boundary conditions of the result: du_i/dy at
+  y = +1 equal to the base flow's (kz = 0) or zero, dv/dy = V' there, and
+  no-slip below.
+
+Tolerance: velocities 1e-12 relative L2 outside the z-oddball plane (FP64;
+2e-5 mixed, whose FP32 transforms round at 6e-8).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import baseflow_ref
+import fringe_ref
+import wall_ref
+from simson_amd import baseflow
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NX, NY, NZ = 64, 33, 8
+NXP = 3 * NX // 2
+DS = 2.0 / 20.0                         # dstar = 2/yl, yl = 20 delta*
+RE, XL, ZL = 400.0 / DS, 100.0 * DS, 20.0 * DS
+BSTART, BLENGTH = -35.0 * DS, 25.0 * DS
+DT = 0.1 * DS
+FRINGE = (1.0 / DS, -45.0 * DS, 0.0, 15.0 * DS, 10.0 * DS)
+WALL = (0.01, 1.0, -20.0 * DS, -10.0 * DS, 3.0 * DS, 3.0 * DS, 0.5 / DS, 0.3 / DS)
+BLASIUS_F2 = 0.33205733621519630 * math.sqrt(2.0)
+BLASIUS_D1 = 1.7207876575205 / math.sqrt(2.0)
+
+
+@pytest.fixture(scope="module")
+def golden():
+    return np.load(os.path.join(ROOT, "tests", "golden", "blasius_64x33x8.npz"))
+
+
+@pytest.fixture(scope="module")
+def table():
+    return baseflow.fsc_table()
+
+
+def rel(a, b):
+    a = np.delete(a, NZ // 2, axis=1)
+    b = np.delete(b, NZ // 2, axis=1)
+    return [float(np.linalg.norm(a[c] - b[c]) / np.linalg.norm(b[c])) for c in range(3)]
+
+
+def test_table_equals_reference_fsc(golden, table):
+    assert table.tab.shape == (20001, 8)
+    assert np.array_equal(table.tab[::20], golden["table"])
+    assert table.dstar2 == float(golden["dstar2"])
+
+
+def test_table_closed_forms(table):
+    t = table.tab
+    assert abs(t[0, 3] - BLASIUS_F2) < 1e-12
+    assert abs(table.dstar2 - BLASIUS_D1) < 1e-9
+    assert abs(t[-1, 2] - 1.0) < 1e-14 and abs(t[-1, 5] - 1.0) < 1e-14
+    assert np.abs(t[:, 4] + t[:, 1] * t[:, 3]).max() < 1e-15       # f''' = -f f''
+    assert np.abs(t[:, 7] + t[:, 6] * t[:, 1]).max() < 1e-15       # g'' = -f g'
+    # an independent adaptive integration from the table's f''(0)
+    from scipy.integrate import solve_ivp
+    sol = solve_ivp(lambda e, y: [y[1], y[2], -y[0] * y[2]], (0.0, 10.0),
+                    [0.0, 0.0, t[0, 3]], rtol=1e-13, atol=1e-15, dense_output=True,
+                    method="DOP853")
+    eta = t[t[:, 0] <= 10.0, 0]
+    ref = sol.sol(eta)
+    assert np.abs(ref[1] - t[:len(eta), 2]).max() < 1e-10
+    assert np.abs(ref[0] - t[:len(eta), 1]).max() < 1e-10
+
+
+def test_fsc_dat_round_trip(tmp_path, table):
+    p = str(tmp_path / "fsc.dat")
+    baseflow.write_fsc(p, table)
+    back = baseflow.read_fsc(p)
+    assert np.array_equal(back.tab, table.tab) and back.dstar2 == table.dstar2
+    assert os.path.getsize(p) == (4 + 40 + 4) + 20001 * (4 + 64 + 4)   # record markers
+
+
+def test_base_flow_closed_forms(golden, table):
+    base, bj, du0upp = baseflow.spatial_base_flow(NX, NY, XL, RE, DS, table, BSTART, BLENGTH)
+    U, V = base[0], base[1]
+    eta = np.cos(np.pi * np.arange(NY) / (NY - 1))
+    assert np.abs(V[-1]).max() == 0.0 or np.abs(V[-1]).max() < 1e-15     # v(-1) = 0
+    assert np.abs(U[-1]).max() < 1e-15 and np.abs(base[2]).max() == 0.0
+    # continuity: v = -int_{-1}^{y} du/dx with du/dx from the x modes (the
+    # Nyquist mode dropped) and numpy's Chebyshev integral, truncated to
+    # degree ny - 1 as icheb.f keeps it before v(-1) = 0 is set
+    C = np.polynomial.chebyshev
+    Uh = np.fft.fft(U, axis=1) / NXP
+    Uh[:, NXP // 2] = 0.0
+    alfa = 2 * np.pi / XL * np.fft.fftfreq(NXP, 1.0 / NXP)
+    dudx = np.fft.ifft(1j * alfa[None, :] * Uh, axis=1).real * NXP
+    ci = C.chebint(C.chebfit(eta, dudx, NY - 1))[:NY]
+    ci[0] -= C.chebval(-1.0, ci)
+    v_ref = -C.chebval(eta, ci).T
+    assert np.abs(V - v_ref).max() < 1e-11 * np.abs(V).max()    # chebfit rounds at ~1e-13
+    # complete blending: the profile at the local distance xb + x0
+    x0 = DS ** 2 * RE / (2 * table.dstar2 ** 2)
+    assert abs(x0 / DS - float(golden["x0_log"])) < 1e-10
+    xs = (np.arange(NXP) - NXP // 2) / NXP * XL
+    j = int(np.argmin(np.abs(xs - 30.0 * DS)))         # well past bstart + blength
+    s = np.sqrt(RE / (2 * (xs[j] + x0)))
+    prof = np.interp((1 + eta) * s, table.tab[:, 0], table.tab[:, 2])
+    assert np.abs(U[:, j] - prof).max() < 1e-6          # linear vs cubic interpolation
+    # du0upp: the reference prints du0upp * dstar
+    assert abs(du0upp * DS - float(golden["du0upp_log"])) < 1e-12 * abs(du0upp * DS) + 1e-20
+    assert abs(bj[0, 0].real - du0upp) == 0.0
+
+
+def test_base_flow_matches_restatement(table):
+    base, bj, du = baseflow.spatial_base_flow(NX, NY, XL, RE, DS, table, BSTART, BLENGTH)
+    rb, rbj, rdu = baseflow_ref.base_flow(NX, NY, XL, RE, DS, table.tab, table.dstar2, BSTART,
+                                          BLENGTH)
+    assert np.abs(base[0] - rb[0]).max() < 1e-15
+    # v is built from du/dx of u = O(1) through alpha up to ~30 and the y
+    # integral: ~1e-13 absolute rounding in either implementation
+    assert np.abs(base[1] - rb[1]).max() < 1e-12
+    assert np.abs(bj - rbj).max() < 1e-12 * np.abs(rbj).max() + 1e-13
+    assert abs(du - rdu) < 1e-13
+
+
+def run_oracle(golden, case):
+    from pyoracle import Oracle
+    tab = baseflow.fsc_table()
+    rb, rbj, rdu = baseflow_ref.base_flow(NX, NY, XL, RE, DS, tab.tab, tab.dstar2, BSTART,
+                                          BLENGTH)
+    o = Oracle(NX, NY, NZ, RE, XL, ZL, cflux=False, retau=0.0)
+    out = []
+    try:
+        ur, ui, puw = o.new_state(golden["init"])
+        o.px = 0.0
+        o.set_fringe(fringe_ref.fringe_lambda(NXP, XL, *FRINGE), rb)
+        o.set_bc(101, rbj, rdu)
+        wall = (lambda tc: wall_ref.wall_v(NX, NZ, XL, ZL, *WALL, tc)) if case == "b" else None
+        for n, t0 in ((1, 0.0), (19, DT)):
+            if wall:
+                wall_ref.run(o, ur, ui, puw, n, DT, t0, wall)
+            else:
+                st, _ = o.run(ur, ui, puw, n, DT)
+                assert st == 0
+            out.append(ur[:3] + 1j * ui[:3])
+    finally:
+        o.close()
+    return out, rbj, rdu
+
+
+@pytest.mark.parametrize("case", ["a", "b"])
+def test_oracle_matches_reference(golden, case):
+    (u1, u20), rbj, rdu = run_oracle(golden, case)
+    e1, e20 = rel(u1, golden[f"{case}_u1"]), rel(u20, golden[f"{case}_u20"])
+    print(f"oracle blasius {case}: 1 step {e1}, 20 steps {e20}")
+    assert max(e1) <= 1e-12 and max(e20) <= 1e-12, (e1, e20)
+
+
+def dtop(col, eta, order=1):
+    c = np.polynomial.chebyshev.chebfit(eta, col, len(eta) - 1)
+    return np.polynomial.chebyshev.chebval(1.0, np.polynomial.chebyshev.chebder(c, order))
+
+
+def test_free_stream_conditions_hold(golden):
+    """The oracle's field after 20 steps: du/dy, dv/dy, dw/dy at y = +1 are the
+    base flow's in the kz = 0 plane and zero elsewhere; no-slip below."""
+    (_, u), rbj, rdu = run_oracle(golden, "a")
+    eta = np.cos(np.pi * np.arange(NY) / (NY - 1))
+    worst = 0.0
+    for z in range(NZ):
+        if z == NZ // 2:
+            continue
+        for x in range(NX // 2):
+            for c in range(3):
+                d = dtop(u[c, z, :, x].real, eta) + 1j * dtop(u[c, z, :, x].imag, eta)
+                target = rbj[c, x] if z == 0 else 0.0
+                if z == 0 and x == 0:
+                    target = rdu if c == 0 else (0.0 if c == 2 else d)
+                worst = max(worst, abs(d - target))
+    assert worst < 1e-10, worst
+    assert np.abs(np.delete(u[:, :, -1, :], NZ // 2, axis=1)).max() < 1e-13
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+@pytest.mark.parametrize("case", ["a", "b"])
+def test_gpu_matches_reference(golden, case, form):
+    from simson_amd.gpu import SIMSON_FP64, SIMSON_MIXED, BlaGPU
+    print(f"[gpu] spatial Blasius {case} ({form}) {NX}x{NY}x{NZ}, ibc 101", flush=True)
+    tab = baseflow.fsc_table()
+    base, bj, du0upp = baseflow.spatial_base_flow(NX, NY, XL, RE, DS, tab, BSTART, BLENGTH)
+    prec = SIMSON_MIXED if form == "mixed" else SIMSON_FP64
+    tol = 1e-12 if form == "fp64" else 2e-5
+    with BlaGPU(NX, NY, NZ, RE, XL, ZL, cflux=False, retau=0.0, precision=prec, ibc=101) as g:
+        g.fringe(*FRINGE, base=base)
+        g.free_stream(bj, du0upp)
+        if case == "b":
+            g.wall(*WALL)
+        g.upload_velocity(golden["init"], 0.0)
+        g.run(1, DT, icfl=3)
+        e1 = rel(g.velocities(), golden[f"{case}_u1"])
+        g.run(19, DT, icfl=3, first=False)
+        e20 = rel(g.velocities(), golden[f"{case}_u20"])
+    print(f"    after 1 step {e1}, 20 steps {e20}", flush=True)
+    assert max(e1) <= tol and max(e20) <= tol, (e1, e20)
