@@ -14,7 +14,9 @@ instrumented stretch of steps gives the per-kernel table.
 
 Also measured: the FP64 step (mixed_vs_fp64), the mixed step with each
 subgrid model, the large channel 384 x 257 x 384 (Re_cl 13000, lx 2 pi,
-lz pi, dt 0.002, 10% rms over |kx|,|kz| <= 32) in both forms, and the
+lz pi, dt 0.002, 10% rms over |kx|,|kz| <= 32) in both forms, the spatial
+Blasius boundary layer 512 x 97 x 64 (Re_delta* 400 at the inflow, fringe,
+free stream above, wall strip at F = 100) in both forms, and the
 mixed form's error: the profile deviation from FP64 after the bench's own
 steps and, over 10^4 steps from a 2000-step spun-up state, the averaged
 statistics against two FP64 controls perturbed at 1e-7.
@@ -260,6 +262,36 @@ def time_form(w: Workload, precision: int, steps: int, warmup: int, model: str =
     return {"sps": steps / (t1 - t0), "sec": t1 - t0, "kern": kern, "cfl": cfl, "u": u}
 
 
+def time_spatial(precision: int, steps: int, warmup: int):
+    """The spatial Blasius boundary layer forced at F = 100 (512 x 97 x 64,
+    Re_delta* 400 at the inflow, box 1000 x 30 x 50 delta*, fringe, ibc 101,
+    the lower-wall strip, dt 0.02 delta*/U): steps/s of the step with the
+    fringe, the strip's wall velocity and the free-stream boundary rows."""
+    from simson_amd.baseflow import tollmien_schlichting_case
+    from simson_amd.gpu import BlaGPU
+
+    c = tollmien_schlichting_case()
+    f = c.field
+    g = BlaGPU(f.nx, f.nyp, f.nz, f.re, f.xl, f.zl, cflux=False, retau=0.0,
+               precision=precision, ibc=101)
+    try:
+        g.fringe(*c.fringe, base=c.base)
+        g.free_stream(c.bj, c.du0upp)
+        g.wall(*c.wall)
+        g.upload_velocity(f.u, 0.0)
+        g.run(warmup, c.dt, icfl=3, first=True)
+        g.sync()
+        t0 = time.perf_counter()
+        cfl = g.run(steps, c.dt, icfl=3, first=False)
+        g.sync()
+        t1 = time.perf_counter()
+    finally:
+        g.close()
+    sps = steps / (t1 - t0)
+    return {"steps_per_s": round(sps, 2), "ms_per_step": round(1e3 / sps, 4),
+            "grid_points_steps_per_s": round(sps * f.nx * f.nyp * f.nz, 1), "cfl_dt": cfl}
+
+
 def time_entry_points(reps: int = 5):
     """Milliseconds per call of the FP64 statistics and pressure entry points
     on the headline workload (boxamp, boxxys 1-15, nonlinp + linearp), each
@@ -375,6 +407,9 @@ def main():
                          "adm-rt, smag, learned; empty for none), reported under forms")
     ap.add_argument("--large-steps", type=int, default=20,
                     help="timed steps of the 384x257x384 channel per form (0: skip)")
+    ap.add_argument("--spatial-steps", type=int, default=200,
+                    help="timed steps of the 512x97x64 spatial boundary layer per form "
+                         "(0: skip)")
     ap.add_argument("--no-long-check", dest="long_check", action="store_false",
                     help="skip the 10^4-step mixed-vs-FP64 statistics check")
     ap.add_argument("--no-entry-points", dest="entry_points", action="store_false",
@@ -416,6 +451,12 @@ def main():
             r = time_form(LARGE, prec, args.large_steps, 2, isteps=3)
             large[name] = form_entry(r, prec == SIMSON_MIXED, LARGE)
             say(f"bench: {LARGE.tag} {name} {r['sps']:.2f} steps/s")
+    spatial = {}
+    if args.spatial_steps > 0:
+        for name, prec in (("fp64", SIMSON_FP64), ("mixed", SIMSON_MIXED)):
+            spatial[name] = time_spatial(prec, args.spatial_steps, 5)
+            say(f"bench: spatial boundary layer 512x97x64 {name} "
+                f"{spatial[name]['steps_per_s']:.1f} steps/s")
     entry = time_entry_points() if args.entry_points else None
     err = None
     if "fp64" in res and "mixed" in res:
@@ -480,6 +521,10 @@ def main():
         "large_channel": ({"workload": f"{LARGE.label} {LARGE.tag}, Re_cl {LARGE.re:g}, "
                                        f"lx 2pi, lz pi, dt {LARGE.dt}", **large}
                           if large else None),
+        "spatial_boundary_layer": ({"workload": "spatial Blasius boundary layer 512x97x64, "
+                                                "Re_delta* 400, fringe, ibc 101, wall strip "
+                                                "at F = 100, dt 0.02", **spatial}
+                                   if spatial else None),
         "entry_points_ms": entry,
         "cpu_baseline": cpu,
     }

@@ -40,8 +40,9 @@ program blagpu
   type(simson_fringe_t) :: fr
   type(simson_wall_t) :: wl
   real(dp) :: cnrk(3)
-  real(dp) :: ampob, amp2d, bstart, blength
-  real(dp), allocatable :: base(:,:,:)
+  real(dp) :: ampob, amp2d, bstart, blength, rlam, spanv, du0upp
+  real(dp), allocatable :: base(:,:,:), bjr(:,:), bji(:,:), tab(:,:)
+  type(simson_baseflow_t) :: bf
   ! field and header
   integer :: nx, nyp, nzc, nfzsym, fltype
   real(dp) :: re, xlf, zl, t, xs, dstar
@@ -192,7 +193,8 @@ program blagpu
   call nxt(line); read(line, *) npl
   close(10)
   if (pert .or. cim .or. suction .or. isfd /= 0 .or. imhd /= 0 .or. &
-       loctyp /= 0 .or. tripf .or. (wbci /= 0 .and. wbci /= 1) .or. ibc /= 0 .or. nst /= 3 &
+       loctyp /= 0 .or. tripf .or. (wbci /= 0 .and. wbci /= 1) .or. &
+       (ibc /= 0 .and. ibc /= 101) .or. nst /= 3 &
        .or. mwave > 0 .or. npl > 0) then
      write(*,*) 'blagpu: bla.i requests a feature outside the channel step library'
      call stopnow(13)
@@ -244,11 +246,41 @@ program blagpu
   read(12) re, pou, xlf, zl, t, xs
   read(12) nx, nyp, nzc, nfzsym
   read(12) fltype, dstar
-  ! temporal (fltype 1) or spatial (fltype 4) Poiseuille flow
-  if (nfzsym /= 0 .or. .not. (fltype == 1 .or. (spat .and. fltype == 4))) call stopnow(13)
+  ! temporal (fltype 1) or spatial (fltype 4) Poiseuille flow, or a spatial
+  ! Blasius, Falkner-Skan or Falkner-Skan-Cooke boundary layer (6, 7, 8)
+  if (nfzsym /= 0 .or. .not. (fltype == 1 .or. &
+       (spat .and. (fltype == 4 .or. (fltype >= 6 .and. fltype <= 8))))) call stopnow(13)
+  if (ibc == 101 .and. fltype < 6) call stopnow(13)
+  if (fltype >= 6 .and. (iamp > 0 .or. ixys > 0)) then
+     write(*,*) 'blagpu: amplitude and x-y statistics files are written for channels only'
+     call stopnow(13)
+  end if
   bstart = 0._dp
   blength = 0._dp
-  if (fltype == 4 .or. fltype == 5) read(12) bstart, blength
+  rlam = 0._dp
+  spanv = 0._dp
+  if (fltype >= 4 .and. fltype <= 6) read(12) bstart, blength
+  if (fltype >= 7 .and. fltype <= 9) read(12) bstart, blength, rlam, spanv
+  ! bla.i gives lengths and times in the units of dstar = 2/yl (1 for
+  ! channels); scale them as rescale.f:33-110 does
+  tmax = tmax * dstar
+  dt = dt * dstar
+  dtmax = dtmax * dstar
+  tsave(1:msave) = tsave(1:msave) * dstar
+  txys = txys * dstar
+  rot = rot / dstar
+  xl = xl * dstar
+  fr%fstart = fr%fstart * dstar
+  fr%fend = fr%fend * dstar
+  fr%frise = fr%frise * dstar
+  fr%ffall = fr%ffall * dstar
+  fr%fmax = fr%fmax / dstar
+  wl%xstart = wl%xstart * dstar
+  wl%xend = wl%xend * dstar
+  wl%xrise = wl%xrise * dstar
+  wl%xfall = wl%xfall * dstar
+  wl%zbet = wl%zbet / dstar
+  wl%tomeg = wl%tomeg / dstar
   allocate(ur(nx/2, nyp, nzc, 7), ui(nx/2, nyp, nzc, 7), urx(nx))
   ur = 0._dp
   ui = 0._dp
@@ -274,7 +306,7 @@ program blagpu
 
   ! ---------------- device state ----------------
   p%nx = nx; p%ny = nyp; p%nz = nzc
-  p%precision = SIMSON_FP64; p%ibc = 0
+  p%precision = SIMSON_FP64; p%ibc = ibc
   ! no pressure gradient unless temporal Poiseuille flow (bla.f:1622-1662)
   if (fltype /= 1) then
      cflux = .false.
@@ -293,12 +325,24 @@ program blagpu
      px = -(retau / re)**2                        ! bla.f:1645 (0 for fltype 4)
   end if
   if (spat) then
-     ! the Poiseuille base flow of fltype 4 at the 3/2 x grid (cbflow.f:117-131)
      allocate(base(3 * nx / 2, nyp, 3))
      base = 0._dp
-     do it = 1, nyp
-        base(:, it, 1) = 1._dp - cos(pi * real(it - 1, dp) / real(nyp - 1, dp))**2
-     end do
+     if (fltype == 4) then
+        ! the Poiseuille base flow of fltype 4 at the 3/2 x grid (cbflow.f:117-131)
+        do it = 1, nyp
+           base(:, it, 1) = 1._dp - cos(pi * real(it - 1, dp) / real(nyp - 1, dp))**2
+        end do
+     else
+        ! boundary layer: the similarity table fsc.dat (rbla.f), the base flow
+        ! of cbflow.f and its free-stream derivatives (bflow.f)
+        call read_fsc()
+        bf%nx = nx; bf%ny = nyp; bf%fltype = fltype
+        bf%xl = xl; bf%re = re; bf%dstar = dstar; bf%bstart = bstart; bf%blength = blength
+        bf%rlam = rlam; bf%spanv = spanv; bf%xsc = 0._dp; bf%u0low = 0._dp; bf%w0low = 0._dp
+        allocate(bjr(nx / 2, 3), bji(nx / 2, 3))
+        call chk(simson_base_flow(bf, tab, int(size(tab, 2), c_int), base, bjr, bji, du0upp))
+        if (ibc == 101) call chk(simson_gpu_free_stream(g, bjr, bji, du0upp))
+     end if
      call chk(simson_gpu_fringe(g, fr, base))
   end if
   if (wbci == 1) call chk(simson_gpu_wall(g, wl))
@@ -536,7 +580,25 @@ contains
     close(19)
   end subroutine write_xys
 
-  ! wdiscbl.f:50-136 (channel: no shift, no extra header records)
+  ! rbla.f:24-125: fsc.dat, header (0, ymax, n, m, dstar2, nscalar) and n
+  ! rows of eta, f, df, d2f, d3f, g, dg, d2g; m must match the flow's
+  subroutine read_fsc()
+    real(dp) :: a, b, m
+    integer :: n, nsc, r
+    open(17, file='fsc.dat', status='old', form='unformatted')
+    read(17) a, b, n, m, bf%dstar2, nsc
+    if (a /= 0._dp .or. nsc /= 0 .or. m /= rlam .or. n < 4) then
+       write(*,*) '** Please provide suitable base flow in fsc.dat'
+       call stopnow(123554)
+    end if
+    allocate(tab(8, n))
+    do r = 1, n
+       read(17) tab(:, r)
+    end do
+    close(17)
+  end subroutine read_fsc
+
+  ! wdiscbl.f:50-136 (no shift)
   subroutine save_field(name)
     character(len=*), intent(in) :: name
     integer :: c, z, y
@@ -546,7 +608,8 @@ contains
     write(11) re, .false., xl, zl, t, 0._dp
     write(11) nx, nyp, nzc, nfzsym
     write(11) fltype, dstar
-    if (fltype == 4 .or. fltype == 5) write(11) bstart, blength
+    if (fltype >= 4 .and. fltype <= 6) write(11) bstart, blength
+    if (fltype >= 7 .and. fltype <= 9) write(11) bstart, blength, rlam, spanv
     do c = 1, 3
        do z = 1, nzc
           do y = 1, nyp
@@ -572,7 +635,8 @@ contains
     write(11) re, .false., xl, zl, t, 0._dp
     write(11) nx, nyp, nzc, nfzsym
     write(11) fltype, dstar
-    if (fltype == 4 .or. fltype == 5) write(11) bstart, blength
+    if (fltype >= 4 .and. fltype <= 6) write(11) bstart, blength
+    if (fltype >= 7 .and. fltype <= 9) write(11) bstart, blength, rlam, spanv
     do z = 1, nzc
        do y = 1, nyp
           urx(1:nx:2) = prr(:, y, z)

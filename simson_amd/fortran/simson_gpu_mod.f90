@@ -32,6 +32,12 @@ module simson_gpu_mod
      real(c_double) :: amp, damp, xstart, xend, xrise, xfall, zbet, tomeg, xsc, zsc
   end type simson_wall_t
 
+  ! spatial boundary-layer base flow (cbflow.f), units of the .u file
+  type, bind(c) :: simson_baseflow_t
+     integer(c_int) :: nx, ny, fltype
+     real(c_double) :: xl, re, dstar, dstar2, bstart, blength, rlam, spanv, xsc, u0low, w0low
+  end type simson_baseflow_t
+
   type, bind(c) :: simson_les_t
      integer(c_int) :: iles
      integer(c_int) :: iord
@@ -124,6 +130,24 @@ module simson_gpu_mod
        type(simson_fringe_t), intent(in) :: f
        real(c_double), intent(in) :: base(*)
      end function simson_gpu_fringe
+
+     integer(c_int) function simson_gpu_free_stream(g, bjr, bji, du0upp) &
+          bind(c, name='simson_gpu_free_stream')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: g
+       real(c_double), intent(in) :: bjr(*), bji(*)
+       real(c_double), value :: du0upp
+     end function simson_gpu_free_stream
+
+     integer(c_int) function simson_base_flow(b, tab, ntab, base, bjr, bji, du0upp) &
+          bind(c, name='simson_base_flow')
+       import :: c_int, c_double, simson_baseflow_t
+       type(simson_baseflow_t), intent(in) :: b
+       real(c_double), intent(in) :: tab(*)
+       integer(c_int), value :: ntab
+       real(c_double), intent(out) :: base(*), bjr(*), bji(*)
+       real(c_double), intent(out) :: du0upp
+     end function simson_base_flow
 
      integer(c_int) function simson_gpu_wall(g, w) bind(c, name='simson_gpu_wall')
        import :: c_int, c_ptr, simson_wall_t
