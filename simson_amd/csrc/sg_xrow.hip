@@ -30,24 +30,34 @@ namespace sg {
 
 namespace {
 
+// M1 x M2: from M = 288 (the split form below) M2 = 16, so that the
+// column DFTs of passes 1 and B stay small; the M1-point block DFTs then
+// run one field at a time
 template <int M>
 struct XPlan {
-    static constexpr int M1 = (M == 384) ? 12
-                              : (M % 4 == 0) ? ((M > 96 && M % 8 == 0) ? 8 : 4)
-                                             : ((M % 2 == 0) ? 2 : 3);
+    static constexpr int M1 = (M >= 288 && M % 16 == 0)
+                                  ? M / 16
+                                  : (M % 4 == 0) ? ((M > 96 && M % 8 == 0) ? 8 : 4)
+                                                 : ((M % 2 == 0) ? 2 : 3);
     static constexpr int M2 = M / M1;
     static constexpr int SROW = M + M / M1;
     static constexpr int TABN = 2 * M;
 };
 
+// SPLIT (from M = 288): the inverse pass 2, the products and the forward
+// pass A run as three LDS passes (per field and block, per physical point,
+// per output field and block) instead of one fused task per block, which
+// would hold seven M1-value arrays in registers (one wave per SIMD at
+// these lengths) with only RW M2 tasks to spread over the block.
 template <typename V, int M>
 struct XBlock {
+    static constexpr bool SPLIT = M >= 288;
     static constexpr int budget = 52 * 1024;
     static constexpr int per_row = 6 * XPlan<M>::SROW * (int)sizeof(V);
     static constexpr int raw = (budget - XPlan<M>::TABN * (int)sizeof(V)) / per_row;
     static constexpr int RW = raw < 1 ? 1 : (raw > 8 ? 8 : raw);
     static constexpr int tasks1 = 6 * RW * XPlan<M>::M1;
-    static constexpr int tasks2 = RW * XPlan<M>::M2;
+    static constexpr int tasks2 = SPLIT ? RW * M : RW * XPlan<M>::M2;
     static constexpr int tmax = tasks1 > tasks2 ? tasks1 : tasks2;
     static constexpr int NT = tmax >= 256 ? 256 : ((tmax + 63) / 64) * 64;
     static constexpr size_t lds = (size_t)(XPlan<M>::TABN + 6 * RW * XPlan<M>::SROW) * sizeof(V);
@@ -180,10 +190,86 @@ __global__ void __launch_bounds__((XBlock<V, M>::NT)) k_xrow4(Step2Args<V> a, co
         Oxm = a.U[3 * F + yo].x;
         Ozm = a.U[4 * F + yo].x;
     }
+    constexpr int FS = RW * SROW;                // field stride in A
+    if constexpr (XBlock<V, M>::SPLIT) {
+        // inverse pass 2: an M1-point DFT per field and block, in place
+        for (int t = tid; t < 6 * RW * M2; t += NT) {
+            const int fr = t / M2, k2 = t - fr * M2;
+            V *blk = A + fr * SROW + S1 * k2;
+            V q[M1];
+#pragma unroll
+            for (int i = 0; i < M1; i++)
+                q[i] = blk[i];
+            reg_dft<V, M1, +1, TABN>(q, tw);
+#pragma unroll
+            for (int i = 0; i < M1; i++)
+                blk[i] = q[i];
+        }
+        __syncthreads();
+        // products at each physical point n = k2 + M2 i (slot S1 k2 + i of
+        // its row), H into the slots of u, v, w (and u.u into om_x's); the
+        // same expressions as the fused form below
+        const T zr = Ozm + r2;
+        for (int t = tid; t < RW * M; t += NT) {
+            const int r = t / M, p = t - r * M;
+            const int k2 = p / M1, i = p - k2 * M1;
+            V *pt = A + r * SROW + S1 * k2 + i;
+            const V u = pt[0], v = pt[FS], w = pt[2 * FS];
+            const V ox = pt[3 * FS], oy = pt[4 * FS], oz = pt[5 * FS];
+            if (a.do_cfl && r < nr) {
+                const T ce = fabs(Um + u.x) * dxi + fabs(v.x) * dyinv + fabs(Wm + w.x) * dzi;
+                const T co = fabs(Um + u.y) * dxi + fabs(v.y) * dyinv + fabs(Wm + w.y) * dzi;
+                cflmax = cfl_max(cfl_max(cflmax, ce), co);
+            }
+            V h2 = vmk<V>((Wm + w.x) * ox.x + w.x * Oxm, (Wm + w.y) * ox.y + w.y * Oxm);
+            V h3 = vmk<V>(-v.x * (Oxm + ox.x), -v.y * (Oxm + ox.y));
+            V h1 = vmk<V>(-(Wm + w.x) * oy.x, -(Wm + w.y) * oy.y);
+            h3 = vmk<V>(h3.x + (Um + u.x) * oy.x, h3.y + (Um + u.y) * oy.y);
+            const T zx = oz.x + zr, zy = oz.y + zr;
+            h1 = vmk<V>(v.x * zx + h1.x, v.y * zy + h1.y);
+            h2 = vmk<V>(h2.x - (Um * oz.x + u.x * zx), h2.y - (Um * oz.y + u.y * zy));
+            if (a.fringe) {
+                const int n = k2 + M2 * i;
+                const V lam = a.flam[n];
+                const V b0 = a.fbase[(size_t)y * M + n];
+                const V b1 = a.fbase[((size_t)ny + y) * M + n];
+                const V b2 = a.fbase[((size_t)2 * ny + y) * M + n];
+                h1 = vmk<V>(h1.x - lam.x * (Um + u.x - b0.x), h1.y - lam.y * (Um + u.y - b0.y));
+                h2 = vmk<V>(h2.x - lam.x * (v.x - b1.x), h2.y - lam.y * (v.y - b1.y));
+                h3 = vmk<V>(h3.x - lam.x * (Wm + w.x - b2.x), h3.y - lam.y * (Wm + w.y - b2.y));
+            }
+            if constexpr (PRESS)
+                pt[3 * FS] = vmk<V>(u.x * u.x + v.x * v.x + w.x * w.x,
+                                    u.y * u.y + v.y * v.y + w.y * w.y);
+            pt[0] = h1;
+            pt[FS] = h2;
+            pt[2 * FS] = h3;
+        }
+        __syncthreads();
+        // forward pass A per output field and block
+        constexpr int NOA = PRESS ? 4 : 3;
+        for (int t = tid; t < NOA * RW * M2; t += NT) {
+            const int hr = t / M2, k2 = t - hr * M2;
+            V *blk = A + hr * SROW + S1 * k2;
+            V q[M1];
+#pragma unroll
+            for (int i = 0; i < M1; i++)
+                q[i] = blk[i];
+            reg_dft<V, M1, -1, TABN>(q, tw);
+#pragma unroll
+            for (int c = 0; c < M1; c++) {
+                if (c > 0 && k2 > 0) {
+                    V wt = tw[2 * k2 * c];
+                    wt.y = -wt.y;
+                    q[c] = vmul(q[c], wt);
+                }
+                blk[c] = q[c];
+            }
+        }
+    } else
     for (int t = tid; t < RW * M2; t += NT) {
         const int r = t / M2, k2 = t - r * M2;
         V *blk = A + r * SROW + S1 * k2;         // field f at blk + f * RW * SROW
-        constexpr int FS = RW * SROW;
         V u[M1], v[M1], w[M1], o[M1], h1[M1], h2[M1], h3[M1];
 #pragma unroll
         for (int i = 0; i < M1; i++) {

@@ -117,3 +117,36 @@ def test_ts_wave_mixed_matches_fp64():
         " ".join(f"{xx:.0f}:{aa:.1e}" for xx, aa in zip(x[win][::16], a64[::16])))
     assert a64.max() > 100 * 1e-6
     assert dev < 2e-2, dev
+
+
+@pytest.mark.parametrize("nx", [384, 512])
+def test_long_x_rows_match_oracle(nx):
+    """Channel steps with x rows of 576 and 768 points (the three-pass
+    split form of the x-row kernel, nx/2 = 288 and 384 complex points),
+    FP64 against the oracle after two steps (1e-12), and the pressure of the
+    same state (its x-row variant, 1e-9 as in tests/test_pressure.py)."""
+    from pyoracle import Oracle
+    from simson_amd import fields
+    ny, nz, re = 33, 8, 3000.0
+    xl, zl = 4 * np.pi, np.pi
+    say(f"[gpu] channel {nx}x{ny}x{nz}: two FP64 steps and the pressure vs oracle")
+    f = fields.channel_field(nx, ny, nz, xl, zl, re, kmax=12, rms=5e-2, seed=11)
+    o = Oracle(nx, ny, nz, re, xl, zl)
+    try:
+        ur, ui, puw = o.new_state(f.u)
+        o.run(ur, ui, puw, 2, 0.005, icfl=0)
+        ref = ur[:3] + 1j * ui[:3]
+        pref = o.pressure(ur, ui)
+    finally:
+        o.close()
+    with BlaGPU(nx, ny, nz, re, xl, zl, cflux=True) as g:
+        g.upload_velocity(f.u, -2.0 / re)
+        g.run(2, 0.005, icfl=0)
+        u = g.velocities()
+        p = g.pressure()
+    e = rel(u, ref, nz)
+    d = np.delete(p - pref, nz // 2, axis=0)
+    ep = float(np.linalg.norm(d) / np.linalg.norm(np.delete(pref, nz // 2, axis=0)))
+    say(f"    velocities {['%.1e' % v for v in e]}, pressure {ep:.1e}")
+    assert max(e) < 1e-12, e
+    assert ep < 1e-9, ep
