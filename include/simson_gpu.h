@@ -256,12 +256,14 @@ int simson_wall_velocity(int nx, int nz, double xl, double zl, const simson_wall
  * returns 0 or SIMSON_E_ARG (bad input, m < -0.0904 or no convergence). */
 int simson_fsc_table(double m, int n, double ymax, double eps, double *tab, double *dstar2);
 
-/* Spatial boundary-layer base flow of cbflow.f:161-300 (fltype 6 Blasius,
- * 7 Falkner-Skan with m = rlam, 8 Falkner-Skan-Cooke with spanv), lengths
- * in the units of the .u file (re and xl as stored there, dstar = 2/yl). */
+/* Boundary-layer base flows of cbflow.f: spatial (161-300; fltype 6
+ * Blasius, 7 Falkner-Skan with m = rlam, 8 Falkner-Skan-Cooke with spanv)
+ * and temporal (58-84; fltype 3, -1, -2: the profile at x0 at every x),
+ * lengths in the units of the .u file (re and xl as stored there, dstar =
+ * 2/yl). */
 typedef struct simson_baseflow {
     int nx, ny;          /* par.f nx, ny */
-    int fltype;          /* 6, 7 or 8 */
+    int fltype;          /* 6, 7, 8 (spatial) or 3, -1, -2 (temporal) */
     double xl, re;       /* box length and Reynolds number of the .u file */
     double dstar;        /* .u dstar: displacement thickness at the box
                             centre in the units of y + 1 */
@@ -281,6 +283,49 @@ typedef struct simson_baseflow {
  * value.  Host only; returns 0 or SIMSON_E_ARG. */
 int simson_base_flow(const simson_baseflow_t *b, const double *tab, int ntab, double *base,
                      double *bjr, double *bji, double *du0upp);
+
+/* The forcing that keeps a temporal boundary layer's parallel profile from
+ * diffusing (nonlinbl.f:771-795; fltype 3, -1, -2): force [3][ny] at
+ * eta_y = cos(pi y/(ny-1)), the x, y, z components of the mean force at the
+ * boundary layer's position x0 + cdev t (bla.f:1826), from the similarity
+ * table tab [ntab][8].  Host only; returns 0 or SIMSON_E_ARG. */
+int simson_parallel_force(const simson_baseflow_t *b, const double *tab, int ntab, double cdev,
+                          double t, double *force);
+
+/* A mean force f [3][ny] (physical y, e.g. from simson_parallel_force)
+ * added to the kx = kz = 0 mode of the nonlinear term H in every following
+ * substep, as nonlinbl.f:771-795 adds it (the y component does not enter:
+ * the mean v is the wall's); NULL switches it off.  A time-dependent force
+ * (cdev != 0) is the caller's to update between substeps.  Synchronises. */
+int simson_gpu_mean_force(simson_gpu_t *g, const double *force);
+
+/* Localized volume force of bla.i loctyp = 3 (locf.f:358-410), in the
+ * units of the .u file (lengths times dstar, amplitudes and the frequency
+ * over dstar, as rescale.f scales bla.i's fpdds/fpds):
+ *   F_i = amp_i exp(-((1 + y)/yscale)^2) f(x) sin(tomeg tc),
+ *   f(x) = ((x - xloc0)/xscale)^xtype exp(-((x - xloc0)/xscale)^2),
+ * x from the box centre (plus xsc), uniform in z, added to the nonlinear
+ * term at the time tc of the start of each substep. */
+typedef struct simson_locf {
+    double ampx, ampy, ampz;   /* fpdds1..3 */
+    double xscale, xloc0;      /* fpds1, fpds2 */
+    double yscale;             /* fpds3 */
+    double xtype;              /* fp1: 0 or 1 */
+    double tomeg;              /* fpdds4 */
+    double xsc;                /* frame position */
+} simson_locf_t;
+
+/* Switch the localized force on (lf) or off (NULL); it is applied to every
+ * later substep with the time of simson_gpu_set_time / simson_gpu_run.
+ * Synchronises. */
+int simson_gpu_local_force(simson_gpu_t *g, const simson_locf_t *lf);
+
+/* The force's spatial pattern without a device: pr/pi [3][ny][nx/2], the
+ * x Fourier coefficients (over the 3/2 grid, divided by nxp) of the kz = 0
+ * plane at eta_y = cos(pi y/(ny-1)), before the factor sin(tomeg tc).
+ * Returns 0 or SIMSON_E_ARG. */
+int simson_local_force_pattern(int nx, int ny, double xl, const simson_locf_t *lf, double *pr,
+                               double *pi);
 
 /* Free-stream data of ibc = 101 (replaces evalbc.f:142-159's use of
  * bflow.f's bu1jr and du0upp): bjr/bji [3][nx/2], component-major, the

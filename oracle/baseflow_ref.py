@@ -1,5 +1,5 @@
-"""CPU restatement of bla's spatial boundary-layer base flow and of its
-free-stream data (test infrastructure).
+"""CPU restatement of bla's boundary-layer base flows, their free-stream
+data and the temporal layer's forcing (test infrastructure).
 
 Only the tests use this module, as the checker of the library's host-side
 base flow (simson_base_flow).  It follows:
@@ -20,14 +20,18 @@ base flow (simson_base_flow).  It follows:
                 alpha, the Chebyshev integral of icheb.f with V(-1) = 0,
                 back to physical points;
 - bflow.f:80-105    dU_i/dy at y = +1 with the weights of preprbl.f:88-121
-                (sum_k k^2 a_k), then x Fourier coefficients / nxp.
+                (sum_k k^2 a_k), then x Fourier coefficients / nxp;
+- cbflow.f:58-84 and nonlinbl.f:771-795  the temporal boundary layer's
+                parallel profile and the mean force that keeps it from
+                diffusing.
 
 Here in numpy's FFTs and a dense cosine transform, independent of the
 library's code.  Pinned by tests/test_blasius.py: the similarity profile's
 closed forms (f''(0), the displacement thickness), continuity of (U, V),
 the Blasius wall-normal velocity where the blending is complete, and the
 reference bla's spatial Blasius run (tests/golden/blasius_64x33x8.npz,
-which also holds the du0upp and x0 that bla prints).
+which also holds the du0upp and x0 that bla prints), and the reference's
+temporal Blasius run (tests/golden/tblasius_32x33x16.npz).
 """
 from __future__ import annotations
 
@@ -125,3 +129,39 @@ def base_flow(nx: int, ny: int, xl: float, re: float, dstar: float, tab: np.ndar
         base.transpose(1, 0, 2).reshape(ny, -1)).reshape(ny, 3, nxp).transpose(1, 0, 2))
     bj = np.fft.fft(d, axis=1)[:, :nx // 2] / nxp
     return base, bj, float(bj[0, 0].real)
+
+
+def temporal_base_flow(nx: int, ny: int, re: float, dstar: float, tab: np.ndarray,
+                       dstar2: float, rlam: float = 0.0, cdev: float = 0.0, t: float = 0.0):
+    """(base [3][ny][nxp], bj [3][nx/2], du0upp, force [3][ny]) of a temporal
+    boundary layer without spanwise velocity (fltype 3, -1): cbflow.f:58-84
+    (the profile at x0 with eta = (1 + y) sqrt(re/(2 x0)) at every x),
+    bflow.f:80-105 for its derivatives at y = +1, and nonlinbl.f:771-795
+    for the forcing at xblc = x0 + cdev t (without the factor nxp nzp)."""
+    nxp = 3 * nx // 2
+    ymax = tab[-1, 0]
+    dyb = ymax / (len(tab) - 1)
+    x0 = dstar ** 2 * re * (rlam + 1.0) / (dstar2 ** 2 * 2.0)
+    eta = np.cos(np.pi * np.arange(ny) / (ny - 1))
+    s = math.sqrt(re / (2.0 * x0))
+    U = np.array([cubip((1.0 + e) * s, tab[:, 2], dyb) for e in eta])
+    base = np.zeros((3, ny, nxp))
+    base[0] = U[:, None]
+    k = np.arange(ny)
+    a = cheb_coeffs(U[:, None])[:, 0]
+    du = float(np.sum(k.astype(float) ** 2 * a))
+    bj = np.zeros((3, nx // 2), complex)
+    bj[0, 0] = du
+    xblc = x0 + cdev * t
+    xx = (xblc / x0) ** rlam
+    h1u = cdev * (rlam / x0) * (xblc / x0) ** (rlam - 1.0)
+    h3u = 0.5 * (rlam + 1.0) / xblc
+    force = np.zeros((3, ny))
+    for y, e in enumerate(eta):
+        ybl = 1.0 + e
+        eb = ybl * math.sqrt((rlam + 1.0) * re * xx / (2.0 * xblc))
+        h2u = (cdev * 0.5 * (rlam - 1.0) * xx ** 1.5 * math.sqrt(0.5 * (rlam + 1.0) * re)
+               * (1.0 / xblc) ** 1.5 * ybl)
+        force[0, y] = (cubip(eb, tab[:, 2], dyb) * h1u + cubip(eb, tab[:, 3], dyb) * h2u
+                       - cubip(eb, tab[:, 4], dyb) * xx * xx * h3u)
+    return base, bj, du, force

@@ -354,6 +354,7 @@ struct orc {
     int ibc;                           /* 0 (no-slip) or 101 (free stream above) */
     double *bjr, *bji;                 /* [3][nxh] dU_i/dy at y = 1 of the base flow */
     double du0upp;                     /* its mean (kx = 0) streamwise part */
+    double *mf;                        /* [3][ny] mean force or NULL */
 };
 
 struct les_s;
@@ -426,6 +427,7 @@ void orc_destroy(orc_t *o)
     free(o->wvi);
     free(o->bjr);
     free(o->bji);
+    free(o->mf);
     free(o->gur);
     free(o->gui);
     free(o);
@@ -570,6 +572,10 @@ static double nonlin_plane(orc_t *o, double *ur, double *ui, int y, int do_cfl)
                 }
             }
         }
+        /* parallel-flow forcing of temporal boundary layers on the mean
+         * mode of H_1 and H_3 (nonlinbl.f:771-795), times nxp nzp */
+        if (o->mf && f != 1)
+            ur[IDX(o, f, 0, y, 0)] += o->mf[(size_t)f * o->ny + y] * (double)nxp * (double)nzp;
     }
     free(B);
     free(R);
@@ -1566,4 +1572,14 @@ int orc_set_bc(orc_t *o, int ibc, const double *bjr, const double *bji, double d
         memcpy(o->bji, bji, sizeof(double) * m);
     }
     return 0;
+}
+
+void orc_set_mean_force(orc_t *o, const double *f)
+{
+    free(o->mf);
+    o->mf = NULL;
+    if (!f)
+        return;
+    o->mf = malloc(sizeof(double) * 3 * (size_t)o->ny);
+    memcpy(o->mf, f, sizeof(double) * 3 * (size_t)o->ny);
 }

@@ -86,6 +86,11 @@ void orc_set_wall(orc_t *o, const double *vr, const double *vi);
  * part for the mean flow.  Returns 0, or -1 for another ibc. */
 int orc_set_bc(orc_t *o, int ibc, const double *bjr, const double *bji, double du0upp);
 
+/* Mean force f [3][ny] (physical y) added to the kx = kz = 0 mode of H_1
+ * and H_3 in every nonlinear step, as nonlinbl.f:771-795 adds the
+ * parallel-flow forcing of temporal boundary layers; NULL: off. */
+void orc_set_mean_force(orc_t *o, const double *f);
+
 /* complex DFT, y_k = sum_j x_j exp(sign 2 pi i jk/n), in place on (re,im) */
 void orc_cdft(int n, double *re, double *im, int sign);
 /* vrfftb convention: half spectrum X[0..n/2] (re,im) -> x[0..n-1] real */

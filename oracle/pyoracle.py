@@ -55,6 +55,7 @@ def lib():
         L.orc_set_fringe.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_set_wall.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_set_bc.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_double]
+        L.orc_set_mean_force.argtypes = [C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -143,6 +144,24 @@ class Oracle:
                                   du0upp)
         if st != 0:
             raise ValueError(f"unsupported ibc {ibc}")
+
+    def set_force(self, f=None):
+        """External force, complex [3][nz][ny][nx/2] at physical y, which the
+        nonlinear step subtracts (times nxp nzp) from H; None: off."""
+        if f is None:
+            lib().orc_set_force(self.h, None, None)
+            return
+        self._force = (np.ascontiguousarray(np.real(f), np.float64),
+                       np.ascontiguousarray(np.imag(f), np.float64))
+        lib().orc_set_force(self.h, self._force[0].ctypes.data, self._force[1].ctypes.data)
+
+    def set_mean_force(self, f=None):
+        """Mean force [3][ny] on the kx = kz = 0 mode of H; None: off."""
+        if f is None:
+            lib().orc_set_mean_force(self.h, None)
+            return
+        self._mf = np.ascontiguousarray(f, np.float64)
+        lib().orc_set_mean_force(self.h, self._mf.ctypes.data)
 
     def set_wall(self, vw=None):
         """Lower-wall v_w [nz][nx/2] (complex spectral) for later linear

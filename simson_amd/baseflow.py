@@ -86,6 +86,32 @@ def spatial_base_flow(nx: int, ny: int, xl: float, re: float, dstar: float,
     return base, bjr + 1j * bji, du.value
 
 
+def temporal_base_flow(nx: int, ny: int, xl: float, re: float, dstar: float,
+                       table: FscTable, fltype: int = 3, spanv: float = 0.0,
+                       cdev: float = 0.0, t: float = 0.0):
+    """Temporal boundary layer (fltype 3, -1, -2) in the units of the .u
+    file: the parallel base flow [3][ny][3nx/2], its free-stream data (bj,
+    du0upp) and the mean force [3][ny] of nonlinbl.f:771-795 at time t."""
+    p = SimsonBaseFlow(nx, ny, fltype, xl, re, dstar, table.dstar2, 0.0, 0.0, table.m,
+                       spanv, 0.0, 0.0, 0.0)
+    base = np.zeros((3, ny, 3 * nx // 2))
+    bjr = np.zeros((3, nx // 2))
+    bji = np.zeros_like(bjr)
+    du = C.c_double()
+    tab = np.ascontiguousarray(table.tab, np.float64)
+    st = lib().simson_base_flow(C.byref(p), tab.reshape(-1), tab.shape[0], base.reshape(-1),
+                                bjr.ctypes.data_as(C.c_void_p), bji.ctypes.data_as(C.c_void_p),
+                                C.byref(du))
+    if st != 0:
+        raise SimsonError(st, "invalid base-flow parameters")
+    force = np.zeros((3, ny))
+    st = lib().simson_parallel_force(C.byref(p), tab.reshape(-1), tab.shape[0], cdev, t,
+                                     force.reshape(-1))
+    if st != 0:
+        raise SimsonError(st, "invalid forcing parameters")
+    return base, bjr + 1j * bji, du.value, force
+
+
 @dataclasses.dataclass
 class SpatialCase:
     """A spatial boundary-layer run in the units of the .u file (lengths

@@ -66,6 +66,14 @@ class BlaParams:
     isfd: int = 0
     imhd: int = 0
     loctyp: int = 0
+    # loctyp = 3: F_i = amp_i exp(-(y/yscale)^2) f(x) sin(tomeg t)
+    # (rparambl.f:908-935, locf.f:358-410)
+    lamp: tuple = (0.0, 0.0, 0.0)
+    lxscale: float = 1.0
+    lxloc0: float = 0.0
+    lyscale: float = 1.0
+    lxtype: float = 0.0
+    ltomeg: float = 0.0
     tripf: bool = False
     wbci: int = 0
     # wbci = 1: blowing and suction strip v(x, z, t) = amp f(x) cos(zbet z)
@@ -221,8 +229,14 @@ def parse(text: str) -> BlaParams:
     if p.imhd != 0:
         raise BlaInputError("MHD is not supported")
     p.loctyp = int(_num(tok()))
-    if p.loctyp != 0:
-        raise BlaInputError("localized forcing is not supported")
+    if p.loctyp == 3:
+        vals = [_num(tok()) for _ in range(8)]
+        p.lamp = tuple(vals[:3])
+        p.lxscale, p.lxloc0, p.lyscale, p.lxtype, p.ltomeg = vals[3:]
+        if p.lxtype not in (0.0, 1.0):
+            raise BlaInputError("loctyp 3 needs x type 0 or 1")
+    elif p.loctyp != 0:
+        raise BlaInputError("only the localized forcing loctyp = 3 is supported")
     p.tripf = _logical(tok())
     if p.tripf:
         raise BlaInputError("trip forcing is not supported")
@@ -331,7 +345,14 @@ def format_channel(p: BlaParams) -> str:
         f"{L(p.sgs)}     Read sgs.i (sgs)",
         "0               SFD (isfd)",
         "0               MHD (imhd)",
-        "0               Type of localized disturbance (loctyp)",
+        f"{p.loctyp}      Type of localized disturbance (loctyp)",
+    ]
+    if p.loctyp == 3:
+        out += [f"{p.lamp[0]!r}  Amplitude x", f"{p.lamp[1]!r}  Amplitude y",
+                f"{p.lamp[2]!r}  Amplitude z", f"{p.lxscale!r}  x scale",
+                f"{p.lxloc0!r}  x position", f"{p.lyscale!r}  y scale",
+                f"{p.lxtype!r}  x type", f"{p.ltomeg!r}  Frequency"]
+    out += [
         ".false.         Trip forcing (tripf)",
         f"{p.wbci}       Boundary condition at lower wall (wbci)",
     ]

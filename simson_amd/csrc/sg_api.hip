@@ -142,6 +142,14 @@ struct simson_gpu {
     double2 *bj = nullptr;
     bool bj_set = false;
     double du0upp = 0.;
+    // mean forcing (simson_gpu_mean_force): Chebyshev coefficients [2][ny]
+    double *mforce = nullptr;
+    bool mforce_on = false;
+    // localized volume force (simson_gpu_local_force): Chebyshev
+    // coefficients [3][nxh][ny] of the kz = 0 plane and its frequency
+    double2 *lforce = nullptr;
+    bool lforce_on = false;
+    double lforce_omega = 0.;
     int wall = 0;
     double wall_tomeg = 0., wall_p0 = 0., wall_m0 = 0., tnow = 0.;
     double2 *wallP = nullptr, *wallM = nullptr;
@@ -249,7 +257,7 @@ void free_all(simson_gpu_t *g)
                     g->twxs, g->xysd, g->SQ, g->SS, g->SW, g->dlt2, g->SP, g->flam, g->fbase,
                     g->flam32, g->fbase32, g->wallP, g->wallM, g->bj,
                     g->lam[0].hb, g->lam[0].hc, g->lam[1].hb, g->lam[1].hc,
-                    g->lam[2].hb, g->lam[2].hc};
+                    g->lam[2].hb, g->lam[2].hc, g->mforce, g->lforce};
     for (void *q : ptrs)
         if (q)
             (void)hipFree(q);
@@ -389,6 +397,9 @@ ColArgs col_args(simson_gpu_t *g)
     a.ibc = g->p.ibc;
     a.bj = g->bj_set ? g->bj : nullptr;
     a.du0upp = g->du0upp;
+    a.mforce = g->mforce_on ? g->mforce : nullptr;
+    a.lforce = nullptr;
+    a.lscale = 0.;
     if (g->p.ibc == 101) {
         // with a base flow the mean mode takes U = 1 above and no-slip
         // below before the free-stream condition replaces the upper value
@@ -713,6 +724,11 @@ int do_linear(simson_gpu_t *g, double an, double bn, double anp1, double bnp1)
         ca.vwall = sv >= 0. ? g->wallP : g->wallM;
         ca.vscale = sv;
         ca.v0 = sv * (sv >= 0. ? g->wall_p0 : g->wall_m0);
+    }
+    if (g->lforce_on) {
+        // the force at the start of the substep (locf.f:393, tc of bla.f:1797)
+        ca.lforce = g->lforce;
+        ca.lscale = std::sin(g->lforce_omega * g->tnow);
     }
     int st = 0;
     timed(g, F_TAU, [&] {
@@ -1332,6 +1348,111 @@ int simson_gpu_free_stream(simson_gpu_t *g, const double *bjr, const double *bji
         CK(hipStreamSynchronize(g->stream));
         g->bj_set = true;
     }
+    return 0;
+}
+
+int simson_parallel_force(const simson_baseflow_t *b, const double *tab, int ntab, double cdev,
+                          double t, double *force)
+{
+    return sg::parallel_force(b, tab, ntab, cdev, t, force);
+}
+
+int simson_gpu_mean_force(simson_gpu_t *g, const double *force)
+{
+    if (!g)
+        return SIMSON_E_ARG;
+    if (!force) {
+        g->mforce_on = false;
+        return 0;
+    }
+    const int ny = g->ny;
+    for (int i = 0; i < 3 * ny; i++)
+        if (!std::isfinite(force[i]))
+            return SIMSON_E_ARG;
+    if (!g->mforce && dalloc(g, &g->mforce, 2 * (size_t)ny))
+        return SIMSON_E_HIP;
+    // Chebyshev coefficients of f_u and f_w (the scaled forward transform
+    // of linearbl.f:262-273 applied to the forcing times nxp nzp)
+    std::vector<double> h(2 * (size_t)ny, 0.);
+    const double pi = 3.14159265358979323846;
+    for (int c = 0; c < 2; c++) {
+        const double *f = force + (c == 0 ? 0 : 2 * ny);
+        for (int k = 0; k < ny; k++) {
+            double s = 0.;
+            for (int j = 0; j < ny; j++) {
+                const double w = (j == 0 || j == ny - 1) ? .5 : 1.;
+                s += w * f[j] * std::cos(pi * (double)(((long)j * k) % (2 * (ny - 1))) /
+                                         (double)(ny - 1));
+            }
+            const double ck = (k == 0 || k == ny - 1) ? .5 : 1.;
+            h[(size_t)c * ny + k] = s * ck * 2. / (double)(ny - 1);
+        }
+    }
+    if (h2d(g, g->mforce, h.data(), h.size()))
+        return SIMSON_E_HIP;
+    CK(hipStreamSynchronize(g->stream));
+    g->mforce_on = true;
+    return 0;
+}
+
+int simson_local_force_pattern(int nx, int ny, double xl, const simson_locf_t *lf, double *pr,
+                               double *pi)
+{
+    if (!pr || !pi)
+        return SIMSON_E_ARG;
+    std::vector<double2> h(3 * (size_t)ny * (nx / 2));
+    const int st = sg::local_force_pattern(nx, ny, xl, lf, h.data());
+    if (st != 0)
+        return st;
+    for (size_t i = 0; i < h.size(); i++) {
+        pr[i] = h[i].x;
+        pi[i] = h[i].y;
+    }
+    return 0;
+}
+
+int simson_gpu_local_force(simson_gpu_t *g, const simson_locf_t *lf)
+{
+    if (!g)
+        return SIMSON_E_ARG;
+    if (!lf) {
+        g->lforce_on = false;
+        return 0;
+    }
+    const int ny = g->ny, nxh = g->nxh;
+    std::vector<double2> phys(3 * (size_t)ny * nxh);
+    const int st = sg::local_force_pattern(g->nx, ny, g->p.xl, lf, phys.data());
+    if (st != 0)
+        return st;
+    if (!g->lforce && dalloc(g, &g->lforce, 3 * (size_t)ny * nxh))
+        return SIMSON_E_HIP;
+    // Chebyshev coefficients in y of each (component, kx) column: the
+    // scaled forward transform of linearbl.f:262-273 applied to the force
+    // times nxp nzp
+    std::vector<double2> h(phys.size());
+    const double pi = 3.14159265358979323846;
+    std::vector<double> cs(2 * (size_t)(ny - 1));
+    for (size_t q = 0; q < cs.size(); q++)
+        cs[q] = std::cos(pi * (double)q / (double)(ny - 1));
+    for (int c = 0; c < 3; c++)
+        for (int x = 0; x < nxh; x++)
+            for (int k = 0; k < ny; k++) {
+                double sr = 0., si = 0.;
+                for (int y = 0; y < ny; y++) {
+                    const double w = (y == 0 || y == ny - 1) ? .5 : 1.;
+                    const double cv = cs[(size_t)(((long)y * k) % (2 * (ny - 1)))];
+                    const double2 v = phys[((size_t)c * ny + y) * nxh + x];
+                    sr += w * v.x * cv;
+                    si += w * v.y * cv;
+                }
+                const double ck = ((k == 0 || k == ny - 1) ? .5 : 1.) * 2. / (double)(ny - 1);
+                h[((size_t)c * nxh + x) * ny + k] = make_double2(sr * ck, si * ck);
+            }
+    if (h2d(g, g->lforce, h.data(), h.size()))
+        return SIMSON_E_HIP;
+    CK(hipStreamSynchronize(g->stream));
+    g->lforce_omega = lf->tomeg;
+    g->lforce_on = true;
     return 0;
 }
 

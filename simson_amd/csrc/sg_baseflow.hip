@@ -9,6 +9,10 @@
 //   its derivative with respect to s.  The table's own truncation error
 //   (~1e-14 for Blasius at n = 20001, ymax = 30) is part of the base flow
 //   the reference integrates, so the same discretisation is kept here.
+// * The temporal boundary layers of cbflow.f:58-84 (fltype 3, -1, -2): the
+//   similarity profile at the leading-edge distance x0 at every x, and the
+//   forcing of nonlinbl.f:771-795 that keeps such a parallel profile from
+//   diffusing.
 // * The spatial boundary layer of cbflow.f:161-300 (fltype 6, 7, 8): at
 //   every point of the 3/2 x grid the similarity profile at the local
 //   distance from the leading edge, blended with the profile one box
@@ -203,12 +207,22 @@ int fsc_table(double m, int n, double ymax, double eps, double *tab, double *dst
     return 0;
 }
 
+namespace {
+
+bool parallel_type(int fltype)
+{
+    return fltype == 3 || fltype == -1 || fltype == -2;
+}
+
+}  // namespace
+
 int spatial_base_flow(const simson_baseflow_t *b, const double *tab, int ntab, double *base,
                       double *bjr, double *bji, double *du0upp)
 {
     if (!b || !tab || !base || ntab < 4 || b->nx < 4 || b->nx % 4 != 0 || b->ny < 5 ||
         !(b->xl > 0.) || !(b->re > 0.) || !(b->dstar > 0.) || !(b->dstar2 > 0.) ||
-        (b->fltype != 6 && b->fltype != 7 && b->fltype != 8) || !(b->blength > 0.))
+        !(b->fltype == 6 || b->fltype == 7 || b->fltype == 8 || parallel_type(b->fltype)) ||
+        (!parallel_type(b->fltype) && !(b->blength > 0.)))
         return SIMSON_E_ARG;
     const int nxp = 3 * b->nx / 2, hp = nxp / 2, ny = b->ny, nxh = b->nx / 2;
     const double xl = b->xl, re = b->re, rlam = b->rlam;
@@ -216,16 +230,31 @@ int spatial_base_flow(const simson_baseflow_t *b, const double *tab, int ntab, d
     const double dyb = ymax / (double)(ntab - 1);
     // leading-edge distance of the box centre (rbla.f)
     const double x0 = b->dstar * b->dstar * re * (rlam + 1.) / (b->dstar2 * b->dstar2 * 2.);
-    // Cooke's g enters only the Falkner-Skan-Cooke flow (fltype 8); the
-    // other types have no spanwise base velocity (rbla.f zeroes g', g'' and
-    // rdiscbl.f sets spanv = 0 for them)
-    const double spanv = b->fltype == 8 ? b->spanv : 0.;
+    // Cooke's g enters only the Falkner-Skan-Cooke flows (fltype 8, -2);
+    // the other types have no spanwise base velocity (rbla.f zeroes g', g''
+    // and rdiscbl.f sets spanv = 0 for them)
+    const double spanv = (b->fltype == 8 || b->fltype == -2) ? b->spanv : 0.;
     std::vector<double> U((size_t)ny * nxp), V((size_t)ny * nxp), W((size_t)ny * nxp);
     std::vector<double> eta(ny);
     for (int y = 0; y < ny; y++)
         eta[y] = std::cos(kPi * (double)y / (double)(ny - 1));
+    if (parallel_type(b->fltype)) {
+        // temporal boundary layers (cbflow.f:58-84): the profile at x0 at
+        // every x, no wall-normal velocity
+        const double s = std::sqrt(re / (2. * x0));
+        for (int y = 0; y < ny; y++) {
+            const double e = (1. + eta[y]) * s;
+            const double u = interp4(e, tab + 2, 8, dyb, ntab) + b->u0low;
+            const double w = spanv * interp4(e, tab + 5, 8, dyb, ntab) + b->w0low;
+            for (int j = 0; j < nxp; j++) {
+                U[(size_t)y * nxp + j] = u;
+                V[(size_t)y * nxp + j] = 0.;
+                W[(size_t)y * nxp + j] = w;
+            }
+        }
+    }
     const double xmin = .5 * x0;
-    for (int j = 0; j < nxp; j++) {
+    for (int j = 0; j < nxp && !parallel_type(b->fltype); j++) {
         // x_j = -xl/2 + j xl/nxp + xsc, wrapped into [bstart, bstart + xl)
         const double xc = (double)(j - hp) / (double)nxp * xl + b->xsc;
         const double xb = xc - ((double)(int)((xc - b->bstart) / xl + 1.) - 1.) * xl;
@@ -273,7 +302,7 @@ int spatial_base_flow(const simson_baseflow_t *b, const double *tab, int ntab, d
         }
     std::vector<double> col(ny), ar(ny), ai(ny), br(ny), bi(ny), vr(ny), vi(ny);
     std::vector<double> Vr((size_t)ny * hp), Vi((size_t)ny * hp);
-    for (int k = 0; k < hp; k++) {
+    for (int k = 0; k < hp && !parallel_type(b->fltype); k++) {
         const double alfa = 2. * kPi / xl * (double)k;
         for (int y = 0; y < ny; y++)
             col[y] = hr[(size_t)y * hp + k];
@@ -308,7 +337,7 @@ int spatial_base_flow(const simson_baseflow_t *b, const double *tab, int ntab, d
             Vi[(size_t)y * hp + k] = vi[y];
         }
     }
-    for (int y = 0; y < ny; y++)
+    for (int y = 0; y < ny && !parallel_type(b->fltype); y++)
         for (int j = 0; j < nxp; j++) {
             double s = Vr[(size_t)y * hp];
             for (int k = 1; k < hp; k++) {
@@ -361,6 +390,42 @@ int spatial_base_flow(const simson_baseflow_t *b, const double *tab, int ntab, d
                 if (du0upp && c == 0 && k == 0)
                     *du0upp = sr / (double)nxp;
             }
+    }
+    return 0;
+}
+
+int parallel_force(const simson_baseflow_t *b, const double *tab, int ntab, double cdev,
+                   double t, double *force)
+{
+    if (!b || !tab || !force || ntab < 4 || b->ny < 5 || !(b->re > 0.) || !(b->dstar > 0.) ||
+        !(b->dstar2 > 0.) || !parallel_type(b->fltype))
+        return SIMSON_E_ARG;
+    const int ny = b->ny;
+    const double re = b->re, rlam = b->rlam;
+    const double ymax = tab[(size_t)(ntab - 1) * 8];
+    const double dyb = ymax / (double)(ntab - 1);
+    const double x0 = b->dstar * b->dstar * re * (rlam + 1.) / (b->dstar2 * b->dstar2 * 2.);
+    // nonlinbl.f:771-795 at the boundary layer's position xblc = x0 + cdev t
+    // (bla.f:1826), without the unnormalised transforms' factor nxp nzp
+    const double xblc = x0 + cdev * t;
+    if (!(xblc > 0.))
+        return SIMSON_E_ARG;
+    const double xx = std::pow(xblc / x0, rlam);
+    const double h1u = cdev * (rlam / x0) * std::pow(xblc / x0, rlam - 1.);
+    const double h3u = .5 * (rlam + 1.) / xblc;
+    for (int y = 0; y < ny; y++) {
+        const double ybl = 1. + std::cos(kPi * (double)y / (double)(ny - 1));
+        const double e = ybl * std::sqrt((rlam + 1.) * re * xx / (2. * xblc));
+        const double h2u = cdev * .5 * (rlam - 1.) * std::pow(xx, 1.5) *
+                           std::sqrt(.5 * (rlam + 1.) * re) * std::pow(1. / xblc, 1.5) * ybl;
+        const double d1f = interp4(e, tab + 2, 8, dyb, ntab);
+        const double d2f = interp4(e, tab + 3, 8, dyb, ntab);
+        const double d3f = interp4(e, tab + 4, 8, dyb, ntab);
+        force[y] = d1f * h1u + d2f * h2u - d3f * xx * xx * h3u;
+        force[ny + y] = 0.;
+        force[2 * ny + y] = 0.;
+        if (b->fltype == -2)
+            force[2 * ny + y] = -b->spanv * h3u * interp4(e, tab + 7, 8, dyb, ntab);
     }
     return 0;
 }

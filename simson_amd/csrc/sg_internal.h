@@ -105,6 +105,14 @@ struct ColArgs {
     int ibc;
     const double2 *bj;
     double du0upp;
+    // mean forcing of temporal boundary layers (nonlinbl.f:771-795) as the
+    // Chebyshev coefficients [2][ny] it adds to H1 and H3 of the kx = kz = 0
+    // column, or null
+    const double *mforce;
+    // localized volume force (loctyp 3): Chebyshev coefficients [3][nxh][ny]
+    // of the kz = 0 plane, added to H_i times lscale, or null
+    const double2 *lforce;
+    double lscale;
 };
 
 // Step-2 launch arguments.
@@ -297,6 +305,9 @@ void launch_factor_lam(const ColArgs &a, double apb, LamFactor lf, const double 
 // allow the generic step-2 kernels dynamic LDS up to lds_max bytes
 int nonlin_prepare(int lds_max);
 int fsc_table(double m, int n, double ymax, double eps, double *tab, double *dstar2);
+int parallel_force(const simson_baseflow_t *b, const double *tab, int ntab, double cdev,
+                   double t, double *force);
+int local_force_pattern(int nx, int ny, double xl, const simson_locf_t *lf, double2 *phys);
 int spatial_base_flow(const simson_baseflow_t *b, const double *tab, int ntab, double *base,
                       double *bjr, double *bji, double *du0upp);
 int  launch_tau(const ColArgs &a, LamFactor lf, LamFactor kf, double an, double bn,

@@ -444,12 +444,32 @@ __global__ void __launch_bounds__(256, (tau_waves_per_eu<SW, E>()))
     double2 *P0 = a.Pp;
     auto kidx = [&](int c, int e) { return 2 * (L * E + e) + c; };
     double2 tmp[2][E];
+    // localized volume force (locf.f, loctyp 3) on H_i of the kz = 0 plane:
+    // its Chebyshev coefficients times sin(tomeg tc)
+    const bool with_force = a.lforce && z == 0;
+    auto add_force = [&](int comp) {
+        if (!with_force)
+            return;
+        const double2 *P = a.lforce + ((size_t)comp * a.nxh + x) * n;
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int k = kidx(c, e);
+                if (k < n) {
+                    const double2 p = P[k];
+                    tmp[c][e].x += a.lscale * p.x;
+                    tmp[c][e].y += a.lscale * p.y;
+                }
+            }
+    };
 
     // h_omega and g = i alpha H1 + i beta H3 (linearbl.f:325-343)
     double hw[2][2][E], dg[2][2][E];
     {
         double g[2][2][E];
         io.load(C0, tmp);                           // H1
+        add_force(0);
 #pragma unroll
         for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -460,6 +480,7 @@ __global__ void __launch_bounds__(256, (tau_waves_per_eu<SW, E>()))
                 g[1][c][e] = alfa * tmp[c][e].x;
             }
         io.load(C0 + 2 * fs, tmp);                  // H3
+        add_force(2);
 #pragma unroll
         for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -476,6 +497,7 @@ __global__ void __launch_bounds__(256, (tau_waves_per_eu<SW, E>()))
     constexpr bool early = SW == 16;
     double f[4][2][E], hv[2][2][E];
     io.load(C0 + fs, tmp);                          // H2
+    add_force(1);
 #pragma unroll
     for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -734,6 +756,16 @@ __global__ void __launch_bounds__(256, (tau_waves_per_eu<SW, E>()))
             const bool ok = k < n;
             h1r[c][e] = ok ? widen(Cc[k]).x : 0.;
             h3r[c][e] = ok ? widen(Cc[2 * fs + k]).x : 0.;
+            if (a.mforce && ok) {
+                // the parallel-flow forcing of the mean mode (nonlinbl.f:
+                // 771-795), already in Chebyshev space
+                h1r[c][e] += a.mforce[k];
+                h3r[c][e] += a.mforce[n + k];
+            }
+            if (a.lforce && ok) {
+                h1r[c][e] += a.lscale * a.lforce[k].x;
+                h3r[c][e] += a.lscale * a.lforce[(size_t)2 * a.nxh * n + k].x;
+            }
             gl[c][e] = ok ? glp[k] : z2;
         }
     auto rhs = [&](double pxs) {

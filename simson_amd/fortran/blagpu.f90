@@ -39,9 +39,10 @@ program blagpu
   ! spatial simulations: fringe (ffun.f) and base flow [3][nyp][nxp]
   type(simson_fringe_t) :: fr
   type(simson_wall_t) :: wl
+  type(simson_locf_t) :: lfc
   real(dp) :: cnrk(3)
   real(dp) :: ampob, amp2d, bstart, blength, rlam, spanv, du0upp
-  real(dp), allocatable :: base(:,:,:), bjr(:,:), bji(:,:), tab(:,:)
+  real(dp), allocatable :: base(:,:,:), bjr(:,:), bji(:,:), tab(:,:), mforce(:,:)
   type(simson_baseflow_t) :: bf
   ! field and header
   integer :: nx, nyp, nzc, nfzsym, fltype
@@ -135,6 +136,19 @@ program blagpu
   call nxt(line); read(line, *) isfd
   call nxt(line); read(line, *) imhd
   call nxt(line); read(line, *) loctyp
+  if (loctyp == 3) then
+     ! rparambl.f:908-935: amplitudes x, y, z, x scale, x position, y scale,
+     ! x type, frequency of the force of locf.f:358-410
+     call nxt(line); read(line, *) lfc%ampx
+     call nxt(line); read(line, *) lfc%ampy
+     call nxt(line); read(line, *) lfc%ampz
+     call nxt(line); read(line, *) lfc%xscale
+     call nxt(line); read(line, *) lfc%xloc0
+     call nxt(line); read(line, *) lfc%yscale
+     call nxt(line); read(line, *) lfc%xtype
+     call nxt(line); read(line, *) lfc%tomeg
+     lfc%xsc = 0._dp
+  end if
   call nxt(line); read(line, *) tripf
   call nxt(line); read(line, *) wbci
   if (wbci == 1) then
@@ -193,7 +207,7 @@ program blagpu
   call nxt(line); read(line, *) npl
   close(10)
   if (pert .or. cim .or. suction .or. isfd /= 0 .or. imhd /= 0 .or. &
-       loctyp /= 0 .or. tripf .or. (wbci /= 0 .and. wbci /= 1) .or. &
+       (loctyp /= 0 .and. loctyp /= 3) .or. tripf .or. (wbci /= 0 .and. wbci /= 1) .or. &
        (ibc /= 0 .and. ibc /= 101) .or. nst /= 3 &
        .or. mwave > 0 .or. npl > 0) then
      write(*,*) 'blagpu: bla.i requests a feature outside the channel step library'
@@ -246,12 +260,13 @@ program blagpu
   read(12) re, pou, xlf, zl, t, xs
   read(12) nx, nyp, nzc, nfzsym
   read(12) fltype, dstar
-  ! temporal (fltype 1) or spatial (fltype 4) Poiseuille flow, or a spatial
-  ! Blasius, Falkner-Skan or Falkner-Skan-Cooke boundary layer (6, 7, 8)
-  if (nfzsym /= 0 .or. .not. (fltype == 1 .or. &
+  ! temporal (fltype 1) or spatial (fltype 4) Poiseuille flow, a spatial
+  ! Blasius, Falkner-Skan or Falkner-Skan-Cooke boundary layer (6, 7, 8), or
+  ! the temporal Blasius boundary layer (3)
+  if (nfzsym /= 0 .or. .not. (fltype == 1 .or. (.not. spat .and. fltype == 3) .or. &
        (spat .and. (fltype == 4 .or. (fltype >= 6 .and. fltype <= 8))))) call stopnow(13)
-  if (ibc == 101 .and. fltype < 6) call stopnow(13)
-  if (fltype >= 6 .and. (iamp > 0 .or. ixys > 0)) then
+  if (ibc == 101 .and. fltype < 6 .and. fltype /= 3) call stopnow(13)
+  if ((fltype >= 6 .or. fltype == 3) .and. (iamp > 0 .or. ixys > 0)) then
      write(*,*) 'blagpu: amplitude and x-y statistics files are written for channels only'
      call stopnow(13)
   end if
@@ -281,6 +296,13 @@ program blagpu
   wl%xfall = wl%xfall * dstar
   wl%zbet = wl%zbet / dstar
   wl%tomeg = wl%tomeg / dstar
+  lfc%ampx = lfc%ampx / dstar
+  lfc%ampy = lfc%ampy / dstar
+  lfc%ampz = lfc%ampz / dstar
+  lfc%tomeg = lfc%tomeg / dstar
+  lfc%xscale = lfc%xscale * dstar
+  lfc%xloc0 = lfc%xloc0 * dstar
+  lfc%yscale = lfc%yscale * dstar
   allocate(ur(nx/2, nyp, nzc, 7), ui(nx/2, nyp, nzc, 7), urx(nx))
   ur = 0._dp
   ui = 0._dp
@@ -345,7 +367,22 @@ program blagpu
      end if
      call chk(simson_gpu_fringe(g, fr, base))
   end if
+  if (fltype == 3) then
+     ! temporal boundary layer: the similarity table fsc.dat (rbla.f), the
+     ! parallel profile's free-stream data (bflow.f) and the parallel-flow
+     ! forcing of nonlinbl.f:771-795 (updated per substep when cdev /= 0)
+     call read_fsc()
+     bf%nx = nx; bf%ny = nyp; bf%fltype = fltype
+     bf%xl = xl; bf%re = re; bf%dstar = dstar; bf%bstart = 0._dp; bf%blength = 0._dp
+     bf%rlam = rlam; bf%spanv = spanv; bf%xsc = 0._dp; bf%u0low = 0._dp; bf%w0low = 0._dp
+     allocate(base(3 * nx / 2, nyp, 3), bjr(nx / 2, 3), bji(nx / 2, 3), mforce(nyp, 3))
+     call chk(simson_base_flow(bf, tab, int(size(tab, 2), c_int), base, bjr, bji, du0upp))
+     if (ibc == 101) call chk(simson_gpu_free_stream(g, bjr, bji, du0upp))
+     call chk(simson_parallel_force(bf, tab, int(size(tab, 2), c_int), cdev, t, mforce))
+     call chk(simson_gpu_mean_force(g, mforce))
+  end if
   if (wbci == 1) call chk(simson_gpu_wall(g, wl))
+  if (loctyp == 3) call chk(simson_gpu_local_force(g, lfc))
   call chk(simson_gpu_upload(g, ur, ui, 3_c_int, c_null_ptr, px))
   ! Clenshaw-Curtis weights (preprbl.f:78-87) for the amplitude file
   allocate(amp(nyp, 20), wint(nyp))
@@ -444,6 +481,12 @@ program blagpu
      bnp1 = dtnp1 * bnrk(mod(it, nst) + 1)
      ! time of the start of this substep for the wall forcing (bla.f:1797)
      call chk(simson_gpu_set_time(g, t + dtn * cnrk(mod(it - 1, nst) + 1)))
+     if (fltype == 3 .and. cdev /= 0._dp) then
+        ! the boundary layer's position x0 + cdev tc of this substep (bla.f:1826)
+        call chk(simson_parallel_force(bf, tab, int(size(tab, 2), c_int), cdev, &
+             t + dtn * cnrk(mod(it - 1, nst) + 1), mforce))
+        call chk(simson_gpu_mean_force(g, mforce))
+     end if
      call chk(simson_gpu_linear(g, an, bn, anp1, bnp1, c_null_ptr))
      it = it + 1
      if (mod(it - 1, nst) == 0) then

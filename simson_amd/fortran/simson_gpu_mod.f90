@@ -38,6 +38,11 @@ module simson_gpu_mod
      real(c_double) :: xl, re, dstar, dstar2, bstart, blength, rlam, spanv, xsc, u0low, w0low
   end type simson_baseflow_t
 
+  ! localized volume force of bla.i loctyp = 3 (locf.f)
+  type, bind(c) :: simson_locf_t
+     real(c_double) :: ampx, ampy, ampz, xscale, xloc0, yscale, xtype, tomeg, xsc
+  end type simson_locf_t
+
   type, bind(c) :: simson_les_t
      integer(c_int) :: iles
      integer(c_int) :: iord
@@ -148,6 +153,28 @@ module simson_gpu_mod
        real(c_double), intent(out) :: base(*), bjr(*), bji(*)
        real(c_double), intent(out) :: du0upp
      end function simson_base_flow
+
+     integer(c_int) function simson_parallel_force(b, tab, ntab, cdev, t, force) &
+          bind(c, name='simson_parallel_force')
+       import :: c_int, c_double, simson_baseflow_t
+       type(simson_baseflow_t), intent(in) :: b
+       real(c_double), intent(in) :: tab(*)
+       integer(c_int), value :: ntab
+       real(c_double), value :: cdev, t
+       real(c_double), intent(out) :: force(*)
+     end function simson_parallel_force
+
+     integer(c_int) function simson_gpu_mean_force(g, force) bind(c, name='simson_gpu_mean_force')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: g
+       real(c_double), intent(in) :: force(*)
+     end function simson_gpu_mean_force
+
+     integer(c_int) function simson_gpu_local_force(g, lf) bind(c, name='simson_gpu_local_force')
+       import :: c_int, c_ptr, simson_locf_t
+       type(c_ptr), value :: g
+       type(simson_locf_t), intent(in) :: lf
+     end function simson_gpu_local_force
 
      integer(c_int) function simson_gpu_wall(g, w) bind(c, name='simson_gpu_wall')
        import :: c_int, c_ptr, simson_wall_t

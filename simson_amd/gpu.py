@@ -32,6 +32,8 @@ EXPORTS = (
     "simson_gpu_les", "simson_les_filter", "simson_gpu_closure", "simson_gpu_sync",
     "simson_gpu_fringe", "simson_fringe_function", "simson_gpu_wall", "simson_gpu_set_time",
     "simson_gpu_free_stream", "simson_fsc_table", "simson_base_flow",
+    "simson_parallel_force", "simson_gpu_mean_force", "simson_gpu_local_force",
+    "simson_local_force_pattern",
     "simson_wall_velocity",
     "simson_mlp_create", "simson_mlp_nparams", "simson_mlp_params", "simson_mlp_set_data",
     "simson_mlp_train", "simson_mlp_grad", "simson_mlp_infer", "simson_mlp_last_error",
@@ -69,6 +71,11 @@ class SimsonBaseFlow(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("fltype", C.c_int)] + [
         (n, C.c_double) for n in ("xl", "re", "dstar", "dstar2", "bstart", "blength", "rlam",
                                   "spanv", "xsc", "u0low", "w0low")]
+
+
+class SimsonLocf(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("ampx", "ampy", "ampz", "xscale", "xloc0", "yscale",
+                                         "xtype", "tomeg", "xsc")]
 
 
 class SimsonError(RuntimeError):
@@ -119,6 +126,12 @@ def lib():
                                        C.POINTER(C.c_double)]
         L.simson_base_flow.argtypes = [C.POINTER(SimsonBaseFlow), _dp, C.c_int, _dp,
                                        C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]
+        L.simson_parallel_force.argtypes = [C.POINTER(SimsonBaseFlow), _dp, C.c_int, C.c_double,
+                                            C.c_double, _dp]
+        L.simson_gpu_mean_force.argtypes = [vp, C.c_void_p]
+        L.simson_gpu_local_force.argtypes = [vp, C.POINTER(SimsonLocf)]
+        L.simson_local_force_pattern.argtypes = [C.c_int, C.c_int, C.c_double,
+                                                 C.POINTER(SimsonLocf), _dp, _dp]
         L.simson_gpu_set_time.argtypes = [vp, C.c_double]
         L.simson_wall_velocity.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double,
                                            C.POINTER(SimsonWall), C.c_double, _dp, _dp]
@@ -294,6 +307,25 @@ class BlaGPU:
         self._check(lib().simson_gpu_free_stream(self.h, br.ctypes.data_as(C.c_void_p),
                                                  bi.ctypes.data_as(C.c_void_p), du0upp))
 
+    def mean_force(self, force=None):
+        """A mean force [3][ny] added to H's kx = kz = 0 mode every substep
+        (temporal boundary layers); None switches it off."""
+        if force is None:
+            self._check(lib().simson_gpu_mean_force(self.h, None))
+            return
+        f = np.ascontiguousarray(force, np.float64)
+        self._check(lib().simson_gpu_mean_force(self.h, f.ctypes.data_as(C.c_void_p)))
+
+    def local_force(self, ampx=None, ampy=0.0, ampz=0.0, xscale=1.0, xloc0=0.0, yscale=1.0,
+                    xtype=0.0, tomeg=0.0, xsc=0.0):
+        """The localized volume force of bla.i loctyp = 3 (units of the .u
+        file); ampx None switches it off."""
+        if ampx is None:
+            self._check(lib().simson_gpu_local_force(self.h, None))
+            return
+        p = SimsonLocf(ampx, ampy, ampz, xscale, xloc0, yscale, xtype, tomeg, xsc)
+        self._check(lib().simson_gpu_local_force(self.h, C.byref(p)))
+
     def set_time(self, t: float):
         """Time of the start of the next substep (time-dependent forcing)."""
         self._check(lib().simson_gpu_set_time(self.h, t))
@@ -391,3 +423,16 @@ def rk3_weights(dt: float):
     a = (8. / 15., 5. / 12., 3. / 4.)
     b = (0., -17. / 60., -5. / 12.)
     return [dt * x for x in a], [dt * x for x in b]
+
+
+def local_force_pattern(nx: int, ny: int, xl: float, ampx: float, ampy: float, ampz: float,
+                        xscale: float, xloc0: float, yscale: float, xtype: float,
+                        tomeg: float = 0.0, xsc: float = 0.0) -> np.ndarray:
+    """The library's loctyp = 3 force pattern, complex [3][ny][nx/2] (host)."""
+    pr = np.zeros((3, ny, nx // 2))
+    pi = np.zeros_like(pr)
+    p = SimsonLocf(ampx, ampy, ampz, xscale, xloc0, yscale, xtype, tomeg, xsc)
+    st = lib().simson_local_force_pattern(nx, ny, xl, C.byref(p), pr, pi)
+    if st != 0:
+        raise SimsonError(st, "invalid localized-force parameters")
+    return pr + 1j * pi

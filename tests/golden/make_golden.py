@@ -12,6 +12,8 @@ Runs in the build container only (it needs /root/reference and amdflang):
     python tests/golden/make_golden.py wall       # wall_32x33x32.npz
     python tests/golden/make_golden.py vart       # vart_32x33x32.npz
     python tests/golden/make_golden.py blasius    # blasius_64x33x8.npz
+    python tests/golden/make_golden.py tblasius   # tblasius_32x33x16.npz
+    python tests/golden/make_golden.py locf       # locf_64x33x8.npz
 
 1. builds the reference `bla` for 32x33x32 (serial, pressure off) with
    oracle/build_ref.sh into oracle/_ref/;
@@ -524,6 +526,114 @@ def blasius():
     print("wrote", os.path.join(HERE, "blasius_64x33x8.npz"), sorted(res))
 
 
+# loctyp 3 volume force (locf.f) on the spatial Blasius case of BL, in
+# displacement thicknesses: amplitudes (x, y, z), x scale, x position,
+# y scale, x type, frequency
+LOCF = ((0.0, 0.01, 0.005), 8.0, -15.0, 2.0, 0.0, 0.3)
+
+
+def locf():
+    """Localized volume force fixture: the spatial Blasius case of blasius()
+    (case a: fringe, ibc 101, no wall strip) with bla.i loctyp = 3
+    (locf.f:358-410): amplitudes (0, 0.01, 0.005), x scale 8, at x = -15,
+    y scale 2, x type 0, frequency 0.3, all in delta* units; 20 steps,
+    saves after 1 and 20.  Keys u1, u20 (the start field is blasius_64x33x8's
+    init)."""
+    from simson_amd import baseflow
+    c = BL
+    if not os.path.exists(BLBIN):
+        subprocess.check_call([os.path.join(REPO, "oracle", "build_ref.sh"),
+                               str(c["nx"]), str(c["ny"]), str(c["nz"]), "1", "no", "0",
+                               os.path.basename(BLBIN)])
+    res = {}
+    work = tempfile.mkdtemp(prefix="simson_golden_locf.")
+    try:
+        table = baseflow.read_fsc(fsc_dat(work))
+        f = blasius_field(table)
+        f.u = np.load(os.path.join(HERE, "blasius_64x33x8.npz"))["init"]
+        ufile.write_u(os.path.join(work, "init.u"), f)
+        fmax, fstart, fend, frise, ffall = c["fringe"]
+        amp, xs, x0, ys, xt, om = LOCF
+        kw = dict(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=60, dt=c["dt"], nst=3,
+                  xl=c["xl"], cflux=False, retau=0.0, ibc=101, icfl=3, spat=True, fmax=fmax,
+                  fstart=fstart, fend=fend, frise=frise, ffall=ffall, loctyp=3, lamp=amp,
+                  lxscale=xs, lxloc0=x0, lyscale=ys, lxtype=xt, ltomeg=om,
+                  tsave=[c["dt"]], nmsave=["t1.u"])
+        with open(os.path.join(work, "bla.i"), "w") as fh:
+            fh.write(blai.format_channel(blai.BlaParams(**kw)))
+        with open(os.path.join(work, "out.txt"), "w") as out:
+            subprocess.check_call([BLBIN], cwd=work, stdout=out, stderr=subprocess.STDOUT)
+        log = open(os.path.join(work, "out.txt")).read()
+        if "bla terminated normally" not in log:
+            raise RuntimeError("reference bla (locf) did not terminate normally")
+        res["u1"] = ufile.read_u(os.path.join(work, "t1.u")).u
+        res["u20"] = ufile.read_u(os.path.join(work, "end.u")).u
+    finally:
+        shutil.rmtree(work)
+    np.savez_compressed(os.path.join(HERE, "locf_64x33x8.npz"), **res)
+    print("wrote", os.path.join(HERE, "locf_64x33x8.npz"), sorted(res))
+
+
+TBL = dict(nx=32, ny=33, nz=16, re=400.0, xl=60.0, yl=20.0, zl=30.0, dt=0.1, kmax=4,
+           rms=1e-2, seed=8)
+TBLBIN = os.path.join(REPO, "oracle", "_ref", "bla_32x33x16_serial")
+
+
+def tblasius_field(table):
+    from simson_amd import baseflow
+    c = TBL
+    ds = 2.0 / c["yl"]
+    base, _, _, _ = baseflow.temporal_base_flow(c["nx"], c["ny"], c["xl"] * ds, c["re"] / ds,
+                                                ds, table)
+    f = fields.boundary_layer_field(c["nx"], c["ny"], c["nz"], c["xl"] * ds, c["zl"] * ds,
+                                    c["re"] / ds, ds, base, 0.0, 0.0, kmax=c["kmax"],
+                                    rms=c["rms"], seed=c["seed"], fltype=3)
+    f.extra = {}
+    return f
+
+
+def tblasius():
+    """Temporal Blasius boundary-layer fixture (fltype 3, not spat, cdev 0,
+    ibc 101, no pressure gradient) at 32x33x16, Re_delta* 400, box 60 x 20
+    x 30 delta*: fsc.dat of the reference similarity solver, the parallel
+    Blasius profile plus a seeded one-wall perturbation (rms 1e-2, |kx|,
+    |kz| <= 4); 20 steps at dt = 0.1 delta*/U with the parallel-flow forcing
+    of nonlinbl.f:771-795, saves after 1 and 20 steps.  Keys: init, u1,
+    u20, and the log's du0upp (times dstar) and x0 (over dstar)."""
+    from simson_amd import baseflow
+    c = TBL
+    if not os.path.exists(TBLBIN):
+        subprocess.check_call([os.path.join(REPO, "oracle", "build_ref.sh"),
+                               str(c["nx"]), str(c["ny"]), str(c["nz"]), "1", "no", "0",
+                               os.path.basename(TBLBIN)])
+    res = {}
+    work = tempfile.mkdtemp(prefix="simson_golden_tbl.")
+    try:
+        table = baseflow.read_fsc(fsc_dat(work))
+        f = tblasius_field(table)
+        ufile.write_u(os.path.join(work, "init.u"), f)
+        res["init"] = ufile.read_u(os.path.join(work, "init.u")).u
+        kw = dict(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=60, dt=c["dt"], nst=3,
+                  xl=c["xl"], cflux=False, retau=0.0, ibc=101, icfl=3, spat=False, cdev=0.0,
+                  tsave=[c["dt"]], nmsave=["t1.u"])
+        with open(os.path.join(work, "bla.i"), "w") as fh:
+            fh.write(blai.format_channel(blai.BlaParams(**kw)))
+        with open(os.path.join(work, "out.txt"), "w") as out:
+            subprocess.check_call([TBLBIN], cwd=work, stdout=out, stderr=subprocess.STDOUT)
+        log = open(os.path.join(work, "out.txt")).read()
+        if "bla terminated normally" not in log:
+            raise RuntimeError("reference bla (temporal Blasius) did not terminate normally")
+        res["u1"] = ufile.read_u(os.path.join(work, "t1.u")).u
+        res["u20"] = ufile.read_u(os.path.join(work, "end.u")).u
+        res["du0upp_log"] = np.array(float(re.search(r"du0upp\s*:\s*([0-9.Ee+-]+)",
+                                                     log).group(1)))
+        res["x0_log"] = np.array(float(re.search(r"x0\s*:\s*([0-9.Ee+-]+)", log).group(1)))
+    finally:
+        shutil.rmtree(work)
+    np.savez_compressed(os.path.join(HERE, "tblasius_32x33x16.npz"), **res)
+    print("wrote", os.path.join(HERE, "tblasius_32x33x16.npz"), sorted(res))
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "pressure":
         pressure()
@@ -541,6 +651,10 @@ if __name__ == "__main__":
         long()
     elif len(sys.argv) > 1 and sys.argv[1] == "blasius":
         blasius()
+    elif len(sys.argv) > 1 and sys.argv[1] == "tblasius":
+        tblasius()
+    elif len(sys.argv) > 1 and sys.argv[1] == "locf":
+        locf()
     elif len(sys.argv) > 1 and sys.argv[1] == "vart":
         vart()
     else:

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared():
     text = open(os.path.join(ROOT, "include", "simson_gpu.h")).read()
-    return sorted(set(re.findall(r"\b(simson_(?:gpu|les|mlp|fringe|wall|fsc|base)_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(simson_(?:gpu|les|mlp|fringe|wall|fsc|base|parallel|local)_\w+)\s*\(", text)))
 
 
 def test_header_and_python_agree():

@@ -1,6 +1,8 @@
-"""Spatial Blasius boundary layer (bla.i spat, fltype 6, ibc 101): the
-similarity table, the base flow with its free-stream data, and the step with
-the free-stream boundary rows.
+"""Blasius boundary layers with a free stream above (ibc 101): the spatial
+layer (bla.i spat, fltype 6) and the temporal one (fltype 3, with the
+parallel-flow forcing of nonlinbl.f:771-795).  The similarity table, the
+base flows with their free-stream data, and the step with the free-stream
+boundary rows.
 
 Pins:
 
@@ -26,6 +28,12 @@ Pins:
   flow likewise; and the boundary conditions of the result: du_i/dy at
   y = +1 equal to the base flow's (kz = 0) or zero, dv/dy = V' there, and
   no-slip below.
+
+The temporal layer (tests/golden/tblasius_32x33x16.npz: 32x33x16,
+Re_delta* 400, box 60 x 20 x 30 delta*, 20 steps) pins the parallel profile,
+its free-stream data and the forcing the same way: library against the
+restatement, the forcing against f f''/(2 x0), the oracle and the GPU
+against the reference run.
 
 Tolerance: velocities 1e-12 relative L2 outside the z-oddball plane (FP64;
 2e-5 mixed, whose FP32 transforms round at 6e-8).
@@ -223,5 +231,100 @@ def test_gpu_matches_reference(golden, case, form):
         e1 = rel(g.velocities(), golden[f"{case}_u1"])
         g.run(19, DT, icfl=3, first=False)
         e20 = rel(g.velocities(), golden[f"{case}_u20"])
+    print(f"    after 1 step {e1}, 20 steps {e20}", flush=True)
+    assert max(e1) <= tol and max(e20) <= tol, (e1, e20)
+
+
+# temporal Blasius boundary layer (fltype 3): tests/golden/tblasius_32x33x16.npz
+TNX, TNY, TNZ = 32, 33, 16
+TRE, TXL, TZL = 400.0 / DS, 60.0 * DS, 30.0 * DS
+
+
+@pytest.fixture(scope="module")
+def golden_t():
+    return np.load(os.path.join(ROOT, "tests", "golden", "tblasius_32x33x16.npz"))
+
+
+def test_temporal_base_flow_matches_restatement(golden_t, table):
+    """The parallel profile, its free-stream data and the parallel-flow
+    forcing of nonlinbl.f:771-795: library against the numpy restatement,
+    du0upp against the value the reference printed; the forcing is
+    f f''/(2 x0) (the Blasius equation f''' = -f f'') and vanishes at the
+    wall and in the free stream."""
+    base, bj, du, force = baseflow.temporal_base_flow(TNX, TNY, TXL, TRE, DS, table)
+    rb, rbj, rdu, rf = baseflow_ref.temporal_base_flow(TNX, TNY, TRE, DS, table.tab,
+                                                       table.dstar2)
+    assert np.abs(base - rb).max() < 1e-15
+    assert np.abs(bj - rbj).max() < 1e-12
+    assert abs(du * DS - float(golden_t["du0upp_log"])) < 1e-12 * abs(du * DS)
+    assert np.abs(force - rf).max() < 1e-15 * np.abs(rf).max() + 1e-30
+    x0 = DS ** 2 * TRE / (2 * table.dstar2 ** 2)
+    assert abs(x0 / DS - float(golden_t["x0_log"])) < 1e-10
+    eta = np.cos(np.pi * np.arange(TNY) / (TNY - 1))
+    e = (1 + eta) * np.sqrt(TRE / (2 * x0))
+    ff2 = np.interp(e, table.tab[:, 0], table.tab[:, 1] * table.tab[:, 3])
+    assert np.abs(force[0] - ff2 / (2 * x0)).max() < 1e-6 * np.abs(force[0]).max()
+    assert abs(force[0, -1]) < 1e-20 and abs(force[0, 0]) < 1e-20
+    assert np.all(force[1:] == 0.0) and np.all(base[1:] == 0.0)
+
+
+def run_oracle_temporal(golden_t, force=True):
+    from pyoracle import Oracle
+    tab = baseflow.fsc_table()
+    _, rbj, rdu, rf = baseflow_ref.temporal_base_flow(TNX, TNY, TRE, DS, tab.tab, tab.dstar2)
+    o = Oracle(TNX, TNY, TNZ, TRE, TXL, TZL, cflux=False, retau=0.0)
+    out = []
+    try:
+        ur, ui, puw = o.new_state(golden_t["init"])
+        o.px = 0.0
+        o.set_bc(101, rbj, rdu)
+        if force:
+            o.set_mean_force(rf)
+        for n in (1, 19):
+            st, _ = o.run(ur, ui, puw, n, DT)
+            assert st == 0
+            out.append(ur[:3] + 1j * ui[:3])
+    finally:
+        o.close()
+    return out
+
+
+def trel(a, b):
+    a = np.delete(a, TNZ // 2, axis=1)
+    b = np.delete(b, TNZ // 2, axis=1)
+    return [float(np.linalg.norm(a[c] - b[c]) / np.linalg.norm(b[c])) for c in range(3)]
+
+
+def test_oracle_temporal_matches_reference(golden_t):
+    """The oracle with ibc 101 and the restated forcing against the
+    reference's temporal Blasius run; without the forcing it is 1e-5 off
+    after one step, so the forcing is pinned."""
+    u1, u20 = run_oracle_temporal(golden_t)
+    e1, e20 = trel(u1, golden_t["u1"]), trel(u20, golden_t["u20"])
+    print(f"oracle temporal blasius: 1 step {e1}, 20 steps {e20}")
+    assert max(e1) <= 1e-12 and max(e20) <= 1e-12, (e1, e20)
+    n1, _ = run_oracle_temporal(golden_t, force=False)
+    assert max(trel(n1, golden_t["u1"])) > 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_gpu_temporal_matches_reference(golden_t, form):
+    from simson_amd.gpu import SIMSON_FP64, SIMSON_MIXED, BlaGPU
+    print(f"[gpu] temporal Blasius ({form}) {TNX}x{TNY}x{TNZ}, ibc 101, parallel-flow forcing",
+          flush=True)
+    tab = baseflow.fsc_table()
+    _, bj, du0upp, force = baseflow.temporal_base_flow(TNX, TNY, TXL, TRE, DS, tab)
+    prec = SIMSON_MIXED if form == "mixed" else SIMSON_FP64
+    tol = 1e-12 if form == "fp64" else 2e-5
+    with BlaGPU(TNX, TNY, TNZ, TRE, TXL, TZL, cflux=False, retau=0.0, precision=prec,
+                ibc=101) as g:
+        g.free_stream(bj, du0upp)
+        g.mean_force(force)
+        g.upload_velocity(golden_t["init"], 0.0)
+        g.run(1, DT, icfl=3)
+        e1 = trel(g.velocities(), golden_t["u1"])
+        g.run(19, DT, icfl=3, first=False)
+        e20 = trel(g.velocities(), golden_t["u20"])
     print(f"    after 1 step {e1}, 20 steps {e20}", flush=True)
     assert max(e1) <= tol and max(e20) <= tol, (e1, e20)

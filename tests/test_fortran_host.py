@@ -437,3 +437,77 @@ def test_fortran_host_blasius_matches_reference_bla():
                 assert err <= 1e-12, (name, c, err)
     finally:
         shutil.rmtree(work)
+
+
+@pytest.mark.gpu
+def test_fortran_host_temporal_blasius_matches_reference_bla():
+    """The host on the temporal Blasius boundary layer (fltype 3, ibc 101,
+    the parallel-flow forcing; tests/golden/tblasius_32x33x16.npz) writes
+    the fields the reference bla wrote."""
+    from simson_amd import baseflow
+    _need_binary()
+    print("[gpu] blagpu temporal Blasius boundary layer 32x33x16", flush=True)
+    golden = np.load(os.path.join(ROOT, "tests", "golden", "tblasius_32x33x16.npz"))
+    ds = 2.0 / 20.0
+    work = tempfile.mkdtemp(prefix="blagpu_tbl.")
+    try:
+        baseflow.write_fsc(os.path.join(work, "fsc.dat"), baseflow.fsc_table())
+        f = ufile.UField(u=golden["init"], re=400.0 / ds, xl=60.0 * ds, zl=30.0 * ds, t=0.0,
+                         xs=0.0, fltype=3, dstar=ds)
+        ufile.write_u(os.path.join(work, "init.u"), f)
+        p = blai.BlaParams(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=60, dt=0.1,
+                           nst=3, xl=60.0, cflux=False, retau=0.0, ibc=101, icfl=3,
+                           spat=False, cdev=0.0, tsave=[0.1], nmsave=["t1.u"])
+        with open(os.path.join(work, "bla.i"), "w") as fh:
+            fh.write(blai.format_channel(p))
+        r = _run(work)
+        assert r.returncode == 0, r.stdout + r.stderr
+        for name, key in (("t1.u", "u1"), ("end.u", "u20")):
+            uf = ufile.read_u(os.path.join(work, name))
+            assert uf.fltype == 3
+            u = np.delete(uf.u, 8, axis=1)
+            rr = np.delete(golden[key], 8, axis=1)
+            for c in range(3):
+                err = _rel(u[c], rr[c])
+                print(f"    {name} comp {c}: rel L2 {err:.2e}", flush=True)
+                assert err <= 1e-12, (name, c, err)
+    finally:
+        shutil.rmtree(work)
+
+
+@pytest.mark.gpu
+def test_fortran_host_local_force_matches_reference_bla():
+    """The host with bla.i loctyp = 3 on the spatial Blasius boundary layer
+    (tests/golden/locf_64x33x8.npz) writes the fields the reference wrote."""
+    from simson_amd import baseflow
+    _need_binary()
+    print("[gpu] blagpu loctyp 3 volume force 64x33x8", flush=True)
+    gb = np.load(os.path.join(ROOT, "tests", "golden", "blasius_64x33x8.npz"))
+    golden = np.load(os.path.join(ROOT, "tests", "golden", "locf_64x33x8.npz"))
+    ds = 2.0 / 20.0
+    work = tempfile.mkdtemp(prefix="blagpu_locf.")
+    try:
+        baseflow.write_fsc(os.path.join(work, "fsc.dat"), baseflow.fsc_table())
+        f = ufile.UField(u=gb["init"], re=400.0 / ds, xl=100.0 * ds, zl=20.0 * ds, t=0.0,
+                         xs=0.0, fltype=6, dstar=ds,
+                         extra={"bstart": -35.0 * ds, "blength": 25.0 * ds})
+        ufile.write_u(os.path.join(work, "init.u"), f)
+        p = blai.BlaParams(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=60, dt=0.1,
+                           nst=3, xl=100.0, cflux=False, retau=0.0, ibc=101, icfl=3,
+                           spat=True, fmax=1.0, fstart=-45.0, fend=0.0, frise=15.0,
+                           ffall=10.0, loctyp=3, lamp=(0.0, 0.01, 0.005), lxscale=8.0,
+                           lxloc0=-15.0, lyscale=2.0, lxtype=0.0, ltomeg=0.3,
+                           tsave=[0.1], nmsave=["t1.u"])
+        with open(os.path.join(work, "bla.i"), "w") as fh:
+            fh.write(blai.format_channel(p))
+        r = _run(work)
+        assert r.returncode == 0, r.stdout + r.stderr
+        for name, key in (("t1.u", "u1"), ("end.u", "u20")):
+            u = np.delete(ufile.read_u(os.path.join(work, name)).u, 4, axis=1)
+            rr = np.delete(golden[key], 4, axis=1)
+            for c in range(3):
+                err = _rel(u[c], rr[c])
+                print(f"    {name} comp {c}: rel L2 {err:.2e}", flush=True)
+                assert err <= 1e-12, (name, c, err)
+    finally:
+        shutil.rmtree(work)
