@@ -174,16 +174,20 @@ int simson_gpu_pressure(simson_gpu_t *g, double *pr, double *pi);
  *   divergence sgs_stress.f:62-131 subtracted from H (nonlinbl.f:1011-1050).
  *   The reference compiles it in only with iiles_smag = 1 (bla.f:28).
  *   Computed in FP64 in both precision forms.
- * iles 0 turns the model off; the HPF eddy-viscosity model (iles 3) is
- * refused (-1).  Call after init, before the first step; needs nzp among
- * the specialised z transform lengths. */
+ * - iles 3, the Smagorinsky model on the high-pass filtered velocity
+ *   (les.f:180-215, 404-620, 883-1380): as iles 2 with the strain of the
+ *   velocities and vorticities filtered as for ADM-RT (cutoff, iord) and,
+ *   with cs = 0, B_ij of the dynamic procedure from the secondary filter
+ *   of linearbl.f:2067-2256.  FP64 in both precision forms.
+ * iles 0 turns the model off.  Call after init, before the first step;
+ * needs nzp among the specialised z transform lengths. */
 typedef struct simson_les {
-    int iles;            /* 0: no model; 1: ADM-RT; 2: Smagorinsky */
-    int iord;            /* sgs.i iord (iles 1) */
-    double cutoff;       /* 1 / sgs.i cutoff_inv (iles 1) */
+    int iles;            /* 0: no model; 1: ADM-RT; 2: Smagorinsky; 3: HPF Smagorinsky */
+    int iord;            /* sgs.i iord (iles 1, 3) */
+    double cutoff;       /* 1 / sgs.i cutoff_inv (iles 1, 3) */
     double chi;          /* sgs.i chi divided by dstar (iles 1) */
-    double cs;           /* sgs.i cs (iles 2; 0: dynamic procedure) */
-    int ineg;            /* sgs.i ineg (iles 2) */
+    double cs;           /* sgs.i cs (iles 2, 3; 0: dynamic procedure) */
+    int ineg;            /* sgs.i ineg (iles 2, 3) */
 } simson_les_t;
 
 int simson_gpu_les(simson_gpu_t *g, const simson_les_t *p);
@@ -355,6 +359,12 @@ int simson_fringe_function(int nx, double xl, const simson_fringe_t *f, double *
  * wall-normal low-pass I - H_y^(iord+1), H_y the compact high-pass filter of
  * hpf_y.f on the Gauss-Lobatto points.  Returns 0 or SIMSON_E_ARG. */
 int simson_les_filter(int nx, int ny, int nz, double cutoff, int iord, double *lpf, double *L);
+
+/* The secondary x-z transfer function lpfxz2 [nz][nx/2] of the dynamic HPF
+ * model (bla.f:1520-1543: the homogeneous filters of init_filter.f sampled
+ * on nx/4 + 1 and nz/4 + 1 points, zero above), without a device.  Returns
+ * 0 or SIMSON_E_ARG. */
+int simson_les_filter_secondary(int nx, int nz, double cutoff, int iord, double *lpf2);
 
 /* ---- Learned subgrid closure (no counterpart in the reference: it is the
  * learned replacement of the tau_ij model of les.f / sgs_stress.f:62-131) ----

@@ -1,5 +1,5 @@
-"""CPU restatement of bla's eddy-viscosity LES model, sgs.i iles = 2
-(test infrastructure).
+"""CPU restatement of bla's eddy-viscosity LES models, sgs.i iles = 2 and
+iles = 3 (test infrastructure).
 
 Only the tests and the benchmark's CPU leg use this module, as the checker
 of the GPU model.  It follows:
@@ -23,6 +23,15 @@ of the GPU model.  It follows:
                     vrfftf (nx) and vcfftf (nz), z oddball zeroed;
 - sgs_stress.f:62-131  sigma_i = i alpha tau_i1 + d tau_i2/dy + i beta tau_i3
                     (the y derivative of the Chebyshev interpolant).
+
+iles = 3 (the HPF model, les.f:180-215, 404-620, 883-1380): the strain and
+|S| come from the high-pass filtered velocities and vorticities that
+linearbl.f:1871-2066 stores in gur (u - lpf (u - H_y^N u) with the ADM-RT
+filters of les_ref, H_y^N u = u - L u), no u0low/w0low shift; the dynamic
+procedure takes the Leonard term from the unfiltered velocities as for
+iles = 2, and its B_ij from the secondary high-pass filtered fields of
+linearbl.f:2067-2256 (the x-z filter of init_filter called with nx/4 + 1
+and nz/4 + 1 points at bla.f:1520-1543, the same y filter).
 
 Tensor order of tau and S: 11 12 13 22 23 33 (les.f:627-633).
 Pinned by tests/test_smag.py against the reference bla compiled with the
@@ -72,23 +81,11 @@ def cutoff_filter(p: np.ndarray) -> np.ndarray:
     return np.fft.irfft(X, n=nx, axis=2) * nx / (nx * nz)
 
 
-def les_tau(ur: np.ndarray, ui: np.ndarray, alfa: np.ndarray, beta: np.ndarray, nx: int,
-            dlt2: np.ndarray, cs: float = 0.0, ineg: int = 1, re: float = 1.0,
-            u0low: float = 0.0, w0low: float = 0.0) -> np.ndarray:
-    """Main storage slots 0-4 (u, v, w, om_x, om_z at physical y) ->
-    normalised stress spectra tau [6][nz][ny][nx/2]."""
-    nz, ny = ur.shape[1], ur.shape[2]
-    odd = nz // 2
-    u = (ur[:3] + 1j * ui[:3]).copy()
-    ox = ur[3] + 1j * ui[3]
-    oz = ur[4] + 1j * ui[4]
-    u[:, odd] = 0.0
-    ox = ox.copy()
-    oz = oz.copy()
-    ox[odd] = 0.0
-    oz[odd] = 0.0
-    u[0, 0, :, 0] -= u0low
-    u[2, 0, :, 0] -= w0low
+def strain(v: np.ndarray, alfa: np.ndarray, beta: np.ndarray, nx: int):
+    """S_ij (order 11 12 13 22 23 33) at physical x-z of the five spectral
+    fields v [5][nz][ny][nx/2] (u, v, w, om_x, om_z, the z oddball zeroed):
+    du/dy = dv/dx - om_z, dv/dy = -du/dx - dw/dz, dw/dy = om_x + dv/dz."""
+    u, ox, oz = v[:3], v[3], v[4]
     ia = 1j * alfa[None, None, :]
     ib = 1j * beta[:, None, None]
     g = [[None] * 3 for _ in range(3)]
@@ -98,10 +95,48 @@ def les_tau(ur: np.ndarray, ui: np.ndarray, alfa: np.ndarray, beta: np.ndarray, 
     g[0][1] = ia * u[1] - oz
     g[1][1] = -ia * u[0] - ib * u[2]
     g[2][1] = ox + ib * u[1]
-    up = [to_phys(u[i], nx) for i in range(3)]
     gp = [[to_phys(g[i][j], nx) for j in range(3)] for i in range(3)]
-    S = [gp[0][0], .5 * (gp[1][0] + gp[0][1]), .5 * (gp[2][0] + gp[0][2]), gp[1][1],
-         .5 * (gp[1][2] + gp[2][1]), gp[2][2]]
+    return [gp[0][0], .5 * (gp[1][0] + gp[0][1]), .5 * (gp[2][0] + gp[0][2]), gp[1][1],
+            .5 * (gp[1][2] + gp[2][1]), gp[2][2]]
+
+
+def hpf_filters(nx: int, ny: int, nz: int, cutoff: float, iord: int):
+    """(lpf, lpf2, L) of the HPF model: the primary x-z transfer function and
+    y low-pass of les_ref, and the secondary x-z transfer function, whose
+    homogeneous filters are those of a grid with half the wavenumbers
+    (zero above kx = nx/4 and |kz| = nz/4)."""
+    import les_ref
+    lpf, L = les_ref.filter_operators(nx, ny, nz, cutoff, iord)
+    gx = np.zeros(nx // 2)
+    gz = np.zeros(nz // 2 + 1)
+    gx[:nx // 4 + 1] = les_ref.homogeneous_transfer(nx // 4 + 1, cutoff)
+    gz[:nz // 4 + 1] = les_ref.homogeneous_transfer(nz // 4 + 1, cutoff)
+    kz = np.array([z if z <= nz // 2 else nz - z for z in range(nz)])
+    lpf2 = 1.0 - (1.0 - np.outer(gz[kz], gx)) ** (iord + 1)
+    return lpf, lpf2, L
+
+
+def les_tau(ur: np.ndarray, ui: np.ndarray, alfa: np.ndarray, beta: np.ndarray, nx: int,
+            dlt2: np.ndarray, cs: float = 0.0, ineg: int = 1, re: float = 1.0,
+            u0low: float = 0.0, w0low: float = 0.0, hpf=None) -> np.ndarray:
+    """Main storage slots 0-4 (u, v, w, om_x, om_z at physical y) ->
+    normalised stress spectra tau [6][nz][ny][nx/2]; hpf = (lpf, lpf2, L)
+    of hpf_filters for iles = 3."""
+    nz, ny = ur.shape[1], ur.shape[2]
+    odd = nz // 2
+    v = (ur[:5] + 1j * ui[:5]).copy()
+    v[:, odd] = 0.0
+    if hpf is None:
+        v[0, 0, :, 0] -= u0low
+        v[2, 0, :, 0] -= w0low
+        vs = v
+    else:
+        lpf, lpf2, L = hpf
+        W = np.einsum("yj,szjx->szyx", L, v)
+        vs = v - lpf[None, :, None, :] * W
+        vb = v - lpf2[None, :, None, :] * W
+    up = [to_phys(v[i], nx) for i in range(3)]
+    S = strain(vs, alfa, beta, nx)
     sabs = np.sqrt(2. * (S[0] ** 2 + S[3] ** 2 + S[5] ** 2) +
                    4. * (S[1] ** 2 + S[2] ** 2 + S[4] ** 2))
     d2 = dlt2[None, :, None]
@@ -109,7 +144,8 @@ def les_tau(ur: np.ndarray, ui: np.ndarray, alfa: np.ndarray, beta: np.ndarray, 
         dr = sabs * d2
         L0 = [cutoff_filter(-up[a] * up[b]) for a, b in PAIRS]
         M0 = [cutoff_filter(-S[k] * dr) for k in range(6)]
-        B = [cutoff_filter(S[k]) for k in range(6)]
+        SB = S if hpf is None else strain(vb, alfa, beta, nx)
+        B = [cutoff_filter(SB[k]) for k in range(6)]
         uf = [cutoff_filter(up[i]) for i in range(3)]
         L = [L0[k] + uf[a] * uf[b] for k, (a, b) in enumerate(PAIRS)]
         tsabs = np.sqrt(2. * (B[0] ** 2 + B[3] ** 2 + B[5] ** 2) +

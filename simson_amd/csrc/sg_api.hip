@@ -126,6 +126,11 @@ struct simson_gpu {
     bool smag = false;
     SmagArgs sm{};
     double2 *SQ = nullptr, *SS = nullptr, *SW = nullptr;
+    int sq_nf = 0;                    // fields SQ holds (12, or 21 for the dynamic HPF model)
+    // HPF model (iles 3): L u of slots 0-4 and the secondary x-z transfer
+    // function (Lm and lpf are the ADM-RT ones above)
+    double2 *LW = nullptr;
+    double *lpf2 = nullptr;
     double *dlt2 = nullptr, *SP = nullptr;
     size_t smag_lds = 0;
     // fringe of spatial simulations (simson_gpu_fringe): lambda [nxp/2] and
@@ -257,7 +262,7 @@ void free_all(simson_gpu_t *g)
                     g->twxs, g->xysd, g->SQ, g->SS, g->SW, g->dlt2, g->SP, g->flam, g->fbase,
                     g->flam32, g->fbase32, g->wallP, g->wallM, g->bj,
                     g->lam[0].hb, g->lam[0].hc, g->lam[1].hb, g->lam[1].hc,
-                    g->lam[2].hb, g->lam[2].hc, g->mforce, g->lforce};
+                    g->lam[2].hb, g->lam[2].hc, g->mforce, g->lforce, g->LW, g->lpf2};
     for (void *q : ptrs)
         if (q)
             (void)hipFree(q);
@@ -608,6 +613,12 @@ void smag_stresses(simson_gpu_t *g)
     a.U = g->U;
     a.u0low = g->u0low;
     a.w0low = g->w0low;
+    if (a.hpf) {
+        // L u of the five stored fields (linearbl.f:1871-2066 and, for the
+        // dynamic procedure, 2067-2256 filter the same L u in x-z)
+        GemmIO io = gemm_io(g, g->U, 0, g->LW, 0, {0, 1, 2, 3, 4}, {0, 1, 2, 3, 4}, g->nz / 2);
+        timed(g, F_LES, [&] { launch_cheb_gemm(g->oLm, io, g->stream); });
+    }
     timed(g, F_SMAG, [&] { launch_smag(a, g->pzs, g->pms, g->smag_lds, g->stream); });
     if (g->mixed) {
         launch_narrow(g->TAU, g->TAU32, 6 * field_size(g), g->stream);
@@ -1566,6 +1577,13 @@ int simson_gpu_fringe(simson_gpu_t *g, const simson_fringe_t *f, const double *b
     return 0;
 }
 
+int simson_les_filter_secondary(int nx, int nz, double cutoff, int iord, double *lpf2)
+{
+    if (!lpf2)
+        return SIMSON_E_ARG;
+    return les_filter_secondary(nx, nz, cutoff, iord, lpf2);
+}
+
 int simson_les_filter(int nx, int ny, int nz, double cutoff, int iord, double *lpf, double *L)
 {
     if (!lpf || !L)
@@ -1573,10 +1591,53 @@ int simson_les_filter(int nx, int ny, int nz, double cutoff, int iord, double *l
     return les_filter(nx, ny, nz, cutoff, iord, lpf, L);
 }
 
+// the ADM-RT filters (init_filter.f, bla.f:1489-1518) on the device: L
+// (FP64, and FP32 in the mixed form when fp32) and lpf; and, when lpf2 is
+// given, the secondary x-z transfer function of bla.f:1520-1543
+int setup_filters(simson_gpu_t *g, const simson_les_t *p, bool fp32, bool secondary)
+{
+    const int ny = g->ny;
+    std::vector<double> lpf((size_t)g->nz * g->nxh), L((size_t)ny * ny), lpf2;
+    if (les_filter(g->nx, ny, g->nz, p->cutoff, p->iord, lpf.data(), L.data()) != 0) {
+        g->err = "invalid LES filter parameters (0 < cutoff < 1, 0 <= iord <= 5)";
+        return SIMSON_E_ARG;
+    }
+    if (secondary) {
+        lpf2.resize(lpf.size());
+        if (les_filter_secondary(g->nx, g->nz, p->cutoff, p->iord, lpf2.data()) != 0) {
+            g->err = "invalid LES filter parameters (0 < cutoff < 1, 0 <= iord <= 5)";
+            return SIMSON_E_ARG;
+        }
+    }
+    if (!g->Lm && (dalloc(g, &g->Lm, L.size()) || dalloc(g, &g->lpf, lpf.size())))
+        return SIMSON_E_HIP;
+    if (fp32 && !g->Lm32 && dalloc(g, &g->Lm32, L.size()))
+        return SIMSON_E_HIP;
+    if (secondary && !g->lpf2 && dalloc(g, &g->lpf2, lpf2.size()))
+        return SIMSON_E_HIP;
+    if (h2d(g, g->Lm, L.data(), L.size()) || h2d(g, g->lpf, lpf.data(), lpf.size()))
+        return SIMSON_E_HIP;
+    if (secondary && h2d(g, g->lpf2, lpf2.data(), lpf2.size()))
+        return SIMSON_E_HIP;
+    std::vector<float> L32;
+    if (fp32) {
+        L32.assign(L.begin(), L.end());
+        if (h2d(g, g->Lm32, L32.data(), L32.size()))
+            return SIMSON_E_HIP;
+    }
+    if (int st = make_op(g, g->oLm, L, g->Lm, fp32 ? g->Lm32 : nullptr, 0, 0))
+        return st;
+    // the copies read host vectors that go out of scope on return
+    CK(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
 // sgs.i iles = 2 (rparamles.f:72-81, 117-165): fixed cs or, with cs = 0,
-// the dynamic procedure with clipping ineg; Delta^2 of preprbl.f:60-71
+// the dynamic procedure with clipping ineg; Delta^2 of preprbl.f:60-71.
+// iles = 3 (rparamles.f:82-91): the same on the high-pass filtered fields.
 int setup_smag(simson_gpu_t *g, const simson_les_t *p)
 {
+    const bool hpf = p->iles == 3;
     if (p->ineg < 0 || p->ineg > 2 || (p->cs != 0. && p->ineg != 1)) {
         g->err = "Smagorinsky: 0 <= ineg <= 2, and ineg = 1 with a fixed cs";
         return SIMSON_E_ARG;
@@ -1599,10 +1660,25 @@ int setup_smag(simson_gpu_t *g, const simson_les_t *p)
     const int ny = g->ny, nz = g->nz, nq = g->nx / 4;
     const size_t F = field_size(g);
     const bool dyn = p->cs == 0.;
+    const int nqf = hpf && dyn ? 21 : 12;
+    if (hpf) {
+        if (int st = setup_filters(g, p, false, dyn))
+            return st;
+        if (!g->LW && dalloc(g, &g->LW, 5 * F))
+            return SIMSON_E_HIP;
+    }
+    if (g->SQ && g->sq_nf < nqf) {
+        (void)hipFree(g->SQ);
+        g->SQ = nullptr;
+        if (dalloc(g, &g->SQ, (size_t)nqf * F))
+            return SIMSON_E_HIP;
+        g->sq_nf = nqf;
+    }
     if (!g->SQ) {
-        if (dalloc(g, &g->SQ, 12 * F) || dalloc(g, &g->SS, 7 * F) ||
+        if (dalloc(g, &g->SQ, (size_t)nqf * F) || dalloc(g, &g->SS, 7 * F) ||
             dalloc(g, &g->dlt2, (size_t)ny))
             return SIMSON_E_HIP;
+        g->sq_nf = nqf;
         if (smag_prepare(lds) != 0) {
             g->err = "Smagorinsky: cannot raise the LDS limit of the x-row kernels";
             return SIMSON_E_HIP;
@@ -1640,6 +1716,11 @@ int setup_smag(simson_gpu_t *g, const simson_les_t *p)
     a.nxh = g->nxh;
     a.nq = nq;
     a.dynamic = dyn;
+    a.hpf = hpf;
+    a.nqf = nqf;
+    a.LW = g->LW;
+    a.lpf = g->lpf;
+    a.lpf2 = g->lpf2;
     a.ineg = p->ineg;
     a.cs2 = p->cs * p->cs;
     a.tw2 = pow(4., 2. / 3.);
@@ -1660,11 +1741,10 @@ int simson_gpu_les(simson_gpu_t *g, const simson_les_t *p)
         g->smag = false;
         return 0;
     }
-    if (p->iles == 2)
+    if (p->iles == 2 || p->iles == 3)
         return setup_smag(g, p);
     if (p->iles != 1) {
-        g->err = "iles must be 0, 1 (ADM-RT) or 2 (Smagorinsky); the HPF model (3) is not "
-                 "supported";
+        g->err = "iles must be 0, 1 (ADM-RT), 2 (Smagorinsky) or 3 (HPF Smagorinsky)";
         return SIMSON_E_ARG;
     }
     if (!has_zfft4(g->nzp) || (g->mixed && !cheb_f32_ok(g->ny))) {
@@ -1672,28 +1752,11 @@ int simson_gpu_les(simson_gpu_t *g, const simson_les_t *p)
         return SIMSON_E_ARG;
     }
     const int ny = g->ny;
-    std::vector<double> lpf((size_t)g->nz * g->nxh), L((size_t)ny * ny);
-    if (les_filter(g->nx, ny, g->nz, p->cutoff, p->iord, lpf.data(), L.data()) != 0) {
-        g->err = "invalid ADM-RT filter parameters (0 < cutoff <= 1, 0 <= iord <= 5)";
-        return SIMSON_E_ARG;
-    }
-    const size_t n3 = 3 * field_size(g);
-    if (!g->Lm) {
-        if (dalloc(g, &g->Lm, L.size()) || dalloc(g, &g->lpf, lpf.size()))
-            return SIMSON_E_HIP;
-        if (g->mixed ? dalloc(g, &g->Lm32, L.size()) || dalloc(g, &g->Wf32, n3)
-                     : dalloc(g, &g->Wf, n3))
-            return SIMSON_E_HIP;
-    }
-    if (h2d(g, g->Lm, L.data(), L.size()) || h2d(g, g->lpf, lpf.data(), lpf.size()))
-        return SIMSON_E_HIP;
-    if (g->mixed) {
-        std::vector<float> L32(L.begin(), L.end());
-        if (h2d(g, g->Lm32, L32.data(), L32.size()))
-            return SIMSON_E_HIP;
-    }
-    if (int st = make_op(g, g->oLm, L, g->Lm, g->Lm32, 0, 0))
+    if (int st = setup_filters(g, p, g->mixed, false))
         return st;
+    const size_t n3 = 3 * field_size(g);
+    if (g->mixed ? !g->Wf32 && dalloc(g, &g->Wf32, n3) : !g->Wf && dalloc(g, &g->Wf, n3))
+        return SIMSON_E_HIP;
     if (g->mixed && !g->oLm.S32 && cheb_mfma_ks(ny) == 0) {
         g->err = "the ADM-RT low-pass has no matrix-core form for this ny";
         return SIMSON_E_ARG;

@@ -189,6 +189,8 @@ int fringe_function(int nx, double xl, const simson_fringe_t *f, double *lam);
 // ADM-RT filter operators (host): lpf [nz][nx/2] and L = I - H_y^(iord+1)
 // [ny][ny]; 0 or SIMSON_E_ARG
 int les_filter(int nx, int ny, int nz, double cutoff, int iord, double *lpf, double *L);
+// the secondary x-z transfer function of the dynamic HPF model [nz][nx/2]
+int les_filter_secondary(int nx, int nz, double cutoff, int iord, double *lpf2);
 
 void launch_zsyn(const NonlinArgs &a, const FftPlan &pz, hipStream_t s);
 void launch_xrow(const NonlinArgs &a, const FftPlan &pm, const double2 *wx, hipStream_t s);
@@ -232,6 +234,14 @@ struct SmagArgs {
     double2 *Q, *SS, *W, *TAU;
     double *part;                 // [ny][nzb][2][nx] partial z sums (dynamic)
     const double *alfa, *beta, *dlt2;
+    // HPF model (iles 3): LW = L u of slots 0-4 [5][nz][ny][nxh] (the ADM-RT
+    // wall-normal low-pass), and the primary and secondary x-z transfer
+    // functions [nz][nxh]; the strain comes from u - lpf LW, the dynamic
+    // procedure's B_ij from u - lpf2 LW
+    const double2 *LW;
+    const double *lpf, *lpf2;
+    int hpf;
+    int nqf;                      // fields of Q: 12, or 21 with the secondary gradients
     int nx, ny, nz, nxh, nq;
     int dynamic, ineg, rt;
     double u0low, w0low, cs2, tw2, re, hscale;

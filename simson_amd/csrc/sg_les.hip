@@ -245,4 +245,25 @@ int les_filter(int nx, int ny, int nz, double cutoff, int iord, double *lpf, dou
     return 0;
 }
 
+int les_filter_secondary(int nx, int nz, double cutoff, int iord, double *lpf2)
+{
+    // the homogeneous filters sampled on nx/4 + 1 and nz/4 + 1 points
+    // (bla.f:1520-1543): the transfer function of a grid with half the
+    // wavenumbers, zero above kx = nx/4 and |kz| = nz/4
+    if (nx < 8 || nz < 4 || iord < 0 || iord > 5 || !(cutoff > 0.) || !(cutoff < 1.))
+        return SIMSON_E_ARG;
+    const int N = iord + 1, nxh = nx / 2, mx = nx / 4, mz = nz / 4;
+    std::vector<double> gx, gz;
+    if (!uniform_transfer(mx + 1, cutoff, gx) || !uniform_transfer(mz + 1, cutoff, gz))
+        return SIMSON_E_ARG;
+    for (int z = 0; z < nz; z++) {
+        const int kz = z <= nz / 2 ? z : nz - z;
+        for (int x = 0; x < nxh; x++) {
+            const double f = (x <= mx && kz <= mz) ? gx[x] * gz[kz] : 0.;
+            lpf2[(size_t)z * nxh + x] = 1. - std::pow(1. - f, N);
+        }
+    }
+    return 0;
+}
+
 }  // namespace sg

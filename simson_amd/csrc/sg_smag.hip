@@ -6,10 +6,21 @@
 // plane, as les.f does (called per plane from bla.f:1945-1998 at the start
 // of each substep).
 //
+// sgs.i iles = 3 (the HPF model, les.f:180-215, 404-620, 883-1380) is the
+// same pipeline with the strain of the high-pass filtered velocities and
+// vorticities u - lpf L u that linearbl.f:1871-2066 stores (L the ADM-RT
+// wall-normal low-pass, lpf its x-z transfer function; L u comes from a
+// y-transform product over the five slots before the pipeline), no
+// u0low/w0low shift, and for the dynamic procedure B_ij from the secondary
+// filter u - lpf2 L u of linearbl.f:2067-2256 (the x-z filter of a grid
+// with half the wavenumbers, bla.f:1520-1543).
+//
 // Kernels (all y planes at once):
 //   k_smag_z   u, v, w and the nine velocity gradients (les.f:278-350, from
 //              the stored velocities and om_x, om_z) synthesised in z
-//              (vcfftb, length nz) into Q [12][ny][nz][nx/2];
+//              (vcfftb, length nz) into Q [12][ny][nz][nx/2]; with the
+//              dynamic HPF model also the nine gradients of the secondary
+//              filtered fields (Q [21][...]);
 //   k_smag_xa  per row: inverse real x transform (vrfftb), S_ij and |S|
 //              (les.f:627-645) kept at physical x-z in SS [7][ny][nz][nx/2]
 //              (even points in .x, odd in .y); for the dynamic procedure the
@@ -62,6 +73,17 @@ __device__ __forceinline__ double sabs_of(const double *s)
                 4. * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
 }
 
+// S_ij (order 11 12 13 22 23 33) of the gradients g[3 i + j] = du_i/dx_j
+__device__ __forceinline__ void strain_of(const double *g, double *s)
+{
+    s[0] = g[0];
+    s[1] = .5 * (g[3] + g[1]);
+    s[2] = .5 * (g[6] + g[2]);
+    s[3] = g[4];
+    s[4] = .5 * (g[5] + g[7]);
+    s[5] = g[8];
+}
+
 // the plan with its twiddle table copied to LDS at dst (tabn values)
 __device__ FftPlan plan_in_lds(const FftPlan &p, double2 *dst)
 {
@@ -80,8 +102,14 @@ __global__ void __launch_bounds__(kT) k_smag_z(SmagArgs a, FftPlan pz, int B)
     const size_t F = (size_t)nz * ny * nxh;
     double2 *buf0 = lds, *buf1 = lds + (size_t)B * nz;
     const FftPlan pl = plan_in_lds(pz, lds + (size_t)2 * B * nz);
-    const int f = blockIdx.y, y = blockIdx.z, x0 = blockIdx.x * B;
+    const int fq = blockIdx.y, y = blockIdx.z, x0 = blockIdx.x * B;
     const int nb = min(B, nxh - x0);
+    // output fields 12..20 are the gradients 3..11 of the secondary filtered
+    // fields; sel 0 reads the stored fields, 1 the primary high-pass filtered
+    // and 2 the secondary ones
+    const int f = fq < 12 ? fq : fq - 9;
+    const int sel = fq < 3 ? 0 : (fq < 12 ? (a.hpf ? 1 : 0) : 2);
+    const double *lp = sel == 1 ? a.lpf : a.lpf2;
     // the fields this block reads: fa, and fb for the y derivatives of u and
     // w (p1 is loaded from fa again where it is not needed)
     int fa = f, fb = f;
@@ -104,6 +132,12 @@ __global__ void __launch_bounds__(kT) k_smag_z(SmagArgs a, FftPlan pz, int B)
             const size_t o = sidx(0, z, y, x, nz, ny, nxh);
             p0[i] = a.U[fa * F + o];
             p1[i] = a.U[fb * F + o];
+            if (sel) {
+                const double l = lp[(size_t)z * nxh + x];
+                const double2 w0 = a.LW[fa * F + o], w1 = a.LW[fb * F + o];
+                p0[i] = make_double2(p0[i].x - l * w0.x, p0[i].y - l * w0.y);
+                p1[i] = make_double2(p1[i].x - l * w1.x, p1[i].y - l * w1.y);
+            }
             al[i] = a.alfa[x];
             be[i] = a.beta[z];
         }
@@ -118,7 +152,7 @@ __global__ void __launch_bounds__(kT) k_smag_z(SmagArgs a, FftPlan pz, int B)
                 const int x = x0 + b;
                 if (f < 3) {
                     v = p0[i];
-                    if (x == 0 && z == 0) {
+                    if (x == 0 && z == 0 && !a.hpf) {
                         if (f == 0)
                             v.x -= a.u0low;
                         if (f == 2)
@@ -151,7 +185,7 @@ __global__ void __launch_bounds__(kT) k_smag_z(SmagArgs a, FftPlan pz, int B)
     for (int idx = threadIdx.x; idx < nz * B; idx += blockDim.x) {
         const int z = idx / B, b = idx - z * B;
         if (b < nb)
-            a.Q[qidx(f, y, z, x0 + b, ny, nz, nxh)] = res[b * nz + z];
+            a.Q[qidx(fq, y, z, x0 + b, ny, nz, nxh)] = res[b * nz + z];
     }
 }
 
@@ -231,10 +265,10 @@ __global__ void __launch_bounds__(kT) k_smag_xa(SmagArgs a, FftPlan pm, int RW)
     // factors of the real-transform packing; both are read from LDS
     const FftPlan pl = plan_in_lds(pm, lds + (size_t)2 * NF * RW * M);
     const int z0 = blockIdx.x * RW, y = blockIdx.y;
-    const int nr = min(RW, nz - z0);
-    pack_rows(A, a.Q, 12, RW, nr, y, z0, ny, nz, M, M, pl.tw);
+    const int nr = min(RW, nz - z0), NQ = a.nqf;
+    pack_rows(A, a.Q, NQ, RW, nr, y, z0, ny, nz, M, M, pl.tw);
     __syncthreads();
-    double2 *res = lds_fft(A, Bf, M, 12 * RW, pl, +1.0);
+    double2 *res = lds_fft(A, Bf, M, NQ * RW, pl, +1.0);
     double2 *oth = res == A ? Bf : A;
     const double d2 = a.dlt2[y];
     for (int idx = threadIdx.x; idx < RW * M; idx += blockDim.x) {
@@ -256,20 +290,30 @@ __global__ void __launch_bounds__(kT) k_smag_xa(SmagArgs a, FftPlan pm, int RW)
                 g[i] = e ? q[3 + i].y : q[3 + i].x;
             // g[3 i + j] = du_i/dx_j
             double *sk = sv[e];
-            sk[0] = g[0];
-            sk[1] = .5 * (g[3] + g[1]);
-            sk[2] = .5 * (g[6] + g[2]);
-            sk[3] = g[4];
-            sk[4] = .5 * (g[5] + g[7]);
-            sk[5] = g[8];
+            strain_of(g, sk);
             sk[6] = sabs_of(sk);
             const double dr = sk[6] * d2;
+            // B_ij: the strain itself, or that of the secondary filtered
+            // fields (dynamic HPF model)
+            double sb[6];
+            if (NQ == 21) {
+#pragma unroll
+                for (int i = 0; i < 9; i++) {
+                    const double2 t = res[((12 + i) * RW + r) * M + n];
+                    g[i] = e ? t.y : t.x;
+                }
+                strain_of(g, sb);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 6; k++)
+                    sb[k] = sk[k];
+            }
             const int pa[6] = {0, 1, 2, 1, 2, 2}, pb[6] = {0, 0, 0, 1, 1, 2};
 #pragma unroll
             for (int k = 0; k < 6; k++) {
                 o[e][k] = -u[pa[k]] * u[pb[k]];
                 o[e][6 + k] = -sk[k] * dr;
-                o[e][12 + k] = sk[k];
+                o[e][12 + k] = sb[k];
             }
 #pragma unroll
             for (int i = 0; i < 3; i++)
@@ -587,7 +631,8 @@ void launch_smag(SmagArgs a, const FftPlan &pz, const FftPlan &pm,
     const int B = zblock(a.nz, a.nxh);
     const size_t lz = ((size_t)2 * B * a.nz + a.nz) * sizeof(double2);
     const size_t twx = (size_t)a.nx * sizeof(double2);
-    hipLaunchKernelGGL(k_smag_z, dim3((a.nxh + B - 1) / B, 12, a.ny), dim3(kT), lz, s, a, pz, B);
+    hipLaunchKernelGGL(k_smag_z, dim3((a.nxh + B - 1) / B, a.nqf, a.ny), dim3(kT), lz, s, a, pz,
+                       B);
     const int NF = a.dynamic ? 21 : 12;
     const int RA = std::min(a.nz, smag_rows(a.nxh, NF, lds_max));
     hipLaunchKernelGGL(k_smag_xa, dim3((a.nz + RA - 1) / RA, a.ny), dim3(kT),

@@ -225,7 +225,8 @@ program blagpu
   if (dt < 0._dp) dtmax = -dt
   cflmax = cflmaxin * sqrt(3._dp)
 
-  ! sgs.i (rparamles.f:23-165): only ADM-RT (iles = 1) or no model
+  ! sgs.i (rparamles.f:23-165): ADM-RT (iles = 1), Smagorinsky (2), the
+  ! Smagorinsky model on the high-pass filtered velocity (3) or no model
   if (sgs) then
      les%iles = 1; cutoff_inv = 1.5_dp; les%iord = 5; les%chi = 0.2_dp
      les%cs = 0._dp; les%ineg = 1
@@ -243,8 +244,16 @@ program blagpu
         if (les%cs == 0._dp) then
            call nxt(line); read(line, *) les%ineg
         end if
+     else if (les%iles == 3) then
+        ! rparamles.f:82-91: the filter, cs, and ineg for the dynamic procedure
+        call nxt(line); read(line, *) cutoff_inv
+        call nxt(line); read(line, *) les%iord
+        call nxt(line); read(line, *) les%cs
+        if (les%cs == 0._dp) then
+           call nxt(line); read(line, *) les%ineg
+        end if
      else if (les%iles /= 0) then
-        write(*,*) 'blagpu: the HPF eddy-viscosity model (iles = 3) is not supported'
+        write(*,*) 'blagpu: iles must be between 0 and 3: ', les%iles
         call stopnow(45543)
      end if
      close(10)

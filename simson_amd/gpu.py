@@ -29,7 +29,8 @@ EXPORTS = (
     "simson_gpu_run", "simson_gpu_cfl", "simson_gpu_download", "simson_gpu_amp", "simson_gpu_amp_noalloc",
     "simson_gpu_xys",
     "simson_gpu_pressure",
-    "simson_gpu_les", "simson_les_filter", "simson_gpu_closure", "simson_gpu_sync",
+    "simson_gpu_les", "simson_les_filter", "simson_les_filter_secondary",
+    "simson_gpu_closure", "simson_gpu_sync",
     "simson_gpu_fringe", "simson_fringe_function", "simson_gpu_wall", "simson_gpu_set_time",
     "simson_gpu_free_stream", "simson_fsc_table", "simson_base_flow",
     "simson_parallel_force", "simson_gpu_mean_force", "simson_gpu_local_force",
@@ -114,6 +115,8 @@ def lib():
         L.simson_gpu_xys.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int, _dp]
         L.simson_gpu_pressure.argtypes = [vp, _dp, _dp]
         L.simson_gpu_les.argtypes = [vp, C.POINTER(SimsonLes)]
+        L.simson_les_filter_secondary.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int,
+                                                  _dp]
         L.simson_les_filter.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
                                         _dp, _dp]
         L.simson_gpu_closure.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -265,8 +268,9 @@ class BlaGPU:
     def les(self, iles=1, cutoff: float = 2.0 / 3.0, iord: int = 5, chi: float = 0.2,
             cs: float = 0.0, ineg: int = 1):
         """Large-eddy model of sgs.i: iles = 1 ADM-RT (cutoff = 1/cutoff_inv),
-        iles = 2 Smagorinsky (cs = 0: dynamic procedure, clipping ineg), 0
-        none.  iles may also be a simson_amd.sgsi.SgsParams."""
+        iles = 2 Smagorinsky (cs = 0: dynamic procedure, clipping ineg),
+        iles = 3 Smagorinsky on the high-pass filtered velocity (cutoff, iord,
+        cs, ineg), 0 none.  iles may also be a simson_amd.sgsi.SgsParams."""
         if not isinstance(iles, int):
             sp = iles
             iles, cutoff, iord, chi = sp.iles, sp.cutoff, sp.iord, sp.chi
@@ -391,6 +395,16 @@ def les_filter(nx: int, ny: int, nz: int, cutoff: float, iord: int):
     if st != 0:
         raise SimsonError(st, "invalid ADM-RT filter parameters")
     return lpf, L
+
+
+def les_filter_secondary(nx: int, nz: int, cutoff: float, iord: int):
+    """The library's secondary x-z transfer function of the dynamic HPF model
+    (lpf2 [nz][nx/2]); host computation, no device needed."""
+    lpf2 = np.zeros((nz, nx // 2))
+    st = lib().simson_les_filter_secondary(nx, nz, cutoff, iord, lpf2)
+    if st != 0:
+        raise SimsonError(st, "invalid LES filter parameters")
+    return lpf2
 
 
 def fringe_function(nx: int, xl: float, fmax: float, fstart: float, fend: float,

@@ -5,14 +5,15 @@ Grammar (version 20060909): comment lines start with ``#``; the values are
 ``version``, ``iles`` and, for ``iles = 1`` (ADM-RT, the relaxation-term
 model), ``cutoff_inv``, ``iord`` and ``chi`` (rparamles.f:62-71); for
 ``iles = 2`` (Smagorinsky), ``cs`` and, when cs = 0 (the dynamic
-procedure), ``ineg`` (rparamles.f:72-81).  The defaults of
+procedure), ``ineg`` (rparamles.f:72-81); for ``iles = 3`` (the
+Smagorinsky model on the high-pass filtered velocity), ``cutoff_inv``,
+``iord``, ``cs`` and, when cs = 0, ``ineg`` (rparamles.f:82-91).  The defaults of
 rparamles.f:28-51 apply to values a model does not read.  The checks of
 rparamles.f:117-165 are repeated (cutoff = 1/cutoff_inv in [0, 1],
 0 <= iord <= 5, 0 <= ineg <= 2, ineg = 1 for a fixed cs); like the
 reference, a bad file stops with code 45543.
 
-The step library supports iles 0, 1 and 2; the HPF eddy-viscosity model
-(iles 3) is refused.
+The step library supports every iles of the grammar (0 to 3).
 """
 from __future__ import annotations
 
@@ -66,7 +67,11 @@ def parse(text: str) -> SgsParams:
             if p.cs == 0.0:
                 p.ineg = int(next(it))
         elif p.iles == 3:
-            raise SgsInputError("the HPF eddy-viscosity model (iles 3) is not supported")
+            p.cutoff_inv = float(next(it).replace("d", "e").replace("D", "e"))
+            p.iord = int(next(it))
+            p.cs = float(next(it).replace("d", "e").replace("D", "e"))
+            if p.cs == 0.0:
+                p.ineg = int(next(it))
         elif p.iles != 0:
             raise SgsInputError(f"iles must be between 0 and 3: {p.iles}")
     except StopIteration:
@@ -94,4 +99,7 @@ def format_sgs(p: SgsParams) -> str:
         out += [f"{p.cutoff_inv!r}", f"{p.iord}", f"{p.chi!r}"]
     elif p.iles == 2:
         out += [f"{p.cs!r}"] + ([f"{p.ineg}"] if p.cs == 0.0 else [])
+    elif p.iles == 3:
+        out += [f"{p.cutoff_inv!r}", f"{p.iord}", f"{p.cs!r}"] + \
+            ([f"{p.ineg}"] if p.cs == 0.0 else [])
     return "\n".join(out) + "\n"

@@ -265,17 +265,29 @@ SMAG_CASES = {
 }
 
 
-def smag():
-    """Eddy-viscosity fixture (sgs.i iles = 2): the reference is compiled
-    with iiles_smag = 1 (oracle/build_ref.sh with SMAG=1), started from the
-    LES field, 10 steps at dt = 0.01.  Keys <case>_u10 (and d1_u1)."""
+HPF_CASES = {
+    # the Smagorinsky model on the high-pass filtered velocity (iles = 3):
+    # a fixed coefficient, and the dynamic procedure with clipping; 10 steps,
+    # saves after 1 step for d1
+    "c": (sgsi.SgsParams(iles=3, cutoff_inv=1.5, iord=5, cs=0.15, ineg=1), []),
+    "d1": (sgsi.SgsParams(iles=3, cutoff_inv=1.5, iord=5, cs=0.0, ineg=1), [0.01]),
+    "d0": (sgsi.SgsParams(iles=3, cutoff_inv=3.0, iord=2, cs=0.0, ineg=0), []),
+}
+
+
+def smag(cases=None, outname="smag_32x33x32.npz"):
+    """Eddy-viscosity fixture (sgs.i iles = 2, or iles = 3 with
+    cases=HPF_CASES): the reference is compiled with iiles_smag = 1
+    (oracle/build_ref.sh with SMAG=1), started from the LES field, 10 steps
+    at dt = 0.01.  Keys <case>_u10 (and d1_u1)."""
+    cases = SMAG_CASES if cases is None else cases
     if not os.path.exists(SBIN):
         env = dict(os.environ, SMAG="1")
         subprocess.check_call([os.path.join(REPO, "oracle", "build_ref.sh"),
                                str(NX), str(NY), str(NZ), "1", "no", "0",
                                os.path.basename(SBIN)], env=env)
     res = {}
-    for key, (sp, tsave) in SMAG_CASES.items():
+    for key, (sp, tsave) in cases.items():
         work = tempfile.mkdtemp(prefix="simson_golden_smag.")
         try:
             ufile.write_u(os.path.join(work, "init.u"), les_field())
@@ -297,8 +309,8 @@ def smag():
             res[f"{key}_u10"] = ufile.read_u(os.path.join(work, "end.u")).u
         finally:
             shutil.rmtree(work)
-    np.savez_compressed(os.path.join(HERE, "smag_32x33x32.npz"), **res)
-    print("wrote", os.path.join(HERE, "smag_32x33x32.npz"), sorted(res))
+    np.savez_compressed(os.path.join(HERE, outname), **res)
+    print("wrote", os.path.join(HERE, outname), sorted(res))
 
 
 FRINGE_CASES = {
@@ -643,6 +655,8 @@ if __name__ == "__main__":
         xys()
     elif len(sys.argv) > 1 and sys.argv[1] == "smag":
         smag()
+    elif len(sys.argv) > 1 and sys.argv[1] == "hpf":
+        smag(HPF_CASES, "hpf_32x33x32.npz")
     elif len(sys.argv) > 1 and sys.argv[1] == "fringe":
         fringe()
     elif len(sys.argv) > 1 and sys.argv[1] == "wall":

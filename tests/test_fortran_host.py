@@ -96,11 +96,12 @@ def _les_dir(sp, **over):
 
 
 def test_rejects_unsupported_sgs_models():
-    """sgs.i is read before any device call: the HPF eddy-viscosity model
-    (iles 3) and out-of-range parameters (ineg 3, iord 7) stop with
-    rparamles.f's code 45543, a wrong version with 345432."""
+    """sgs.i is read before any device call: an unknown model (iles 4) and
+    out-of-range parameters (ineg 3, iord 7) stop with rparamles.f's code
+    45543, a wrong version with 345432."""
     _need_binary()
-    for text, code in (("20060909\n3\n1.5\n5\n0.\n1\n", "45543"),
+    for text, code in (("20060909\n4\n", "45543"),
+                       ("20060909\n3\n1.5\n7\n0.1\n", "45543"),
                        ("20060909\n2\n0.\n3\n", "45543"),
                        ("20060909\n1\n1.5\n7\n0.2\n", "45543"),
                        ("20060101\n1\n", "345432")):
@@ -248,6 +249,28 @@ def test_fortran_host_smag_matches_reference_bla():
     print("[gpu] blagpu with sgs.i (dynamic Smagorinsky) 32x33x32", flush=True)
     golden = np.load(os.path.join(ROOT, "tests", "golden", "smag_32x33x32.npz"))
     work = _les_dir(sgsi.SgsParams(iles=2, cs=0.0, ineg=1))
+    try:
+        r = _run(work)
+        assert r.returncode == 0, r.stdout + r.stderr
+        for name, key in (("t1.u", "d1_u1"), ("end.u", "d1_u10")):
+            u = np.delete(ufile.read_u(os.path.join(work, name)).u, 16, axis=1)
+            rr = np.delete(golden[key], 16, axis=1)
+            for c in range(3):
+                err = _rel(u[c], rr[c])
+                print(f"    {name} comp {c}: rel L2 {err:.2e}", flush=True)
+                assert err <= 1e-12, (name, c, err)
+    finally:
+        shutil.rmtree(work)
+
+
+@pytest.mark.gpu
+def test_fortran_host_hpf_matches_reference_bla():
+    """The host with sgs.i iles = 3 (the dynamic HPF model, ineg = 1) writes
+    the fields the reference bla wrote (tests/golden/hpf_32x33x32.npz)."""
+    _need_binary()
+    print("[gpu] blagpu with sgs.i (dynamic HPF Smagorinsky) 32x33x32", flush=True)
+    golden = np.load(os.path.join(ROOT, "tests", "golden", "hpf_32x33x32.npz"))
+    work = _les_dir(sgsi.SgsParams(iles=3, cutoff_inv=1.5, iord=5, cs=0.0, ineg=1))
     try:
         r = _run(work)
         assert r.returncode == 0, r.stdout + r.stderr

@@ -65,7 +65,7 @@ def test_sgs_file_round_trip_and_checks():
     with pytest.raises(sgsi.SgsInputError):
         sgsi.parse("20060909\n1\n1.5\n6\n0.2\n")       # iord > 5
     with pytest.raises(sgsi.SgsInputError):
-        sgsi.parse("20060909\n3\n1.5\n5\n0.\n1\n")   # HPF eddy viscosity: not supported
+        sgsi.parse("20060909\n4\n")                     # no such model
     with pytest.raises(sgsi.SgsInputError) as e:
         sgsi.parse("20000101\n1\n")
     assert e.value.code == 345432
