@@ -48,9 +48,10 @@ extern "C" {
 typedef struct simson_params {
     int nx, ny, nz;      /* par.f nx, ny (= nyp, nfyd = 0), nz; nfxd = nfzd = 1 */
     int precision;       /* SIMSON_FP64 or SIMSON_MIXED */
-    int ibc;             /* bla.i ibc: 0 (no-slip walls) or 101 (free
-                            stream du_i/dy = dU_i/dy above, no-slip below;
-                            needs cflux = 0, see simson_gpu_free_stream) */
+    int ibc;             /* bla.i ibc: 0 (no-slip walls), 100 (free
+                            stream u_i = U_i above) or 101 (du_i/dy =
+                            dU_i/dy above), no-slip below; 100 and 101
+                            need cflux = 0, see simson_gpu_free_stream */
     int cflux;           /* bla.i cflux (constant mass flux) */
     double re;           /* Reynolds number from the .u header */
     double xl, zl;       /* box lengths (bla.i xl, .u zl) */
@@ -331,7 +332,7 @@ int simson_gpu_local_force(simson_gpu_t *g, const simson_locf_t *lf);
 int simson_local_force_pattern(int nx, int ny, double xl, const simson_locf_t *lf, double *pr,
                                double *pi);
 
-/* Free-stream data of ibc = 101 (replaces evalbc.f:142-159's use of
+/* Free-stream data of ibc = 101 or 100 (replaces evalbc.f:142-159's use of
  * bflow.f's bu1jr and du0upp): bjr/bji [3][nx/2], component-major, the
  * base flow's dU/dy, dV/dy, dW/dy at y = +1 per kx (Fourier coefficients
  * normalised by 1/nxp, bflow.f:80-105), imposed on the kz = 0 plane, or both
@@ -339,8 +340,10 @@ int simson_local_force_pattern(int nx, int ny, double xl, const simson_locf_t *l
  * condition (linearbl.f:1516-1531).  With ibc = 101 the mean mode's wall
  * values are U = 1 above (then replaced by du0upp) and 0 below, as bla.f:
  * 1360-1365 sets them for every base flow.  Until called, b = 0 and
- * du0upp = 0.  Returns SIMSON_E_ARG unless init had ibc = 101.
- * Synchronises. */
+ * du0upp = 0.  With ibc = 100 bjr/bji hold instead the base flow's values
+ * U, V, W at y = +1 (bu1jr(:, i, 1)) and du0upp is not used: u_i(+1) =
+ * U_i(+1) on the kz = 0 plane, and the mean mode keeps U = 1 above.
+ * Returns SIMSON_E_ARG unless init had ibc = 100 or 101.  Synchronises. */
 int simson_gpu_free_stream(simson_gpu_t *g, const double *bjr, const double *bji, double du0upp);
 
 /* The time t of the start of the next substep, the tc of bla.f:1797 that

@@ -351,8 +351,8 @@ struct orc {
     double *fr, *fi;                   /* [3][nz][ny][nxh] external force or NULL */
     double *flam, *fbase;              /* fringe [nxp] and base flow [3][ny][nxp] or NULL */
     double *wvr, *wvi;                 /* lower-wall v [nz][nxh] (spectral) or NULL */
-    int ibc;                           /* 0 (no-slip) or 101 (free stream above) */
-    double *bjr, *bji;                 /* [3][nxh] dU_i/dy at y = 1 of the base flow */
+    int ibc;                           /* 0 (no-slip), 100 or 101 (free stream above) */
+    double *bjr, *bji;                 /* [3][nxh] U_i (ibc 100) or dU_i/dy (101) at y = 1 */
     double du0upp;                     /* its mean (kx = 0) streamwise part */
     double *mf;                        /* [3][ny] mean force or NULL */
 };
@@ -802,13 +802,31 @@ static void linear_plane(orc_t *o, double *ur, double *ui, double *puw, int z,
             }
             fuw1[0] += 2. * re * o->px;
         }
+        /* ibc = 100 (evalbc.f:142-159 with the values bu1jr(:, i, 1) of
+         * bflow.f:80-84): u_i(+1) = U_i(+1) of the base flow on the kz = 0
+         * plane (zero elsewhere, a wall), so v(+1) = V, and by continuity
+         * and the definition of omega_y dv/dy(+1) = -i (alpha U + beta W),
+         * omega_y(+1) = i (beta U - alpha W); the mean mode keeps its
+         * Dirichlet values below */
+        double dUr = 0., dUi = 0., dVr = 0., dVi = 0., dOr = 0., dOi = 0.;
+        if (o->ibc == 100 && z == 0 && o->bjr && !k0) {
+            double Ur = o->bjr[x], Ui = o->bji[x];
+            double Wr = o->bjr[2 * nxh + x], Wi = o->bji[2 * nxh + x];
+            dVr = o->bjr[nxh + x];
+            dVi = o->bji[nxh + x];
+            dUr = alfa * Ui + beta * Wi;           /* dv/dy(+1) */
+            dUi = -(alfa * Ur + beta * Wr);
+            dOr = -(beta * Ui - alfa * Wi);        /* omega_y(+1) */
+            dOi = beta * Ur - alfa * Wr;
+        }
         /* Helmholtz solves with lam^2 (linearbl.f:625-626): particular
-         * phi (bc 0,0), homogeneous phi_h (bc 1,1), omega_y (bc 0,0) */
+         * phi (bc 0,0), homogeneous phi_h (bc 1,1), omega_y (bc 0,0, or
+         * its free-stream value above) */
         tau_solve(n, lam2, fvr, 0., 0., phr, W->gam, W->rho);
         tau_solve(n, lam2, fvi, 0., 0., phi, W->gam, W->rho);
         tau_solve(n, lam2, zero, 1., 1., phh, W->gam, W->rho);
-        tau_solve(n, lam2, fwr, 0., 0., omr, W->gam, W->rho);
-        tau_solve(n, lam2, fwi, 0., 0., omi, W->gam, W->rho);
+        tau_solve(n, lam2, fwr, .5 * dOr, .5 * dOr, omr, W->gam, W->rho);
+        tau_solve(n, lam2, fwi, .5 * dOi, .5 * dOi, omi, W->gam, W->rho);
         /* v from phi with k^2 (linearbl.f:1000-1001); an imposed lower-wall
          * v_w (wbci = 1) enters as the Dirichlet values v(1) = 0,
          * v(-1) = v_w: (v_w/2, -v_w/2) for the even and odd classes
@@ -818,11 +836,12 @@ static void linear_plane(orc_t *o, double *ur, double *ui, double *puw, int z,
             bwr = .5 * o->wvr[(size_t)z * nxh + x];
             bwi = .5 * o->wvi[(size_t)z * nxh + x];
         }
-        tau_solve(n, k2, phr, bwr, -bwr, vr, W->gam, W->rho);
-        tau_solve(n, k2, phi, bwi, -bwi, vi, W->gam, W->rho);
+        tau_solve(n, k2, phr, bwr + .5 * dVr, -bwr + .5 * dVr, vr, W->gam, W->rho);
+        tau_solve(n, k2, phi, bwi + .5 * dVi, -bwi + .5 * dVi, vi, W->gam, W->rho);
         tau_solve(n, k2, phh, 0., 0., vh, W->gam, W->rho);
         /* superpose the homogeneous solution per parity so that dv/dy = 0
-         * at both walls (linearbl.f:1023-1070) */
+         * at both walls (linearbl.f:1023-1070), or dv/dy(+1) = dUr + i dUi
+         * with ibc = 100: half of it in each class */
         dcheb(n, vr, dvr);
         dcheb(n, vi, dvi);
         dcheb(n, vh, dvh);
@@ -834,8 +853,8 @@ static void linear_plane(orc_t *o, double *ur, double *ui, double *puw, int z,
         }
         double cr[2], ci[2];
         for (int s = 0; s < 2; s++) {
-            cr[s] = (0. - Sr[1 - s]) * (1. / Sh[1 - s]);
-            ci[s] = (0. - Si[1 - s]) * (1. / Sh[1 - s]);
+            cr[s] = (.5 * dUr - Sr[1 - s]) * (1. / Sh[1 - s]);
+            ci[s] = (.5 * dUi - Si[1 - s]) * (1. / Sh[1 - s]);
         }
         for (int y = 0; y < n; y++) {
             vr[y] += cr[y & 1] * vh[y];
@@ -872,9 +891,10 @@ static void linear_plane(orc_t *o, double *ur, double *ui, double *puw, int z,
             /* mean flow (linearbl.f:1216-1692, tau branch, ibc = 0) */
             double *u0 = W->v[43], *w0 = W->v[44], *du0 = W->v[45], *dw0 = W->v[46];
             double u0upp = o->u0upp, u0low = o->u0low, w0upp = o->w0upp, w0low = o->w0low;
-            if (o->ibc == 101) {
-                /* with a base flow: U = 1 above (replaced by the free-stream
-                 * condition below), 0 below (bla.f:1360-1365) */
+            if (o->ibc == 100 || o->ibc == 101) {
+                /* with a base flow: U = 1 above (with ibc = 101 replaced by
+                 * the free-stream condition below), 0 below
+                 * (bla.f:1360-1365) */
                 u0upp = 1.;
                 u0low = w0upp = w0low = 0.;
             }
@@ -1557,7 +1577,7 @@ static void les_gu(orc_t *o, const double *ur, const double *ui)
 
 int orc_set_bc(orc_t *o, int ibc, const double *bjr, const double *bji, double du0upp)
 {
-    if (ibc != 0 && ibc != 101)
+    if (ibc != 0 && ibc != 100 && ibc != 101)
         return -1;
     o->ibc = ibc;
     o->du0upp = du0upp;

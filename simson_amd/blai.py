@@ -281,11 +281,11 @@ def parse(text: str) -> BlaParams:
         raise BlaInputError("nst must be 3 or 4")
     if p.xl <= 0:
         raise BlaInputError("xl must be > 0")
-    if p.ibc not in (0, 101):
-        raise BlaInputError("only ibc=0 (no-slip walls) and ibc=101 (free stream) are "
-                            "supported")
-    if p.ibc == 101 and p.cflux:
-        raise BlaInputError("ibc=101 needs cflux = .false.")
+    if p.ibc not in (0, 100, 101):
+        raise BlaInputError("only ibc=0 (no-slip walls), ibc=100 and ibc=101 (free stream: "
+                            "the base flow's values or derivatives) are supported")
+    if p.ibc in (100, 101) and p.cflux:
+        raise BlaInputError(f"ibc={p.ibc} needs cflux = .false.")
     return p
 
 

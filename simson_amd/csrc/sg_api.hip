@@ -405,10 +405,10 @@ ColArgs col_args(simson_gpu_t *g)
     a.mforce = g->mforce_on ? g->mforce : nullptr;
     a.lforce = nullptr;
     a.lscale = 0.;
-    if (g->p.ibc == 101) {
+    if (g->p.ibc == 100 || g->p.ibc == 101) {
         // with a base flow the mean mode takes U = 1 above and no-slip
-        // below before the free-stream condition replaces the upper value
-        // (bla.f:1360-1365)
+        // below (bla.f:1360-1365); with ibc = 101 the free-stream condition
+        // then replaces the upper value
         a.u0upp = 1.;
         a.u0low = a.w0upp = a.w0low = 0.;
     }
@@ -795,7 +795,7 @@ int simson_gpu_init(const simson_params_t *p, simson_gpu_t **out)
         return SIMSON_E_ARG;
     *out = nullptr;
     if ((p->precision != SIMSON_FP64 && p->precision != SIMSON_MIXED) ||
-        (p->ibc != 0 && p->ibc != 101) || (p->ibc == 101 && p->cflux) ||
+        (p->ibc != 0 && p->ibc != 100 && p->ibc != 101) || (p->ibc >= 100 && p->cflux) ||
         p->nx < 4 || p->nz < 4 ||
         p->nx % 4 != 0 || p->nz % 2 != 0 || p->ny < 9 || p->ny % 2 == 0 ||
         p->re <= 0 || p->xl <= 0 || p->zl <= 0)
@@ -885,7 +885,7 @@ int simson_gpu_init(const simson_params_t *p, simson_gpu_t **out)
         for (int i = 0; i < 3; i++)
             if (dalloc(g, &g->lam[i].hb, F) || dalloc(g, &g->lam[i].hc, 4 * ncol))
                 return fail(SIMSON_E_HIP);
-    if (p->ibc == 101 && dalloc(g, &g->bj, 3 * (size_t)nxh))
+    if (p->ibc >= 100 && dalloc(g, &g->bj, 3 * (size_t)nxh))
         return fail(SIMSON_E_HIP);
     // wavenumbers and grid (preprbl.f:34-55)
     std::vector<double> alfa(nxh), beta(nz), eta(ny), deta(ny);
@@ -1345,7 +1345,8 @@ int simson_base_flow(const simson_baseflow_t *b, const double *tab, int ntab, do
 
 int simson_gpu_free_stream(simson_gpu_t *g, const double *bjr, const double *bji, double du0upp)
 {
-    if (!g || g->p.ibc != 101 || (!bjr) != (!bji) || !std::isfinite(du0upp))
+    if (!g || (g->p.ibc != 100 && g->p.ibc != 101) || (!bjr) != (!bji) ||
+        !std::isfinite(du0upp))
         return SIMSON_E_ARG;
     g->du0upp = du0upp;
     g->bj_set = false;

@@ -1,4 +1,4 @@
-// Step 3 of the bla substep (bla/linearbl.f, tau method, ibc = 0 or 101) with one
+// Step 3 of the bla substep (bla/linearbl.f, tau method, ibc = 0, 100 or 101) with one
 // wavefront per (kx,kz) column.
 //
 // Lane L of the wave holds, for parity class c (k = c mod 2), the E class
@@ -395,8 +395,9 @@ constexpr int tau_waves_per_eu()
     return E == 1 ? 3 : 2;
 }
 
-// FS: free stream at y = +1 (ibc = 101) instead of the upper wall
-template <int SW, int E, typename CV, bool FS>
+// FT: the upper boundary y = +1: 0 a wall, 1 the free stream with the base
+// flow's derivatives (ibc = 101), 2 with its values (ibc = 100)
+template <int SW, int E, typename CV, int FT>
 __global__ void __launch_bounds__(256, (tau_waves_per_eu<SW, E>()))
     k_tau_w(ColArgs a, LamFactor lf, LamFactor kf, double an, double bn, double anp1, double bnp1)
 {
@@ -529,7 +530,22 @@ __global__ void __launch_bounds__(256, (tau_waves_per_eu<SW, E>()))
         }
     if (early)
         io.store(P0 + fs, tmp);
-    const double bc0[4][2] = {{0., 0.}, {0., 0.}, {0., 0.}, {0., 0.}};
+    // ibc = 100 (evalbc.f:142-159 with the values bu1jr(:, i, 1)): u_i(+1) =
+    // U_i(+1) of the base flow on the kz = 0 plane (a wall elsewhere), so
+    // v(+1) = V, and by continuity and the definition of omega_y dv/dy(+1)
+    // = -i (alpha U + beta W), omega_y(+1) = i (beta U - alpha W)
+    double2 vup = make_double2(0., 0.), dvp = vup, omp = vup;
+    if constexpr (FT == 2) {
+        if (z == 0 && a.bj && !k0) {
+            const double2 ju = a.bj[x], jw = a.bj[2 * a.nxh + x];
+            vup = a.bj[a.nxh + x];
+            dvp = make_double2(alfa * ju.y + beta * jw.y, -(alfa * ju.x + beta * jw.x));
+            omp = make_double2(-(beta * ju.y - alfa * jw.y), beta * ju.x - alfa * jw.x);
+        }
+    }
+    // boundary rows per class: half the value at +1 in each (zero below)
+    const double bc0[4][2] = {{0., 0.}, {0., 0.}, {.5 * omp.x, .5 * omp.x},
+                              {.5 * omp.y, .5 * omp.y}};
     // phi and omega: Helmholtz solves with lam^2 (linearbl.f:625-626)
     io.load(lf.gp, tmp);
     solve_w<SW, 4, E, KT>(n, lam2, tmp, tw, f, bc0, l);
@@ -554,11 +570,13 @@ __global__ void __launch_bounds__(256, (tau_waves_per_eu<SW, E>()))
             wv.x *= .5 * a.vscale;
             wv.y *= .5 * a.vscale;
         }
-        const double bcv[2][2] = {{wv.x, -wv.x}, {wv.y, -wv.y}};
+        const double bcv[2][2] = {{wv.x + .5 * vup.x, -wv.x + .5 * vup.x},
+                                  {wv.y + .5 * vup.y, -wv.y + .5 * vup.y}};
         solve_w<SW, 2, E, KT>(n, k2, tmp, tw, v, bcv, l);
     }
     // superpose v_h per parity so that dv/dy = 0 at the walls
-    // (linearbl.f:1023-1070): class-c wall slope sum_k k^2 v_k
+    // (linearbl.f:1023-1070), or dv/dy(+1) = dvp with ibc = 100 (half of it
+    // in each class): class-c wall slope sum_k k^2 v_k
     {
         double s[4] = {0., 0., 0., 0.};
 #pragma unroll
@@ -575,15 +593,15 @@ __global__ void __launch_bounds__(256, (tau_waves_per_eu<SW, E>()))
         io.load(lf.vh, tmp);
 #pragma unroll
         for (int c = 0; c < 2; c++) {
-            const double cr = (0. - s[c]) * (1. / shc[c]);
-            const double ci = (0. - s[2 + c]) * (1. / shc[c]);
+            const double cr = (.5 * dvp.x - s[c]) * (1. / shc[c]);
+            const double ci = (.5 * dvp.y - s[2 + c]) * (1. / shc[c]);
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 v[0][c][e] = v[0][c][e] + cr * tmp[c][e].x;
                 v[1][c][e] = v[1][c][e] + ci * tmp[c][e].x;
             }
         }
-        if constexpr (FS) {
+        if constexpr (FT == 1) {
             // free stream at y = +1 (ibc = 101; evald.f, evalbc.f:142-159,
             // solvbc.f, linearbl.f:946-1150): du_i/dy = dU_i/dy of the base
             // flow there (kz = 0 plane, zero elsewhere).  By continuity and
@@ -808,7 +826,7 @@ __global__ void __launch_bounds__(256, (tau_waves_per_eu<SW, E>()))
         if (L == 0)
             *a.px = px;
     }
-    if constexpr (FS) {
+    if constexpr (FT == 1) {
         // free stream above (linearbl.f:1500-1531): add phi_h (the k = 0
         // lam^2 solution with 1 per class) so that du0/dy(+1) = du0upp and
         // dw0/dy(+1) = 0; B = dphi_h/dy at +1
@@ -877,10 +895,13 @@ void launch_seg(const ColArgs &a, LamFactor lf, LamFactor kf, double an, double 
     constexpr int per_block = 4 * (WV / SW);
     dim3 grid((ncol + per_block - 1) / per_block);
     if (a.ibc == 101)
-        hipLaunchKernelGGL((k_tau_w<SW, E, CV, true>), grid, dim3(256), 0, s, a, lf, kf, an, bn,
+        hipLaunchKernelGGL((k_tau_w<SW, E, CV, 1>), grid, dim3(256), 0, s, a, lf, kf, an, bn,
+                           anp1, bnp1);
+    else if (a.ibc == 100)
+        hipLaunchKernelGGL((k_tau_w<SW, E, CV, 2>), grid, dim3(256), 0, s, a, lf, kf, an, bn,
                            anp1, bnp1);
     else
-        hipLaunchKernelGGL((k_tau_w<SW, E, CV, false>), grid, dim3(256), 0, s, a, lf, kf, an, bn,
+        hipLaunchKernelGGL((k_tau_w<SW, E, CV, 0>), grid, dim3(256), 0, s, a, lf, kf, an, bn,
                            anp1, bnp1);
 }
 

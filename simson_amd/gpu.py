@@ -161,7 +161,8 @@ class BlaGPU:
                  precision: int = SIMSON_FP64, ibc: int = 0):
         """precision: SIMSON_FP64, or SIMSON_MIXED (FP32 FFTs, products and
         Chebyshev transforms; FP64 tau solves, mean flow and partial RHS).
-        ibc: 0 (no-slip walls) or 101 (free stream above, see free_stream)."""
+        ibc: 0 (no-slip walls), 100 or 101 (free stream above with the base
+        flow's values or derivatives, see free_stream)."""
         self.nx, self.ny, self.nz = nx, ny, nz
         self.precision = precision
         p = SimsonParams(nx, ny, nz, precision, ibc, int(cflux), re, xl, zl, rot, retau)
@@ -301,7 +302,8 @@ class BlaGPU:
 
     def free_stream(self, bj=None, du0upp: float = 0.0):
         """ibc = 101 data: bj complex [3][nx/2], the base flow's dU_i/dy at
-        y = +1 per kx (kz = 0 plane; None for zero), and du0upp."""
+        y = +1 per kx (kz = 0 plane; None for zero), and du0upp; with
+        ibc = 100 bj holds the base flow's values U_i at y = +1 instead."""
         if bj is None:
             self._check(lib().simson_gpu_free_stream(self.h, None, None, du0upp))
             return

@@ -586,6 +586,44 @@ def locf():
     print("wrote", os.path.join(HERE, "locf_64x33x8.npz"), sorted(res))
 
 
+def blasius100():
+    """The spatial Blasius case a of blasius() with ibc = 100 (the base
+    flow's values, not its derivatives, imposed at y = +1: evalbc.f:142-159
+    with bu1jr(:, i, 1)); 20 steps, saves after 1 and 20.  Keys u1, u20 (the
+    start field is blasius_64x33x8's init)."""
+    from simson_amd import baseflow
+    c = BL
+    if not os.path.exists(BLBIN):
+        subprocess.check_call([os.path.join(REPO, "oracle", "build_ref.sh"),
+                               str(c["nx"]), str(c["ny"]), str(c["nz"]), "1", "no", "0",
+                               os.path.basename(BLBIN)])
+    res = {}
+    work = tempfile.mkdtemp(prefix="simson_golden_bl100.")
+    try:
+        table = baseflow.read_fsc(fsc_dat(work))
+        f = blasius_field(table)
+        f.u = np.load(os.path.join(HERE, "blasius_64x33x8.npz"))["init"]
+        ufile.write_u(os.path.join(work, "init.u"), f)
+        fmax, fstart, fend, frise, ffall = c["fringe"]
+        kw = dict(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=60, dt=c["dt"], nst=3,
+                  xl=c["xl"], cflux=False, retau=0.0, ibc=100, icfl=3, spat=True, fmax=fmax,
+                  fstart=fstart, fend=fend, frise=frise, ffall=ffall,
+                  tsave=[c["dt"]], nmsave=["t1.u"])
+        with open(os.path.join(work, "bla.i"), "w") as fh:
+            fh.write(blai.format_channel(blai.BlaParams(**kw)))
+        with open(os.path.join(work, "out.txt"), "w") as out:
+            subprocess.check_call([BLBIN], cwd=work, stdout=out, stderr=subprocess.STDOUT)
+        log = open(os.path.join(work, "out.txt")).read()
+        if "bla terminated normally" not in log:
+            raise RuntimeError("reference bla (ibc 100) did not terminate normally")
+        res["u1"] = ufile.read_u(os.path.join(work, "t1.u")).u
+        res["u20"] = ufile.read_u(os.path.join(work, "end.u")).u
+    finally:
+        shutil.rmtree(work)
+    np.savez_compressed(os.path.join(HERE, "blasius100_64x33x8.npz"), **res)
+    print("wrote", os.path.join(HERE, "blasius100_64x33x8.npz"), sorted(res))
+
+
 TBL = dict(nx=32, ny=33, nz=16, re=400.0, xl=60.0, yl=20.0, zl=30.0, dt=0.1, kmax=4,
            rms=1e-2, seed=8)
 TBLBIN = os.path.join(REPO, "oracle", "_ref", "bla_32x33x16_serial")
@@ -669,6 +707,8 @@ if __name__ == "__main__":
         tblasius()
     elif len(sys.argv) > 1 and sys.argv[1] == "locf":
         locf()
+    elif len(sys.argv) > 1 and sys.argv[1] == "blasius100":
+        blasius100()
     elif len(sys.argv) > 1 and sys.argv[1] == "vart":
         vart()
     else:

@@ -43,7 +43,7 @@ def test_init_rejects_unsupported_settings():
         gpu.SimsonParams(32, 33, 28, 0, 0, 1, 3000.0, 12.5, 6.28, 0.0, 0.0),   # nzp=42 (7)
         gpu.SimsonParams(32, 385, 32, 0, 0, 1, 3000.0, 12.5, 6.28, 0.0, 0.0),  # ny > 383
         gpu.SimsonParams(32, 33, 32, 0, 101, 1, 3000.0, 12.5, 6.28, 0.0, 0.0),  # 101 + cflux
-        gpu.SimsonParams(32, 33, 32, 0, 100, 0, 3000.0, 12.5, 6.28, 0.0, 0.0),  # ibc=100
+        gpu.SimsonParams(32, 33, 32, 0, 100, 1, 3000.0, 12.5, 6.28, 0.0, 0.0),  # ibc=100, cflux
     ]
     # (lengths without a specialised FFT kernel are checked against the
     # device's LDS limit, so only on a device)

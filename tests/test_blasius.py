@@ -1,4 +1,5 @@
-"""Blasius boundary layers with a free stream above (ibc 101): the spatial
+"""Blasius boundary layers with a free stream above (ibc 101, and ibc 100
+on the first spatial case): the spatial
 layer (bla.i spat, fltype 6) and the temporal one (fltype 3, with the
 parallel-flow forcing of nonlinbl.f:771-795).  The similarity table, the
 base flows with their free-stream data, and the step with the free-stream
@@ -27,7 +28,11 @@ Pins:
   case b with the lower-wall strip; the library on the GPU with its own base
   flow likewise; and the boundary conditions of the result: du_i/dy at
   y = +1 equal to the base flow's (kz = 0) or zero, dv/dy = V' there, and
-  no-slip below.
+  no-slip below;
+* ibc 100 (the base flow's values imposed at y = +1 instead of its
+  derivatives) on case a: tests/golden/blasius100_64x33x8.npz, the oracle
+  and the library against it after 1 and 20 steps, and u_i(+1) = U_i(+1)
+  (kz = 0) or zero in the result.
 
 The temporal layer (tests/golden/tblasius_32x33x16.npz: 32x33x16,
 Re_delta* 400, box 60 x 20 x 30 delta*, 20 steps) pins the parallel profile,
@@ -184,6 +189,72 @@ def test_oracle_matches_reference(golden, case):
     e1, e20 = rel(u1, golden[f"{case}_u1"]), rel(u20, golden[f"{case}_u20"])
     print(f"oracle blasius {case}: 1 step {e1}, 20 steps {e20}")
     assert max(e1) <= 1e-12 and max(e20) <= 1e-12, (e1, e20)
+
+
+def top_values(rb):
+    """Fourier coefficients (over nxp, divided by nxp, kx < nx/2) of the base
+    flow at y = +1: the data of ibc = 100 (bflow.f:80-84, bu1jr(:, i, 1))."""
+    return np.fft.fft(rb[:, 0, :], axis=1)[:, :NX // 2] / NXP
+
+
+@pytest.fixture(scope="module")
+def golden100():
+    return np.load(os.path.join(ROOT, "tests", "golden", "blasius100_64x33x8.npz"))
+
+
+def test_oracle_ibc100_matches_reference(golden, golden100):
+    """ibc = 100 (the base flow's values imposed at y = +1) on case a: the
+    reference run of tests/golden/blasius100_64x33x8.npz, 1 and 20 steps;
+    the result satisfies u_i(+1) = U_i(+1) in the kz = 0 plane, zero
+    elsewhere, and differs from the ibc = 101 run by O(1) in v and w."""
+    from pyoracle import Oracle
+    tab = baseflow.fsc_table()
+    rb, _, _ = baseflow_ref.base_flow(NX, NY, XL, RE, DS, tab.tab, tab.dstar2, BSTART, BLENGTH)
+    bv = top_values(rb)
+    o = Oracle(NX, NY, NZ, RE, XL, ZL, cflux=False, retau=0.0)
+    try:
+        ur, ui, puw = o.new_state(golden["init"])
+        o.px = 0.0
+        o.set_fringe(fringe_ref.fringe_lambda(NXP, XL, *FRINGE), rb)
+        o.set_bc(100, bv, 0.0)
+        o.run(ur, ui, puw, 1, DT)
+        e1 = rel(ur[:3] + 1j * ui[:3], golden100["u1"])
+        o.run(ur, ui, puw, 19, DT)
+        u = ur[:3] + 1j * ui[:3]
+        e20 = rel(u, golden100["u20"])
+    finally:
+        o.close()
+    print(f"oracle ibc 100: 1 step {e1}, 20 steps {e20}")
+    assert max(e1) <= 1e-12 and max(e20) <= 1e-12, (e1, e20)
+    top = u[:, :, 0, :]
+    target = np.zeros_like(top)
+    target[:, 0, :] = bv
+    target[0, 0, 0] = 1.0                   # the mean mode's U = 1 above
+    top = np.delete(top, NZ // 2, axis=1)
+    target = np.delete(target, NZ // 2, axis=1)
+    assert np.abs(top - target).max() < 1e-12
+    assert max(rel(golden100["u20"], golden["a_u20"])[1:]) > 0.1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_gpu_ibc100_matches_reference(golden, golden100, form):
+    from simson_amd.gpu import SIMSON_FP64, SIMSON_MIXED, BlaGPU
+    print(f"[gpu] spatial Blasius a ({form}) {NX}x{NY}x{NZ}, ibc 100", flush=True)
+    tab = baseflow.fsc_table()
+    base, _, _ = baseflow.spatial_base_flow(NX, NY, XL, RE, DS, tab, BSTART, BLENGTH)
+    prec = SIMSON_MIXED if form == "mixed" else SIMSON_FP64
+    tol = 1e-12 if form == "fp64" else 2e-5
+    with BlaGPU(NX, NY, NZ, RE, XL, ZL, cflux=False, retau=0.0, precision=prec, ibc=100) as g:
+        g.fringe(*FRINGE, base=base)
+        g.free_stream(top_values(base), 0.0)
+        g.upload_velocity(golden["init"], 0.0)
+        g.run(1, DT, icfl=3)
+        e1 = rel(g.velocities(), golden100["u1"])
+        g.run(19, DT, icfl=3, first=False)
+        e20 = rel(g.velocities(), golden100["u20"])
+    print(f"    after 1 step {e1}, 20 steps {e20}", flush=True)
+    assert max(e1) <= tol and max(e20) <= tol, (e1, e20)
 
 
 def dtop(col, eta, order=1):
