@@ -208,7 +208,7 @@ program blagpu
   close(10)
   if (pert .or. cim .or. suction .or. isfd /= 0 .or. imhd /= 0 .or. &
        (loctyp /= 0 .and. loctyp /= 3) .or. tripf .or. (wbci /= 0 .and. wbci /= 1) .or. &
-       (ibc /= 0 .and. ibc /= 101) .or. nst /= 3 &
+       (ibc /= 0 .and. ibc /= 100 .and. ibc /= 101) .or. nst /= 3 &
        .or. mwave > 0 .or. npl > 0) then
      write(*,*) 'blagpu: bla.i requests a feature outside the channel step library'
      call stopnow(13)
@@ -274,7 +274,7 @@ program blagpu
   ! the temporal Blasius boundary layer (3)
   if (nfzsym /= 0 .or. .not. (fltype == 1 .or. (.not. spat .and. fltype == 3) .or. &
        (spat .and. (fltype == 4 .or. (fltype >= 6 .and. fltype <= 8))))) call stopnow(13)
-  if (ibc == 101 .and. fltype < 6 .and. fltype /= 3) call stopnow(13)
+  if (ibc >= 100 .and. fltype < 6 .and. fltype /= 3) call stopnow(13)
   if ((fltype >= 6 .or. fltype == 3) .and. (iamp > 0 .or. ixys > 0)) then
      write(*,*) 'blagpu: amplitude and x-y statistics files are written for channels only'
      call stopnow(13)
@@ -372,7 +372,8 @@ program blagpu
         bf%rlam = rlam; bf%spanv = spanv; bf%xsc = 0._dp; bf%u0low = 0._dp; bf%w0low = 0._dp
         allocate(bjr(nx / 2, 3), bji(nx / 2, 3))
         call chk(simson_base_flow(bf, tab, int(size(tab, 2), c_int), base, bjr, bji, du0upp))
-        if (ibc == 101) call chk(simson_gpu_free_stream(g, bjr, bji, du0upp))
+        if (ibc == 100) call top_values()
+        if (ibc >= 100) call chk(simson_gpu_free_stream(g, bjr, bji, du0upp))
      end if
      call chk(simson_gpu_fringe(g, fr, base))
   end if
@@ -386,7 +387,8 @@ program blagpu
      bf%rlam = rlam; bf%spanv = spanv; bf%xsc = 0._dp; bf%u0low = 0._dp; bf%w0low = 0._dp
      allocate(base(3 * nx / 2, nyp, 3), bjr(nx / 2, 3), bji(nx / 2, 3), mforce(nyp, 3))
      call chk(simson_base_flow(bf, tab, int(size(tab, 2), c_int), base, bjr, bji, du0upp))
-     if (ibc == 101) call chk(simson_gpu_free_stream(g, bjr, bji, du0upp))
+     if (ibc == 100) call top_values()
+     if (ibc >= 100) call chk(simson_gpu_free_stream(g, bjr, bji, du0upp))
      call chk(simson_parallel_force(bf, tab, int(size(tab, 2), c_int), cdev, t, mforce))
      call chk(simson_gpu_mean_force(g, mforce))
   end if
@@ -634,6 +636,29 @@ contains
 
   ! rbla.f:24-125: fsc.dat, header (0, ymax, n, m, dstar2, nscalar) and n
   ! rows of eta, f, df, d2f, d3f, g, dg, d2g; m must match the flow's
+  ! ibc = 100: the base flow's values at y = +1 (the first grid row), as
+  ! x Fourier coefficients over the 3/2 grid divided by nxp, kx < nx/2
+  ! (bflow.f:80-84 and 103-111, bu1jr(:, i, 1)), in place of the
+  ! derivatives in bjr, bji
+  subroutine top_values()
+    integer :: i, j, kx, nxpl
+    real(dp) :: ph
+    nxpl = size(base, 1)
+    bjr = 0._dp
+    bji = 0._dp
+    do i = 1, 3
+       do kx = 0, nx / 2 - 1
+          do j = 0, nxpl - 1
+             ph = -2._dp * pi * real(mod(kx * j, nxpl), dp) / real(nxpl, dp)
+             bjr(kx + 1, i) = bjr(kx + 1, i) + base(j + 1, 1, i) * cos(ph)
+             bji(kx + 1, i) = bji(kx + 1, i) + base(j + 1, 1, i) * sin(ph)
+          end do
+          bjr(kx + 1, i) = bjr(kx + 1, i) / real(nxpl, dp)
+          bji(kx + 1, i) = bji(kx + 1, i) / real(nxpl, dp)
+       end do
+    end do
+  end subroutine top_values
+
   subroutine read_fsc()
     real(dp) :: a, b, m
     integer :: n, nsc, r

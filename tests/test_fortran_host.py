@@ -421,16 +421,19 @@ def test_fortran_host_writes_pressure_files():
 
 
 @pytest.mark.gpu
-def test_fortran_host_blasius_matches_reference_bla():
+@pytest.mark.parametrize("ibc", [101, 100])
+def test_fortran_host_blasius_matches_reference_bla(ibc):
     """The host on the spatial Blasius boundary layer (fltype 6, ibc 101,
-    fringe and lower-wall strip: case b of tests/golden/blasius_64x33x8.npz)
-    with bla.i in displacement thicknesses, the .u file in the units of
-    dstar = 2/yl and fsc.dat from the library's similarity solver (equal to
-    the reference fsc's): it writes the fields the reference bla wrote."""
+    fringe and lower-wall strip: case b of tests/golden/blasius_64x33x8.npz;
+    ibc 100 without the strip: tests/golden/blasius100_64x33x8.npz) with
+    bla.i in displacement thicknesses, the .u file in the units of dstar =
+    2/yl and fsc.dat from the library's similarity solver (equal to the
+    reference fsc's): it writes the fields the reference bla wrote."""
     from simson_amd import baseflow
     _need_binary()
-    print("[gpu] blagpu spatial Blasius boundary layer 64x33x8", flush=True)
+    print(f"[gpu] blagpu spatial Blasius boundary layer 64x33x8, ibc {ibc}", flush=True)
     golden = np.load(os.path.join(ROOT, "tests", "golden", "blasius_64x33x8.npz"))
+    g100 = np.load(os.path.join(ROOT, "tests", "golden", "blasius100_64x33x8.npz"))
     ds = 2.0 / 20.0
     work = tempfile.mkdtemp(prefix="blagpu_bl.")
     try:
@@ -439,21 +442,23 @@ def test_fortran_host_blasius_matches_reference_bla():
                          xs=0.0, fltype=6, dstar=ds,
                          extra={"bstart": -35.0 * ds, "blength": 25.0 * ds})
         ufile.write_u(os.path.join(work, "init.u"), f)
+        wall = dict(wbci=1, wamp=0.01, wdamp=1.0, wxstart=-20.0, wxend=-10.0, wxrise=3.0,
+                    wxfall=3.0, wzbet=0.5, wtomeg=0.3) if ibc == 101 else {}
         p = blai.BlaParams(namnin="init.u", namnut="end.u", tmax=1.0e6, maxit=60, dt=0.1,
-                           nst=3, xl=100.0, cflux=False, retau=0.0, ibc=101, icfl=3,
+                           nst=3, xl=100.0, cflux=False, retau=0.0, ibc=ibc, icfl=3,
                            spat=True, fmax=1.0, fstart=-45.0, fend=0.0, frise=15.0,
-                           ffall=10.0, wbci=1, wamp=0.01, wdamp=1.0, wxstart=-20.0,
-                           wxend=-10.0, wxrise=3.0, wxfall=3.0, wzbet=0.5, wtomeg=0.3,
-                           tsave=[0.1], nmsave=["t1.u"])
+                           ffall=10.0, tsave=[0.1], nmsave=["t1.u"], **wall)
         with open(os.path.join(work, "bla.i"), "w") as fh:
             fh.write(blai.format_channel(p))
         r = _run(work)
         assert r.returncode == 0, r.stdout + r.stderr
-        for name, key in (("t1.u", "b_u1"), ("end.u", "b_u20")):
+        ref = {"t1.u": golden["b_u1"], "end.u": golden["b_u20"]} if ibc == 101 else \
+            {"t1.u": g100["u1"], "end.u": g100["u20"]}
+        for name in ("t1.u", "end.u"):
             uf = ufile.read_u(os.path.join(work, name))
             assert uf.fltype == 6 and uf.extra["bstart"] == -35.0 * ds
             u = np.delete(uf.u, 4, axis=1)
-            rr = np.delete(golden[key], 4, axis=1)
+            rr = np.delete(ref[name], 4, axis=1)
             for c in range(3):
                 err = _rel(u[c], rr[c])
                 print(f"    {name} comp {c}: rel L2 {err:.2e}", flush=True)
